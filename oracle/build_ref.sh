@@ -1,0 +1,69 @@
+#!/bin/bash
+# Compiles the reference's FP64 leaf kernels, from the PICSAR sources where
+# they lie, into oracle/_ref/libpxrref.so (test infrastructure only).
+#
+# Nothing from the reference tree is copied into the repository: routines
+# are cut out of the reference files into a temporary directory, compiled
+# there, and only the shared library is kept.  Routines are extracted by
+# name because three of the source files cannot be compiled whole outside
+# the reference's build (they USE mpi and module state, or carry an
+# unrelated routine that flang rejects):
+#   modules.F90           -> modules PICSAR_precision, constants, precomputed
+#   direct_current_deposition_3d.F90 -> depose_jxjyjz_scalar_{1_1_1,2_2_2,3_3_3}
+#   sorting.F90           -> pxr_particle_bin_sorting
+#   submain.F90           -> FD_weights
+# -DLIBRARY selects normalised units (clight = mu0 = eps0 = 1).
+set -euo pipefail
+REF=${PXR_REF:-/root/reference}
+HERE=$(cd "$(dirname "$0")" && pwd)
+OUT="$HERE/_ref"
+FC=${FC:-amdflang}
+FFLAGS="-cpp -DLIBRARY -O2 -fPIC -ffp-contract=off"
+SRC="$REF/src"
+if [ ! -d "$SRC" ]; then
+  echo "build_ref: reference tree not found at $REF" >&2
+  exit 2
+fi
+command -v "$FC" >/dev/null || { echo "build_ref: $FC not found" >&2; exit 2; }
+TMP=$(mktemp -d)
+trap 'rm -rf "$TMP"' EXIT
+
+# extract "MODULE first ... END MODULE last" (inclusive)
+awk 'BEGIN{IGNORECASE=1} /^ *MODULE +PICSAR_precision/{on=1} on{print} /^ *END MODULE +precomputed/{if(on) exit}' \
+  "$SRC/modules/modules.F90" > "$TMP/pxr_consts.F90"
+
+# extract one SUBROUTINE by name (up to its END SUBROUTINE)
+extract_sub() { # file name
+  awk -v name="$2" 'BEGIN{IGNORECASE=1; pat="^ *SUBROUTINE +" name " *\\("}
+    $0 ~ pat {on=1}
+    on {print}
+    on && /^ *END +SUBROUTINE/ {exit}' "$1"
+}
+D="$SRC/particle_deposition/current_deposition/direct/direct_current_deposition_3d.F90"
+: > "$TMP/direct_scalar.F90"
+for o in 1_1_1 2_2_2 3_3_3; do extract_sub "$D" "depose_jxjyjz_scalar_$o" >> "$TMP/direct_scalar.F90"; done
+extract_sub "$SRC/housekeeping/sorting.F90" pxr_particle_bin_sorting > "$TMP/binsort.F90"
+extract_sub "$SRC/submain.F90" FD_weights > "$TMP/fdweights.F90"
+
+cd "$TMP"
+$FC $FFLAGS -c pxr_consts.F90 -o pxr_consts.o
+OBJS="pxr_consts.o"
+for f in field_gathering/energy_conserving/field_gathering_o1_3d.F90 \
+         field_gathering/energy_conserving/field_gathering_o2_3d.F90 \
+         field_gathering/energy_conserving/field_gathering_o3_3d.F90 \
+         particle_pushers/boris_pusher/boris_3d.F90 \
+         particle_pushers/vay_pusher/vay_3d.F90 \
+         particle_deposition/current_deposition/esirkepov/esirkepov_3d.F90 \
+         field_solvers/Maxwell/yee_solver/yee.F90; do
+  o=$(basename "$f" .F90).o
+  $FC $FFLAGS -c "$SRC/$f" -o "$o"
+  OBJS="$OBJS $o"
+done
+for f in direct_scalar binsort fdweights; do
+  $FC $FFLAGS -c "$f.F90" -o "$f.o"
+  OBJS="$OBJS $f.o"
+done
+mkdir -p "$OUT"
+$FC -shared -o "$OUT/libpxrref.so.tmp" $OBJS
+mv "$OUT/libpxrref.so.tmp" "$OUT/libpxrref.so"
+echo "build_ref: wrote $OUT/libpxrref.so"
