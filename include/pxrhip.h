@@ -1,0 +1,225 @@
+/*
+ * pxrhip.h -- C ABI of libpxrhip.so, the MI355X (gfx950) implementation of
+ * PICSAR's per-step PIC hot path.
+ *
+ * Conventions
+ *   - Every array argument is a DEVICE pointer (memory from hipMalloc or a
+ *     torch ROCm tensor), except the small lo/hi bound triples and the FDTD
+ *     stencil coefficient arrays, which are host pointers.
+ *   - Particle arrays are contiguous float64 of length np.
+ *   - Grid arrays are Fortran-ordered float64 with bounds
+ *     (-ngx:nx+ngx, -ngy:ny+ngy, -ngz:nz+ngz): element (i,j,k) is at
+ *     (i+ngx) + NX*((j+ngy) + NY*(k+ngz)), NX = nx+1+2*ngx, NY = ny+1+2*ngy.
+ *   - Integer scalars are int64 (the reference's INTEGER(idp)); logicals are
+ *     int64 0/1 (LOGICAL(lp)); floating scalars are double.
+ *   - Arguments keep the reference routine's order.  Where the reference
+ *     reads a module variable (fields%l_nodalgrid) it is passed explicitly as
+ *     the trailing argument.
+ *   - Outputs accumulate exactly where the reference accumulates (gathered
+ *     fields, deposited currents).
+ *   - Work is enqueued on the library's current stream (pxrhip_set_stream).
+ *     With checks on (the default) a call that can find out-of-range
+ *     particles synchronises that stream and reports them.
+ *   - Every function returns PXRHIP_OK (0) or a negative error code;
+ *     pxrhip_last_error() returns the message of the last failure.
+ */
+#ifndef PXRHIP_H
+#define PXRHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PXRHIP_OK 0
+#define PXRHIP_ERR_ARG (-1)         /* invalid argument (size, order, pointer) */
+#define PXRHIP_ERR_HIP (-2)         /* a HIP runtime call failed */
+#define PXRHIP_ERR_BOUNDS (-3)      /* particles outside the array (skipped) */
+#define PXRHIP_ERR_UNSUPPORTED (-4) /* valid in the reference, not built here */
+#define PXRHIP_ERR_CAPACITY (-5)    /* a buffer is too small for the request */
+
+/* ---- runtime ------------------------------------------------------------ */
+/* No counterpart in the reference (it has no device): library set-up. */
+int pxrhip_init(int device);
+int pxrhip_finalize(void);
+const char *pxrhip_last_error(void);
+int pxrhip_set_stream(void *hip_stream);
+void *pxrhip_get_stream(void);
+int pxrhip_synchronize(void);
+/* on != 0: leaf calls synchronise and report out-of-range particles */
+int pxrhip_set_checks(int on);
+int pxrhip_version(void);
+/* number of particles skipped as out of range since the last reset */
+int pxrhip_bad_count(long long *count, int reset);
+
+/* ---- field gathering ---------------------------------------------------- */
+/* geteb3d_energy_conserving, field_gathering_manager_3d.F90:295-351.
+ * nox = noy = noz in {1,2,3}; adds the Yee-staggered E and B at each
+ * particle into ex..bz.  field_gathe_algo, lvect and ll4symtry select CPU
+ * code variants in the reference and are accepted but do not change the
+ * result.  l_nodal is fields%l_nodalgrid (modules.F90). */
+int pxrhip_geteb3d_energy_conserving(
+    int64_t np, const double *xp, const double *yp, const double *zp,
+    double *ex, double *ey, double *ez, double *bx, double *by, double *bz,
+    double xmin, double ymin, double zmin, double dx, double dy, double dz,
+    int64_t nx, int64_t ny, int64_t nz, int64_t nxguard, int64_t nyguard,
+    int64_t nzguard, int64_t nox, int64_t noy, int64_t noz, const double *exg,
+    const double *eyg, const double *ezg, const double *bxg, const double *byg,
+    const double *bzg, int64_t ll4symtry, int64_t l_lower_order_in_v,
+    int64_t lvect, int64_t field_gathe_algo, int64_t l_nodal);
+
+/* ---- particle pushers --------------------------------------------------- */
+/* pxr_boris_push_u_3d, boris_3d.F90:51-141 */
+int pxrhip_boris_push_u_3d(int64_t np, double *uxp, double *uyp, double *uzp,
+                           double *gaminv, const double *ex, const double *ey,
+                           const double *ez, const double *bx, const double *by,
+                           const double *bz, double q, double m, double dt);
+/* pxr_ebcancelpush3d, vay_3d.F90:56-241 (which = 0 full, 1/2 half pushes) */
+int pxrhip_ebcancelpush3d(int64_t np, double *uxp, double *uyp, double *uzp,
+                          double *gi, const double *ex, const double *ey,
+                          const double *ez, const double *bx, const double *by,
+                          const double *bz, double q, double m, double dt,
+                          int64_t which);
+/* pxr_pushxyz, boris_3d.F90:849-893 */
+int pxrhip_pushxyz(int64_t np, double *xp, double *yp, double *zp,
+                   const double *uxp, const double *uyp, const double *uzp,
+                   const double *gaminv, double dt);
+/* pxr_set_gamma, boris_3d.F90:1059-1107 */
+int pxrhip_set_gamma(int64_t np, const double *uxp, const double *uyp,
+                     const double *uzp, double *gaminv);
+
+/* ---- current deposition ------------------------------------------------- */
+/* depose_jxjyjz, current_deposition_manager_3d.F90:47-67 ->
+ * depose_jxjyjz_generic (:106-222).  current_depo_algo: 0 Esirkepov
+ * (esirkepov_3d.F90:82/1734/3263), 1 arbitrary-order Esirkepov (orders 1..3
+ * here), 2 and 3 direct (direct_current_deposition_3d.F90:99/879/1992).
+ * nox = noy = noz in {1,2,3}.  Adds into jx, jy, jz. */
+int pxrhip_depose_jxjyjz(double *jx, double *jy, double *jz, int64_t np,
+                         const double *xp, const double *yp, const double *zp,
+                         const double *uxp, const double *uyp,
+                         const double *uzp, const double *gaminv,
+                         const double *w, double q, double xmin, double ymin,
+                         double zmin, double dt, double dx, double dy,
+                         double dz, int64_t nx, int64_t ny, int64_t nz,
+                         int64_t nxguard, int64_t nyguard, int64_t nzguard,
+                         int64_t nox, int64_t noy, int64_t noz,
+                         int64_t current_depo_algo, int64_t l_nodal);
+
+/* ---- FDTD ---------------------------------------------------------------- */
+/* pxrpush_em3d_evec, yee.F90:637-750.  lo/hi triples are host int64[3];
+ * each array carries its own bounds, as in the reference. */
+int pxrhip_push_em3d_evec(
+    const int64_t *xlo, const int64_t *xhi, const int64_t *ylo,
+    const int64_t *yhi, const int64_t *zlo, const int64_t *zhi, double *ex,
+    const int64_t *exlo, const int64_t *exhi, double *ey, const int64_t *eylo,
+    const int64_t *eyhi, double *ez, const int64_t *ezlo, const int64_t *ezhi,
+    const double *bx, const int64_t *bxlo, const int64_t *bxhi,
+    const double *by, const int64_t *bylo, const int64_t *byhi,
+    const double *bz, const int64_t *bzlo, const int64_t *bzhi,
+    const double *jx, const int64_t *jxlo, const int64_t *jxhi,
+    const double *jy, const int64_t *jylo, const int64_t *jyhi,
+    const double *jz, const int64_t *jzlo, const int64_t *jzhi, double mudt,
+    double dtsdx, double dtsdy, double dtsdz);
+/* pxrpush_em3d_bvec, yee.F90:1258-1370 */
+int pxrhip_push_em3d_bvec(
+    const int64_t *xlo, const int64_t *xhi, const int64_t *ylo,
+    const int64_t *yhi, const int64_t *zlo, const int64_t *zhi,
+    const double *ex, const int64_t *exlo, const int64_t *exhi,
+    const double *ey, const int64_t *eylo, const int64_t *eyhi,
+    const double *ez, const int64_t *ezlo, const int64_t *ezhi, double *bx,
+    const int64_t *bxlo, const int64_t *bxhi, double *by, const int64_t *bylo,
+    const int64_t *byhi, double *bz, const int64_t *bzlo, const int64_t *bzhi,
+    double dtsdx, double dtsdy, double dtsdz);
+/* pxrpush_em3d_evec_norder, yee.F90:46-133; dtsdx..dtsdz are host arrays of
+ * norder/2 coefficients (norder even, <= 32). */
+int pxrhip_push_em3d_evec_norder(
+    double *ex, double *ey, double *ez, const double *bx, const double *by,
+    const double *bz, const double *jx, const double *jy, const double *jz,
+    double mudt, const double *dtsdx, const double *dtsdy,
+    const double *dtsdz, int64_t nx, int64_t ny, int64_t nz, int64_t norderx,
+    int64_t nordery, int64_t norderz, int64_t nxguard, int64_t nyguard,
+    int64_t nzguard, int64_t nxs, int64_t nys, int64_t nzs,
+    int64_t l_nodalgrid);
+/* pxrpush_em3d_bvec_norder, yee.F90:752-833 */
+int pxrhip_push_em3d_bvec_norder(
+    const double *ex, const double *ey, const double *ez, double *bx,
+    double *by, double *bz, const double *dtsdx, const double *dtsdy,
+    const double *dtsdz, int64_t nx, int64_t ny, int64_t nz, int64_t norderx,
+    int64_t nordery, int64_t norderz, int64_t nxguard, int64_t nyguard,
+    int64_t nzguard, int64_t nxs, int64_t nys, int64_t nzs,
+    int64_t l_nodalgrid);
+
+/* ---- guard handling on one periodic rank -------------------------------- */
+/* field_bc -> exchange_mpi_3d_grid_array_with_guards
+ * (field_boundaries.F90:81-260) when every neighbour is the rank itself. */
+int pxrhip_field_bc_periodic(double *field, int64_t nx, int64_t ny, int64_t nz,
+                             int64_t nxg, int64_t nyg, int64_t nzg);
+/* summation_bcs (field_boundaries.F90:403-500) on one periodic rank. */
+int pxrhip_current_bc_periodic(double *array, int64_t nx, int64_t ny,
+                               int64_t nz, int64_t nxg, int64_t nyg,
+                               int64_t nzg);
+
+/* ---- sorting ------------------------------------------------------------- */
+/* pxr_particle_bin_sorting, sorting.F90:247-361: stable counting sort of
+ * np2 particles by sorting-grid cell; pid points at the first pid column
+ * (np2 values), the only one the reference permutes.  Keys outside
+ * [1, nx3*ny3*nz3] are reported (PXRHIP_ERR_BOUNDS) and nothing is moved. */
+int pxrhip_particle_bin_sorting(int64_t np2, double *xp, double *yp,
+                                double *zp, double *ux, double *uy, double *uz,
+                                double *gam, double *pid, int64_t wpid,
+                                double xmin2, double ymin2, double zmin2,
+                                double xmax2, double ymax2, double zmax2,
+                                double dxf, double dyf, double dzf);
+
+/* ---- tiled step (the managers' work on device-resident tiles) ----------- */
+/* Geometry shared by the tile-level entry points: one periodic rank with
+ * nx*ny*nz cells, field and current arrays of identical shape with guards
+ * ngx..ngz, split into ntx*nty*ntz tiles as set_tile_split_for_species
+ * does (tiling.F90:94-271): n0 = nx_grid/ntx nodes per tile, the last tile
+ * takes the remainder.  Particles of one species are stored SoA, packed by
+ * tile: tile t holds [tile_off[t], tile_off[t] + tile_np[t]). */
+typedef struct {
+  int64_t nx, ny, nz;    /* cells */
+  int64_t ngx, ngy, ngz; /* guard nodes of every grid array */
+  int64_t ntx, nty, ntz; /* tiles */
+  double xmin, ymin, zmin, dx, dy, dz;
+} pxrhip_grid;
+
+typedef struct {
+  double *x, *y, *z, *ux, *uy, *uz, *gaminv, *w; /* SoA particle arrays */
+  int64_t *tile_off; /* device, ntiles+1 entries (packed offsets) */
+  int64_t *tile_np;  /* device, ntiles entries */
+  int64_t capacity;  /* allocated length of each particle array */
+  double q, m;
+} pxrhip_species;
+
+/* One fused pass over every particle of one species:
+ * energy-conserving gather (field_gathering_plus_particle_pusher_*,
+ * particle_pusher_manager_3d.F90:431-672) -> pusher (0 Boris + set_gamma,
+ * 1 Vay) -> pxr_pushxyz -> current deposition of the new positions
+ * (current_deposition_manager_3d.F90:224-505), accumulated into jx..jz.
+ * The stencil blocks of each chunk of particles are staged in LDS.
+ * depo_algo: 0 Esirkepov, 3 direct.  chunk: particles per work item
+ * (0 = library default). */
+int pxrhip_tiles_gather_push_deposit(
+    const pxrhip_grid *grid, const pxrhip_species *sp, const double *exg,
+    const double *eyg, const double *ezg, const double *bxg, const double *byg,
+    const double *bzg, double *jx, double *jy, double *jz, double dt,
+    int64_t order, int64_t l_lower_order_in_v, int64_t pusher,
+    int64_t depo_algo, int64_t chunk);
+
+/* particle_bcs (periodic wrap, particle_boundaries.F90:350-426) followed by
+ * particle_sorting_sub (sorting.F90:106-215): every particle is wrapped into
+ * the periodic box, moved to the tile that now holds it and the tile is
+ * stably sorted by cell of the sorting grid (cell size sdx..sdz, origin
+ * shifted by shx..shz from the tile's lower edge).  The result is written
+ * to dst (same capacity as src); dst's tile_off/tile_np are filled in. */
+int pxrhip_tiles_rebin_sort(const pxrhip_grid *grid, const pxrhip_species *src,
+                            pxrhip_species *dst, double sdx, double sdy,
+                            double sdz, double shx, double shy, double shz);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PXRHIP_H */

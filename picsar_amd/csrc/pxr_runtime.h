@@ -1,0 +1,55 @@
+// pxr_runtime.h -- host-side runtime shared by the C-ABI entry points:
+// current stream, error reporting, the device error counter and a small
+// cache of library-owned scratch buffers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pxrhip.h"
+
+namespace pxr {
+
+struct Runtime {
+  int device = -1;
+  hipStream_t stream = nullptr;
+  bool checks = true;
+  unsigned long long *d_bad = nullptr;  // device counter of skipped particles
+  int num_cus = 256;
+};
+
+Runtime &rt();
+int set_error(int code, const char *fmt, ...);
+int ensure_init();
+// library-owned scratch buffer number `slot`, grown to at least `bytes`
+void *scratch(int slot, size_t bytes);
+// read the bad-particle counter (synchronises the stream), reset it to 0
+int take_bad(unsigned long long *n);
+
+inline unsigned grid1d(int64_t n, int block) {
+  int64_t b = (n + block - 1) / block;
+  return (unsigned)(b < 1 ? 1 : b);
+}
+
+}  // namespace pxr
+
+#define PXR_HIP(call)                                                          \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      return pxr::set_error(PXRHIP_ERR_HIP, "%s failed: %s", #call,           \
+                            hipGetErrorString(e_));                            \
+  } while (0)
+
+#define PXR_LAUNCHED()                                                         \
+  do {                                                                         \
+    hipError_t e_ = hipGetLastError();                                         \
+    if (e_ != hipSuccess)                                                      \
+      return pxr::set_error(PXRHIP_ERR_HIP, "kernel launch failed: %s",        \
+                            hipGetErrorString(e_));                            \
+  } while (0)
+
+#define PXR_INIT()                                                             \
+  do {                                                                         \
+    int r_ = pxr::ensure_init();                                               \
+    if (r_ != PXRHIP_OK) return r_;                                            \
+  } while (0)

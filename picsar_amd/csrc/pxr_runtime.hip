@@ -1,0 +1,134 @@
+// pxr_runtime.hip -- library set-up, stream and error handling.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <vector>
+
+#include "pxr_runtime.h"
+
+namespace pxr {
+
+static Runtime g_rt;
+static thread_local char g_err[1024] = "";
+
+struct Scratch {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+static std::vector<Scratch> g_scratch(16);
+
+Runtime &rt() { return g_rt; }
+
+int set_error(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+int ensure_init() {
+  if (g_rt.device >= 0) return PXRHIP_OK;
+  return pxrhip_init(-1);
+}
+
+void *scratch(int slot, size_t bytes) {
+  if (slot < 0 || slot >= (int)g_scratch.size()) return nullptr;
+  Scratch &s = g_scratch[slot];
+  if (s.bytes >= bytes && s.p) return s.p;
+  if (s.p) {
+    hipStreamSynchronize(g_rt.stream);
+    hipFree(s.p);
+    s.p = nullptr;
+    s.bytes = 0;
+  }
+  size_t want = bytes < 4096 ? 4096 : bytes;
+  if (hipMalloc(&s.p, want) != hipSuccess) {
+    s.p = nullptr;
+    return nullptr;
+  }
+  s.bytes = want;
+  return s.p;
+}
+
+int take_bad(unsigned long long *n) {
+  unsigned long long h = 0;
+  PXR_HIP(hipMemcpyAsync(&h, g_rt.d_bad, sizeof(h), hipMemcpyDeviceToHost, g_rt.stream));
+  PXR_HIP(hipStreamSynchronize(g_rt.stream));
+  if (h) PXR_HIP(hipMemsetAsync(g_rt.d_bad, 0, sizeof(h), g_rt.stream));
+  *n = h;
+  return PXRHIP_OK;
+}
+
+}  // namespace pxr
+
+using namespace pxr;
+
+extern "C" {
+
+int pxrhip_init(int device) {
+  if (g_rt.device >= 0 && (device < 0 || device == g_rt.device)) return PXRHIP_OK;
+  int dev = device;
+  if (dev < 0) PXR_HIP(hipGetDevice(&dev));
+  PXR_HIP(hipSetDevice(dev));
+  hipDeviceProp_t prop;
+  PXR_HIP(hipGetDeviceProperties(&prop, dev));
+  if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return set_error(PXRHIP_ERR_UNSUPPORTED, "device %d is %s; this library is built for gfx950",
+                     dev, prop.gcnArchName);
+  g_rt.num_cus = prop.multiProcessorCount;
+  if (!g_rt.d_bad) {
+    PXR_HIP(hipMalloc(&g_rt.d_bad, 2 * sizeof(unsigned long long)));
+    PXR_HIP(hipMemset(g_rt.d_bad, 0, 2 * sizeof(unsigned long long)));
+  }
+  g_rt.device = dev;
+  return PXRHIP_OK;
+}
+
+int pxrhip_finalize(void) {
+  if (g_rt.device < 0) return PXRHIP_OK;
+  hipDeviceSynchronize();
+  for (auto &s : g_scratch) {
+    if (s.p) hipFree(s.p);
+    s.p = nullptr;
+    s.bytes = 0;
+  }
+  if (g_rt.d_bad) hipFree(g_rt.d_bad);
+  g_rt.d_bad = nullptr;
+  g_rt.device = -1;
+  return PXRHIP_OK;
+}
+
+const char *pxrhip_last_error(void) { return g_err; }
+
+int pxrhip_set_stream(void *s) {
+  g_rt.stream = (hipStream_t)s;
+  return PXRHIP_OK;
+}
+
+void *pxrhip_get_stream(void) { return (void *)g_rt.stream; }
+
+int pxrhip_synchronize(void) {
+  PXR_HIP(hipStreamSynchronize(g_rt.stream));
+  return PXRHIP_OK;
+}
+
+int pxrhip_set_checks(int on) {
+  g_rt.checks = on != 0;
+  return PXRHIP_OK;
+}
+
+int pxrhip_version(void) { return 100; }
+
+int pxrhip_bad_count(long long *count, int reset) {
+  PXR_INIT();
+  unsigned long long h = 0;
+  PXR_HIP(hipMemcpyAsync(&h, g_rt.d_bad, sizeof(h), hipMemcpyDeviceToHost, g_rt.stream));
+  PXR_HIP(hipStreamSynchronize(g_rt.stream));
+  if (reset) PXR_HIP(hipMemsetAsync(g_rt.d_bad, 0, sizeof(h), g_rt.stream));
+  if (count) *count = (long long)h;
+  return PXRHIP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,367 @@
+"""Drop-in mirror of PICSAR's Python module for the hot path.
+
+PICSAR exposes its Fortran subroutines through Forthon as
+``picsar_python.picsarpy.picsar`` (examples/example_scripts_python/test.py:4,11).
+This module exposes the same names with the same argument order, backed by
+the HIP kernels of libpxrhip.so:
+
+    from picsar_amd.picsarpy import picsar as pxr
+    pxr.geteb3d_energy_conserving(np, xp, yp, zp, ex, ey, ez, bx, by, bz,
+                                  xmin, ymin, zmin, dx, dy, dz, nx, ny, nz,
+                                  nxguard, nyguard, nzguard, nox, noy, noz,
+                                  exg, eyg, ezg, bxg, byg, bzg,
+                                  ll4symtry, l_lower_order_in_v, lvect,
+                                  field_gathe_algo)
+
+Arrays may be torch ROCm tensors (used in place, zero copy) or numpy arrays
+(copied to the GPU for the call and written back, like Forthon's in-place
+update).  Grid arrays are Fortran-ordered with bounds (-ng:n+ng) on each
+axis: a numpy array with order='F' and shape (nx+1+2ng, ny+1+2ng, nz+1+2ng),
+or a tensor with that shape and strides (1, NX, NX*NY) (see
+``picsar_amd.arrays.grid``), or any contiguous tensor of that many elements.
+
+Module variables the reference kernels read (fields%l_nodalgrid) are
+attributes of ``picsar``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import PxrHipError  # noqa: F401
+
+_i64arr3 = C.c_int64 * 3
+
+
+def _torch():
+    import torch
+    return torch
+
+
+class _Staged:
+    """A numpy argument validated but not yet copied to the device."""
+
+    def __init__(self, a, order, write):
+        self.a, self.order, self.write = a, order, write
+        self.t = None
+
+
+class _Call:
+    """Collects the array arguments of one call.  Every argument is checked
+    before anything is copied; numpy arrays are then staged on the device
+    and the modified ones written back after the call."""
+
+    def __init__(self):
+        self.staged = []
+
+    def _tensor_ptr(self, t, need, name, fortran_shape=None):
+        torch = _torch()
+        if t.dtype != torch.float64:
+            raise TypeError(f"{name}: expected float64, got {t.dtype}")
+        if not t.is_cuda:
+            raise TypeError(f"{name}: tensor must live on the GPU")
+        if t.numel() < need:
+            raise ValueError(f"{name}: {t.numel()} elements, need {need}")
+        if fortran_shape is not None and t.dim() == 3:
+            ok_f = tuple(t.shape) == tuple(fortran_shape) and t.stride() == (
+                1, fortran_shape[0], fortran_shape[0] * fortran_shape[1])
+            ok_c = t.is_contiguous() and tuple(t.shape) == tuple(reversed(fortran_shape))
+            if not (ok_f or ok_c):
+                raise ValueError(f"{name}: grid tensor must be Fortran-ordered "
+                                 f"{tuple(fortran_shape)}")
+        elif not t.is_contiguous():
+            raise ValueError(f"{name}: tensor must be contiguous")
+        return t.data_ptr()
+
+    def arr(self, a, need, name, write=False, fortran_shape=None):
+        if isinstance(a, np.ndarray):
+            if a.dtype != np.float64:
+                raise TypeError(f"{name}: expected float64, got {a.dtype}")
+            if a.size < need:
+                raise ValueError(f"{name}: {a.size} elements, need {need}")
+            order = "F" if (fortran_shape is not None and a.ndim == 3) else "C"
+            if order == "F" and not a.flags.f_contiguous:
+                raise ValueError(f"{name}: grid array must be Fortran-ordered")
+            h = _Staged(a, order, write)
+            self.staged.append(h)
+            return h
+        torch = _torch()
+        if isinstance(a, torch.Tensor):
+            return self._tensor_ptr(a, need, name, fortran_shape)
+        raise TypeError(f"{name}: expected a torch tensor or numpy array")
+
+    def ptrs(self, handles):
+        """Copy the staged numpy arrays to the device; device pointers."""
+        torch = _torch()
+        out = []
+        for h in handles:
+            if isinstance(h, _Staged):
+                if h.t is None:
+                    flat = h.a.ravel(order="F" if h.order == "F" else "K")
+                    h.t = torch.from_numpy(np.ascontiguousarray(flat)).to("cuda")
+                out.append(h.t.data_ptr())
+            else:
+                out.append(h)
+        return out
+
+    def finish(self):
+        for h in self.staged:
+            if h.write and h.t is not None:
+                v = h.t.cpu().numpy()
+                if h.order == "F":
+                    h.a[...] = v.reshape(h.a.shape, order="F")
+                else:
+                    h.a.reshape(-1)[...] = v
+        self.staged.clear()
+
+
+def _stream():
+    torch = _torch()
+    s = torch.cuda.current_stream().cuda_stream
+    _lib.call("pxrhip_set_stream", C.c_void_p(s))
+
+
+def _gshape(nx, ny, nz, gx, gy, gz):
+    return (int(nx) + 1 + 2 * int(gx), int(ny) + 1 + 2 * int(gy), int(nz) + 1 + 2 * int(gz))
+
+
+class _Picsar:
+    """The ``picsar`` namespace: subroutines and module variables."""
+
+    def __init__(self):
+        self.l_nodalgrid = False  # fields%l_nodalgrid (modules.F90)
+
+    # ------------------------------------------------------------------
+    def set_checks(self, on: bool):
+        """Leaf calls synchronise and raise on out-of-range particles (default on)."""
+        _lib.call("pxrhip_set_checks", 1 if on else 0)
+
+    # ------------------------------------------------------------------
+    # field_gathering_manager_3d.F90:295
+    def geteb3d_energy_conserving(self, np_, xp, yp, zp, ex, ey, ez, bx, by, bz, xmin, ymin,
+                                  zmin, dx, dy, dz, nx, ny, nz, nxguard, nyguard, nzguard, nox,
+                                  noy, noz, exg, eyg, ezg, bxg, byg, bzg, ll4symtry=False,
+                                  l_lower_order_in_v=True, lvect=8, field_gathe_algo=0):
+        n = int(np_)
+        if n < 0:
+            raise ValueError("np < 0")
+        shp = _gshape(nx, ny, nz, nxguard, nyguard, nzguard)
+        ng = shp[0] * shp[1] * shp[2]
+        c = _Call()
+        P = [c.arr(a, n, nm) for a, nm in ((xp, "xp"), (yp, "yp"), (zp, "zp"))]
+        O = [c.arr(a, n, nm, write=True) for a, nm in
+             ((ex, "ex"), (ey, "ey"), (ez, "ez"), (bx, "bx"), (by, "by"), (bz, "bz"))]
+        G = [c.arr(a, ng, nm, fortran_shape=shp) for a, nm in
+             ((exg, "exg"), (eyg, "eyg"), (ezg, "ezg"), (bxg, "bxg"), (byg, "byg"),
+              (bzg, "bzg"))]
+        P, O, G = c.ptrs(P), c.ptrs(O), c.ptrs(G)
+        _stream()
+        _lib.call("pxrhip_geteb3d_energy_conserving", n, *P, *O, float(xmin), float(ymin),
+                  float(zmin), float(dx), float(dy), float(dz), int(nx), int(ny), int(nz),
+                  int(nxguard), int(nyguard), int(nzguard), int(nox), int(noy), int(noz),
+                  *G, int(bool(ll4symtry)), int(bool(l_lower_order_in_v)), int(lvect),
+                  int(field_gathe_algo), int(bool(self.l_nodalgrid)))
+        c.finish()
+
+    # boris_3d.F90:51
+    def pxr_boris_push_u_3d(self, np_, uxp, uyp, uzp, gaminv, ex, ey, ez, bx, by, bz, q, m, dt):
+        n = int(np_)
+        c = _Call()
+        U = [c.arr(a, n, nm, write=True) for a, nm in
+             ((uxp, "uxp"), (uyp, "uyp"), (uzp, "uzp"), (gaminv, "gaminv"))]
+        E = [c.arr(a, n, nm) for a, nm in
+             ((ex, "ex"), (ey, "ey"), (ez, "ez"), (bx, "bx"), (by, "by"), (bz, "bz"))]
+        U, E = c.ptrs(U), c.ptrs(E)
+        _stream()
+        _lib.call("pxrhip_boris_push_u_3d", n, *U, *E, float(q), float(m), float(dt))
+        c.finish()
+
+    # vay_3d.F90:56
+    def pxr_ebcancelpush3d(self, np_, uxp, uyp, uzp, gi, ex, ey, ez, bx, by, bz, q, m, dt,
+                           which=0):
+        n = int(np_)
+        c = _Call()
+        U = [c.arr(a, n, nm, write=True) for a, nm in
+             ((uxp, "uxp"), (uyp, "uyp"), (uzp, "uzp"), (gi, "gi"))]
+        E = [c.arr(a, n, nm) for a, nm in
+             ((ex, "ex"), (ey, "ey"), (ez, "ez"), (bx, "bx"), (by, "by"), (bz, "bz"))]
+        U, E = c.ptrs(U), c.ptrs(E)
+        _stream()
+        _lib.call("pxrhip_ebcancelpush3d", n, *U, *E, float(q), float(m), float(dt), int(which))
+        c.finish()
+
+    # boris_3d.F90:849
+    def pxr_pushxyz(self, np_, xp, yp, zp, uxp, uyp, uzp, gaminv, dt):
+        n = int(np_)
+        c = _Call()
+        X = [c.arr(a, n, nm, write=True) for a, nm in ((xp, "xp"), (yp, "yp"), (zp, "zp"))]
+        U = [c.arr(a, n, nm) for a, nm in
+             ((uxp, "uxp"), (uyp, "uyp"), (uzp, "uzp"), (gaminv, "gaminv"))]
+        X, U = c.ptrs(X), c.ptrs(U)
+        _stream()
+        _lib.call("pxrhip_pushxyz", n, *X, *U, float(dt))
+        c.finish()
+
+    # boris_3d.F90:1059
+    def pxr_set_gamma(self, np_, uxp, uyp, uzp, gaminv):
+        n = int(np_)
+        c = _Call()
+        U = [c.arr(a, n, nm) for a, nm in ((uxp, "uxp"), (uyp, "uyp"), (uzp, "uzp"))]
+        g = c.arr(gaminv, n, "gaminv", write=True)
+        U, (g,) = c.ptrs(U), c.ptrs([g])
+        _stream()
+        _lib.call("pxrhip_set_gamma", n, *U, g)
+        c.finish()
+
+    # current_deposition_manager_3d.F90:47
+    def depose_jxjyjz(self, jx, jy, jz, np_, xp, yp, zp, uxp, uyp, uzp, gaminv, w, q, xmin, ymin,
+                      zmin, dt, dx, dy, dz, nx, ny, nz, nxguard, nyguard, nzguard, nox, noy, noz,
+                      current_depo_algo=0):
+        n = int(np_)
+        shp = _gshape(nx, ny, nz, nxguard, nyguard, nzguard)
+        ng = shp[0] * shp[1] * shp[2]
+        c = _Call()
+        J = [c.arr(a, ng, nm, write=True, fortran_shape=shp) for a, nm in
+             ((jx, "jx"), (jy, "jy"), (jz, "jz"))]
+        P = [c.arr(a, n, nm) for a, nm in
+             ((xp, "xp"), (yp, "yp"), (zp, "zp"), (uxp, "uxp"), (uyp, "uyp"), (uzp, "uzp"),
+              (gaminv, "gaminv"), (w, "w"))]
+        J, P = c.ptrs(J), c.ptrs(P)
+        _stream()
+        _lib.call("pxrhip_depose_jxjyjz", *J, n, *P, float(q), float(xmin), float(ymin),
+                  float(zmin), float(dt), float(dx), float(dy), float(dz), int(nx), int(ny),
+                  int(nz), int(nxguard), int(nyguard), int(nzguard), int(nox), int(noy), int(noz),
+                  int(current_depo_algo), int(bool(self.l_nodalgrid)))
+        c.finish()
+
+    # yee.F90:637
+    def pxrpush_em3d_evec(self, xlo, xhi, ylo, yhi, zlo, zhi, ex, exlo, exhi, ey, eylo, eyhi, ez,
+                          ezlo, ezhi, bx, bxlo, bxhi, by, bylo, byhi, bz, bzlo, bzhi, jx, jxlo,
+                          jxhi, jy, jylo, jyhi, jz, jzlo, jzhi, mudt, dtsdx, dtsdy, dtsdz):
+        c = _Call()
+        B = [_i64arr3(*map(int, v)) for v in (xlo, xhi, ylo, yhi, zlo, zhi)]
+        args = []
+        for a, lo, hi, nm, wr in ((ex, exlo, exhi, "ex", True), (ey, eylo, eyhi, "ey", True),
+                                  (ez, ezlo, ezhi, "ez", True), (bx, bxlo, bxhi, "bx", False),
+                                  (by, bylo, byhi, "by", False), (bz, bzlo, bzhi, "bz", False),
+                                  (jx, jxlo, jxhi, "jx", False), (jy, jylo, jyhi, "jy", False),
+                                  (jz, jzlo, jzhi, "jz", False)):
+            shp = tuple(int(h) - int(l) + 1 for l, h in zip(lo, hi))
+            lo_, hi_ = _i64arr3(*map(int, lo)), _i64arr3(*map(int, hi))
+            B += [lo_, hi_]
+            args += [c.arr(a, shp[0] * shp[1] * shp[2], nm, write=wr, fortran_shape=shp), lo_, hi_]
+        args = c.ptrs(args)
+        _stream()
+        _lib.call("pxrhip_push_em3d_evec", *B[:6], *args, float(mudt), float(dtsdx),
+                  float(dtsdy), float(dtsdz))
+        c.finish()
+
+    # yee.F90:1258
+    def pxrpush_em3d_bvec(self, xlo, xhi, ylo, yhi, zlo, zhi, ex, exlo, exhi, ey, eylo, eyhi, ez,
+                          ezlo, ezhi, bx, bxlo, bxhi, by, bylo, byhi, bz, bzlo, bzhi, dtsdx, dtsdy,
+                          dtsdz):
+        c = _Call()
+        B = [_i64arr3(*map(int, v)) for v in (xlo, xhi, ylo, yhi, zlo, zhi)]
+        args = []
+        for a, lo, hi, nm, wr in ((ex, exlo, exhi, "ex", False), (ey, eylo, eyhi, "ey", False),
+                                  (ez, ezlo, ezhi, "ez", False), (bx, bxlo, bxhi, "bx", True),
+                                  (by, bylo, byhi, "by", True), (bz, bzlo, bzhi, "bz", True)):
+            shp = tuple(int(h) - int(l) + 1 for l, h in zip(lo, hi))
+            lo_, hi_ = _i64arr3(*map(int, lo)), _i64arr3(*map(int, hi))
+            B += [lo_, hi_]
+            args += [c.arr(a, shp[0] * shp[1] * shp[2], nm, write=wr, fortran_shape=shp), lo_, hi_]
+        args = c.ptrs(args)
+        _stream()
+        _lib.call("pxrhip_push_em3d_bvec", *B[:6], *args, float(dtsdx), float(dtsdy),
+                  float(dtsdz))
+        c.finish()
+
+    @staticmethod
+    def _coefs(v, norder):
+        a = np.ascontiguousarray(np.asarray(v, dtype=np.float64).ravel())
+        if a.size < int(norder) // 2:
+            raise ValueError("stencil coefficient array shorter than norder/2")
+        return a
+
+    # yee.F90:46
+    def pxrpush_em3d_evec_norder(self, ex, ey, ez, bx, by, bz, jx, jy, jz, mudt, dtsdx, dtsdy,
+                                 dtsdz, nx, ny, nz, norderx, nordery, norderz, nxguard, nyguard,
+                                 nzguard, nxs, nys, nzs, l_nodalgrid=None):
+        shp = _gshape(nx, ny, nz, nxguard, nyguard, nzguard)
+        ng = shp[0] * shp[1] * shp[2]
+        c = _Call()
+        F = [c.arr(a, ng, nm, write=(i < 3), fortran_shape=shp) for i, (a, nm) in enumerate(
+            ((ex, "ex"), (ey, "ey"), (ez, "ez"), (bx, "bx"), (by, "by"), (bz, "bz"), (jx, "jx"),
+             (jy, "jy"), (jz, "jz")))]
+        cx, cy, cz = (self._coefs(v, o) for v, o in ((dtsdx, norderx), (dtsdy, nordery),
+                                                      (dtsdz, norderz)))
+        nod = self.l_nodalgrid if l_nodalgrid is None else l_nodalgrid
+        F = c.ptrs(F)
+        _stream()
+        _lib.call("pxrhip_push_em3d_evec_norder", *F, float(mudt), cx.ctypes.data, cy.ctypes.data,
+                  cz.ctypes.data, int(nx), int(ny), int(nz), int(norderx), int(nordery),
+                  int(norderz), int(nxguard), int(nyguard), int(nzguard), int(nxs), int(nys),
+                  int(nzs), int(bool(nod)))
+        c.finish()
+
+    # yee.F90:752
+    def pxrpush_em3d_bvec_norder(self, ex, ey, ez, bx, by, bz, dtsdx, dtsdy, dtsdz, nx, ny, nz,
+                                 norderx, nordery, norderz, nxguard, nyguard, nzguard, nxs, nys,
+                                 nzs, l_nodalgrid=None):
+        shp = _gshape(nx, ny, nz, nxguard, nyguard, nzguard)
+        ng = shp[0] * shp[1] * shp[2]
+        c = _Call()
+        F = [c.arr(a, ng, nm, write=(i >= 3), fortran_shape=shp) for i, (a, nm) in enumerate(
+            ((ex, "ex"), (ey, "ey"), (ez, "ez"), (bx, "bx"), (by, "by"), (bz, "bz")))]
+        cx, cy, cz = (self._coefs(v, o) for v, o in ((dtsdx, norderx), (dtsdy, nordery),
+                                                      (dtsdz, norderz)))
+        nod = self.l_nodalgrid if l_nodalgrid is None else l_nodalgrid
+        F = c.ptrs(F)
+        _stream()
+        _lib.call("pxrhip_push_em3d_bvec_norder", *F, cx.ctypes.data, cy.ctypes.data,
+                  cz.ctypes.data, int(nx), int(ny), int(nz), int(norderx), int(nordery),
+                  int(norderz), int(nxguard), int(nyguard), int(nzguard), int(nxs), int(nys),
+                  int(nzs), int(bool(nod)))
+        c.finish()
+
+    # sorting.F90:247
+    def pxr_particle_bin_sorting(self, np2, xp, yp, zp, ux, uy, uz, gam, pid, wpid, xmin2, ymin2,
+                                 zmin2, xmax2, ymax2, zmax2, dxf, dyf, dzf):
+        n = int(np2)
+        c = _Call()
+        A = [c.arr(a, n, nm, write=True) for a, nm in
+             ((xp, "xp"), (yp, "yp"), (zp, "zp"), (ux, "ux"), (uy, "uy"), (uz, "uz"),
+              (gam, "gam"), (pid, "pid"))]
+        A = c.ptrs(A)
+        _stream()
+        _lib.call("pxrhip_particle_bin_sorting", n, *A, int(wpid), float(xmin2), float(ymin2),
+                  float(zmin2), float(xmax2), float(ymax2), float(zmax2), float(dxf), float(dyf),
+                  float(dzf))
+        c.finish()
+
+    # field_boundaries.F90:81 (field_bc on one periodic rank)
+    def field_bc(self, field, nxg, nyg, nzg, nx, ny, nz):
+        shp = _gshape(nx, ny, nz, nxg, nyg, nzg)
+        c = _Call()
+        p = c.arr(field, shp[0] * shp[1] * shp[2], "field", write=True, fortran_shape=shp)
+        (p,) = c.ptrs([p])
+        _stream()
+        _lib.call("pxrhip_field_bc_periodic", p, int(nx), int(ny), int(nz), int(nxg), int(nyg),
+                  int(nzg))
+        c.finish()
+
+    # field_boundaries.F90:403 (summation_bcs on one periodic rank)
+    def summation_bcs(self, array, nxg, nyg, nzg, nx, ny, nz):
+        shp = _gshape(nx, ny, nz, nxg, nyg, nzg)
+        c = _Call()
+        p = c.arr(array, shp[0] * shp[1] * shp[2], "array", write=True, fortran_shape=shp)
+        (p,) = c.ptrs([p])
+        _stream()
+        _lib.call("pxrhip_current_bc_periodic", p, int(nx), int(ny), int(nz), int(nxg), int(nyg),
+                  int(nzg))
+        c.finish()
+
+
+picsar = _Picsar()
