@@ -182,17 +182,29 @@ int pxrhip_particle_bin_sorting(int64_t np2, double *xp, double *yp,
 typedef struct {
   int64_t nx, ny, nz;    /* cells */
   int64_t ngx, ngy, ngz; /* guard nodes of every grid array */
-  int64_t ntx, nty, ntz; /* tiles */
+  int64_t ntx, nty, ntz; /* tiles (each >= 4 nodes wide, as the reference requires) */
   double xmin, ymin, zmin, dx, dy, dz;
+  /* per-tile sorting grid: cell sizes (absolute, i.e. the input value times
+   * dx as submain.F90:601 does) and the shift of its origin from the
+   * tile's lower edge (particle_sorting_sub, sorting.F90:150-160) */
+  double sorting_dx, sorting_dy, sorting_dz;
+  double sorting_shiftx, sorting_shifty, sorting_shiftz;
 } pxrhip_grid;
 
 typedef struct {
   double *x, *y, *z, *ux, *uy, *uz, *gaminv, *w; /* SoA particle arrays */
-  int64_t *tile_off; /* device, ntiles+1 entries (packed offsets) */
-  int64_t *tile_np;  /* device, ntiles entries */
-  int64_t capacity;  /* allocated length of each particle array */
+  int64_t *tile_off;  /* device, ntiles+1 entries: packed tile offsets, last = total */
+  int64_t *tile_np;   /* device, ntiles entries */
+  int64_t *bin_start; /* device, nbins+1 entries: first particle of every
+                         sorting-grid cell of every tile (filled by
+                         pxrhip_tiles_rebin_sort), or NULL */
+  int64_t capacity;   /* allocated length of each particle array */
   double q, m;
 } pxrhip_species;
+
+/* Number of sorting-grid cells over all tiles (the length of bin_start
+ * minus one) for this geometry. */
+int pxrhip_tiles_nbins(const pxrhip_grid *grid, int64_t *nbins);
 
 /* One fused pass over every particle of one species:
  * energy-conserving gather (field_gathering_plus_particle_pusher_*,
@@ -209,15 +221,22 @@ int pxrhip_tiles_gather_push_deposit(
     int64_t order, int64_t l_lower_order_in_v, int64_t pusher,
     int64_t depo_algo, int64_t chunk);
 
-/* particle_bcs (periodic wrap, particle_boundaries.F90:350-426) followed by
+/* particle_bcs on one periodic rank (particle_boundaries.F90:83-426 and
+ * the periodic case of particle_bcs_mpi_blocking, :1040-1300) followed by
  * particle_sorting_sub (sorting.F90:106-215): every particle is wrapped into
- * the periodic box, moved to the tile that now holds it and the tile is
- * stably sorted by cell of the sorting grid (cell size sdx..sdz, origin
- * shifted by shx..shz from the tile's lower edge).  The result is written
- * to dst (same capacity as src); dst's tile_off/tile_np are filled in. */
-int pxrhip_tiles_rebin_sort(const pxrhip_grid *grid, const pxrhip_species *src,
-                            pxrhip_species *dst, double sdx, double sdy,
-                            double sdz, double shx, double shy, double shz);
+ * the periodic box, assigned to the tile that now holds it (tile index of
+ * particle_boundaries.F90:397) and the tiles are ordered by sorting-grid
+ * cell (pxr_particle_bin_sorting's key).  src is read (its positions are
+ * wrapped in place) and the result is written to dst, which must have the
+ * same capacity; dst's tile_off, tile_np and bin_start are filled in.
+ * Particles of one cell keep no particular order among themselves. */
+int pxrhip_tiles_rebin_sort(const pxrhip_grid *grid, pxrhip_species *src,
+                            pxrhip_species *dst);
+
+/* Reads the device-side error counter of the tiled path (particles found
+ * outside the arrays and skipped) and resets it; a non-zero count is
+ * returned as PXRHIP_ERR_BOUNDS. */
+int pxrhip_tiles_check(void);
 
 #ifdef __cplusplus
 }

@@ -44,8 +44,10 @@ SIGNATURES = {
     "pxrhip_field_bc_periodic": [_P] + [_I] * 6,
     "pxrhip_current_bc_periodic": [_P] + [_I] * 6,
     "pxrhip_particle_bin_sorting": [_I] + [_P] * 8 + [_I] + [_F] * 9,
+    "pxrhip_tiles_nbins": [_P, _P],
+    "pxrhip_tiles_check": [],
     "pxrhip_tiles_gather_push_deposit": [_P, _P] + [_P] * 6 + [_P] * 3 + [_F] + [_I] * 5,
-    "pxrhip_tiles_rebin_sort": [_P, _P, _P] + [_F] * 6,
+    "pxrhip_tiles_rebin_sort": [_P, _P, _P],
 }
 
 

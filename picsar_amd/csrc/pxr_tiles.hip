@@ -1,0 +1,895 @@
+// pxr_tiles.hip -- the tiled step on device-resident particles.
+//
+// Work of the reference's managers, done per species over all tiles:
+//   * pxrhip_tiles_gather_push_deposit: gather + push + position push
+//     (field_gathering_plus_particle_pusher_cacheblock_sub,
+//     particle_pusher_manager_3d.F90:431-672) fused with the current
+//     deposition of the pushed particles (pxrdepose_currents_on_grid_jxjyjz,
+//     current_deposition_manager_3d.F90:224-1830).  A persistent kernel pulls
+//     chunks of cell-sorted particles from a work queue; for each chunk it
+//     stages the six field blocks its stencils touch in LDS, gathers from
+//     LDS, pushes in registers, accumulates the chunk's current block in LDS
+//     and adds that block to the global current arrays once.
+//   * pxrhip_tiles_rebin_sort: periodic wrap + tile re-binning
+//     (particle_boundaries.F90:350-426, :1040-1300) and the per-tile cell
+//     sort (sorting.F90:106-361) as one counting sort over (tile, cell) bins.
+#include <limits.h>
+#include <string.h>
+
+#include <vector>
+
+#include "pxr_device.h"
+#include "pxr_kernels.h"
+#include "pxr_runtime.h"
+
+using namespace pxr;
+
+namespace pxr {
+
+// ---------------------------------------------------------------------------
+// tile geometry (host), following set_tile_split_for_species
+// (tiling.F90:94-271) axis by axis
+// ---------------------------------------------------------------------------
+struct AxisSplit {
+  int nt = 1, n0 = 1;                   // tiles, nodes per tile (first tile)
+  std::vector<double> tmin, tmax;       // particle membership [tmin, tmax)
+};
+
+static int split_axis(int64_t ncells, int64_t nt_in, double xmin, double dx, AxisSplit &a) {
+  const int64_t ngrid = ncells + 1;  // nodes (mpi_routines.F90:1505-1510)
+  int64_t nt = nt_in;
+  int64_t n0 = ngrid / nt;
+  if (n0 < 4) return set_error(PXRHIP_ERR_ARG, "fewer than 4 nodes per tile (the reference falls back to 1 tile)");
+  const int64_t n0_last = n0 + (ngrid - n0 * nt);
+  const double xmax = xmin + (double)ncells * dx;
+  a.nt = (int)nt;
+  a.tmin.assign(nt, 0.0);
+  a.tmax.assign(nt, 0.0);
+  for (int64_t i = 0; i < nt; i++) {
+    double gmin = 0, gmax = 0, tmin = 0, tmax = 0;
+    int64_t ngt = 0;
+    if (i == 0) {
+      ngt = n0;
+      gmin = xmin;
+      gmax = gmin + (double)(n0 - 1) * dx;
+      tmin = gmin;
+      tmax = gmax + dx / 2.0;
+    }
+    if (i > 0 && i < nt - 1) {
+      ngt = n0;
+      gmin = xmin + (double)(i * n0) * dx;
+      gmax = gmin + (double)(ngt - 1) * dx;
+      tmin = gmin - dx / 2.0;
+      tmax = gmax + dx / 2.0;
+    }
+    if (i == nt - 1) {
+      ngt = n0_last;
+      gmin = xmin + (double)(i * n0) * dx;
+      gmax = gmin + (double)(ngt - 1) * dx;
+      tmin = gmin - dx / 2.0;
+      tmax = xmax;
+    }
+    a.tmin[i] = tmin;
+    a.tmax[i] = tmax;
+  }
+  // particle_bcs_tiles reads the node count of tile 1 (particle_boundaries.F90:364)
+  a.n0 = (int)(nt == 1 ? n0_last : n0);
+  return PXRHIP_OK;
+}
+
+struct TileTable {
+  pxrhip_grid key;
+  bool valid = false;
+  int ntiles = 0;
+  int nt[3] = {1, 1, 1}, n0[3] = {1, 1, 1};
+  int64_t nbins = 0, nrows = 0;
+  int max_bins_tile = 0;
+  // device arrays, per tile
+  double *sorg = nullptr;     // 3*ntiles: sorting-grid origin
+  int *n3 = nullptr;          // 3*ntiles: sorting cells per axis
+  int64_t *bin_base = nullptr;  // ntiles+1
+};
+
+static TileTable g_tt;
+
+static void free_table(TileTable &t) {
+  if (t.sorg || t.n3 || t.bin_base) (void)hipStreamSynchronize(rt().stream);
+  if (t.sorg) (void)hipFree(t.sorg);
+  if (t.n3) (void)hipFree(t.n3);
+  if (t.bin_base) (void)hipFree(t.bin_base);
+  t.sorg = nullptr;
+  t.n3 = nullptr;
+  t.bin_base = nullptr;
+  t.valid = false;
+}
+
+static int check_grid_desc(const pxrhip_grid *g) {
+  if (!g) return set_error(PXRHIP_ERR_ARG, "null grid");
+  if (g->nx < 1 || g->ny < 1 || g->nz < 1) return set_error(PXRHIP_ERR_ARG, "empty grid");
+  if (g->ngx < 1 || g->ngy < 1 || g->ngz < 1) return set_error(PXRHIP_ERR_ARG, "guards must be >= 1");
+  if (g->ntx < 1 || g->nty < 1 || g->ntz < 1) return set_error(PXRHIP_ERR_ARG, "tile counts must be >= 1");
+  if (!(g->dx > 0) || !(g->dy > 0) || !(g->dz > 0)) return set_error(PXRHIP_ERR_ARG, "dx, dy, dz must be > 0");
+  if (!(g->sorting_dx > 0) || !(g->sorting_dy > 0) || !(g->sorting_dz > 0))
+    return set_error(PXRHIP_ERR_ARG, "sorting cell sizes must be > 0");
+  double tot = (double)(g->nx + 1 + 2 * g->ngx) * (double)(g->ny + 1 + 2 * g->ngy) *
+               (double)(g->nz + 1 + 2 * g->ngz);
+  if (tot > 4.0e9) return set_error(PXRHIP_ERR_ARG, "grid too large");
+  return PXRHIP_OK;
+}
+
+static int get_table(const pxrhip_grid *g, TileTable **out) {
+  int r = check_grid_desc(g);
+  if (r) return r;
+  if (g_tt.valid && memcmp(&g_tt.key, g, sizeof(pxrhip_grid)) == 0) {
+    *out = &g_tt;
+    return PXRHIP_OK;
+  }
+  AxisSplit ax[3];
+  const int64_t nc[3] = {g->nx, g->ny, g->nz}, ntl[3] = {g->ntx, g->nty, g->ntz};
+  const double x0[3] = {g->xmin, g->ymin, g->zmin}, d[3] = {g->dx, g->dy, g->dz};
+  const double sd[3] = {g->sorting_dx, g->sorting_dy, g->sorting_dz};
+  const double sh[3] = {g->sorting_shiftx, g->sorting_shifty, g->sorting_shiftz};
+  for (int a = 0; a < 3; a++)
+    if ((r = split_axis(nc[a], ntl[a], x0[a], d[a], ax[a]))) return r;
+  const int nt = ax[0].nt * ax[1].nt * ax[2].nt;
+  std::vector<double> sorg(3 * (size_t)nt);
+  std::vector<int> n3(3 * (size_t)nt);
+  std::vector<int64_t> base(nt + 1);
+  int64_t nb = 0, nrows = 0;
+  int maxb = 0;
+  for (int iz = 0; iz < ax[2].nt; iz++)
+    for (int iy = 0; iy < ax[1].nt; iy++)
+      for (int ix = 0; ix < ax[0].nt; ix++) {
+        const int t = ix + ax[0].nt * (iy + ax[1].nt * iz);
+        const int ii[3] = {ix, iy, iz};
+        int64_t cells = 1;
+        for (int a = 0; a < 3; a++) {
+          // particle_sorting_sub: xmin2 = x_tile_min + shift, xmax2 = x_tile_max + sdx
+          const double lo = ax[a].tmin[ii[a]] + sh[a];
+          const double hi = ax[a].tmax[ii[a]] + sd[a];
+          const double sdi = 1.0 / sd[a];  // sorting.F90:274
+          const int64_t n = (int64_t)ceil((hi - lo) * sdi);
+          if (n < 1 || n > (1 << 20)) return set_error(PXRHIP_ERR_ARG, "bad sorting grid");
+          sorg[3 * t + a] = lo;
+          n3[3 * t + a] = (int)n;
+          cells *= n;
+        }
+        if (cells > (1 << 26)) return set_error(PXRHIP_ERR_ARG, "sorting grid of a tile too fine");
+        base[t] = nb;
+        nb += cells;
+        nrows += (int64_t)n3[3 * t + 1] * n3[3 * t + 2];
+        if (cells > maxb) maxb = (int)cells;
+      }
+  base[nt] = nb;
+  if (nb >= ((int64_t)1 << 31)) return set_error(PXRHIP_ERR_ARG, "too many sorting cells");
+  free_table(g_tt);
+  PXR_HIP(hipMalloc(&g_tt.sorg, sizeof(double) * sorg.size()));
+  PXR_HIP(hipMalloc(&g_tt.n3, sizeof(int) * n3.size()));
+  PXR_HIP(hipMalloc(&g_tt.bin_base, sizeof(int64_t) * base.size()));
+  PXR_HIP(hipMemcpy(g_tt.sorg, sorg.data(), sizeof(double) * sorg.size(), hipMemcpyHostToDevice));
+  PXR_HIP(hipMemcpy(g_tt.n3, n3.data(), sizeof(int) * n3.size(), hipMemcpyHostToDevice));
+  PXR_HIP(hipMemcpy(g_tt.bin_base, base.data(), sizeof(int64_t) * base.size(), hipMemcpyHostToDevice));
+  g_tt.key = *g;
+  g_tt.ntiles = nt;
+  for (int a = 0; a < 3; a++) {
+    g_tt.nt[a] = ax[a].nt;
+    g_tt.n0[a] = ax[a].n0;
+  }
+  g_tt.nbins = nb;
+  g_tt.nrows = nrows;
+  g_tt.max_bins_tile = maxb;
+  g_tt.valid = true;
+  *out = &g_tt;
+  return PXRHIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// rebin + sort kernels
+// ---------------------------------------------------------------------------
+struct RebinArgs {
+  double *x, *y, *z;
+  const int64_t *tile_off;  // src, ntiles+1
+  int ntiles;
+  double xmin[3], xmax[3], len[3], dx[3], sdi[3];
+  int nt[3], n0[3];
+  const double *sorg;
+  const int *n3;
+  const int64_t *bin_base;
+  int *key;       // global bin of every particle
+  int *hist;      // nbins counters
+  unsigned long long *bad;
+};
+
+// periodic wrap (particle_bcs_mpi_blocking, periodic case), tile index
+// (particle_boundaries.F90:397) and sorting key (sorting.F90:294-304)
+__global__ __launch_bounds__(256) void k_rb_keys(RebinArgs a, i64 cap) {
+  const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  const i64 n = a.tile_off[a.ntiles];
+  const bool act = i < n && i < cap;
+  int gbin = -1;
+  if (act) {
+    double p[3] = {a.x[i], a.y[i], a.z[i]};
+    int it[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      const double v = p[d];
+      if (v < a.xmin[d]) p[d] = v + a.len[d];
+      if (v >= a.xmax[d]) p[d] = v - a.len[d];
+      int t = (int)floor(((p[d] - a.xmin[d]) + a.dx[d] / 2.0) / ((double)a.n0[d] * a.dx[d]));
+      it[d] = min(max(t, 0), a.nt[d] - 1);
+    }
+    a.x[i] = p[0];
+    a.y[i] = p[1];
+    a.z[i] = p[2];
+    const int t = it[0] + a.nt[0] * (it[1] + a.nt[1] * it[2]);
+    int c[3];
+    bool ok = true;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      const double f = floor((p[d] - a.sorg[3 * t + d]) * a.sdi[d]);
+      const int n3 = a.n3[3 * t + d];
+      if (!(f >= 0.0) || f >= (double)n3) ok = false;  // also catches NaN
+      c[d] = ok ? (int)f : 0;
+    }
+    if (!ok) count_bad(a.bad);
+    const int cell = c[2] * a.n3[3 * t] * a.n3[3 * t + 1] + c[1] * a.n3[3 * t] + c[0];
+    gbin = (int)(a.bin_base[t] + cell);
+    a.key[i] = gbin;
+  }
+  int rank, cnt, leader;
+  wave_peers(act, gbin, rank, cnt, leader);
+  if (act && rank == 0) atomicAdd(a.hist + gbin, cnt);
+}
+
+// exclusive scan of n counts into 64-bit starts: three passes over blocks
+// of SCAN_B elements
+#define SCAN_B 2048
+__global__ __launch_bounds__(256) void k_scan_blocks(const int *in, i64 n, i64 *bsum) {
+  __shared__ i64 s[256];
+  const i64 b0 = (i64)blockIdx.x * SCAN_B;
+  i64 acc = 0;
+  for (int k = threadIdx.x; k < SCAN_B; k += 256) {
+    const i64 i = b0 + k;
+    if (i < n) acc += in[i];
+  }
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) bsum[blockIdx.x] = s[0];
+}
+
+// in place exclusive scan of nb 64-bit values, one workgroup
+__global__ __launch_bounds__(1024) void k_scan_i64_1wg(i64 *v, i64 nb, i64 *total) {
+  __shared__ i64 s[1024];
+  __shared__ i64 carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (i64 base = 0; base < nb; base += 1024) {
+    const i64 i = base + threadIdx.x;
+    const i64 x = i < nb ? v[i] : 0;
+    s[threadIdx.x] = x;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      const i64 y = threadIdx.x >= (unsigned)off ? s[threadIdx.x - off] : 0;
+      __syncthreads();
+      s[threadIdx.x] += y;
+      __syncthreads();
+    }
+    const i64 incl = s[threadIdx.x];
+    if (i < nb) v[i] = carry + incl - x;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry += incl;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && total) *total = carry;
+}
+
+// block-local exclusive scan plus the block's offset; writes out[0..n] with
+// out[n] = total
+__global__ __launch_bounds__(256) void k_scan_apply(const int *in, i64 n, const i64 *boff,
+                                                    i64 *out, i64 *cursor) {
+  __shared__ i64 s[256];
+  const i64 b0 = (i64)blockIdx.x * SCAN_B;
+  constexpr int PER = SCAN_B / 256;
+  i64 loc[PER];
+  i64 acc = 0;
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const i64 i = b0 + (i64)threadIdx.x * PER + k;
+    loc[k] = i < n ? in[i] : 0;
+    acc += loc[k];
+  }
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    const i64 y = threadIdx.x >= (unsigned)off ? s[threadIdx.x - off] : 0;
+    __syncthreads();
+    s[threadIdx.x] += y;
+    __syncthreads();
+  }
+  i64 run = boff[blockIdx.x] + s[threadIdx.x] - acc;
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const i64 i = b0 + (i64)threadIdx.x * PER + k;
+    if (i < n) {
+      out[i] = run;
+      if (cursor) cursor[i] = run;
+    }
+    run += loc[k];
+    if (i == n - 1) out[n] = run;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_fill_i32_t(int *p, i64 n) {
+  const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+struct ScatterArgs {
+  const double *src[8];
+  double *dst[8];
+  const int64_t *src_off;
+  int ntiles;
+  const int *key;
+  unsigned long long *cursor;  // running start of every bin
+};
+
+__global__ __launch_bounds__(256) void k_rb_scatter(ScatterArgs a, i64 cap) {
+  const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  const i64 n = a.src_off[a.ntiles];
+  const bool act = i < n && i < cap;
+  const int k = act ? a.key[i] : -1;
+  int rank, cnt, leader;
+  wave_peers(act, k, rank, cnt, leader);
+  unsigned long long slot = 0;
+  if (act && rank == 0) slot = atomicAdd(a.cursor + k, (unsigned long long)cnt);
+  slot = __shfl(slot, leader);
+  if (!act) return;
+  const i64 d = (i64)slot + rank;
+#pragma unroll
+  for (int c = 0; c < 8; c++) a.dst[c][d] = a.src[c][i];
+}
+
+__global__ __launch_bounds__(256) void k_rb_tiles(int ntiles, const int64_t *bin_base,
+                                                  const int64_t *bin_start, int64_t *off,
+                                                  int64_t *np) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > ntiles) return;
+  const int64_t o = bin_start[bin_base[t]];
+  off[t] = o;
+  if (t < ntiles) np[t] = bin_start[bin_base[t + 1]] - o;
+}
+
+// ---------------------------------------------------------------------------
+// chunk table of the fused kernel
+// ---------------------------------------------------------------------------
+struct Chunk {
+  long long begin;
+  int n;
+  int tile;
+};
+
+struct ChunkArgs {
+  int ntiles, ch;
+  const int64_t *tile_off, *tile_np, *bin_start;
+  const int *n3;
+  const int64_t *bin_base;
+  int64_t *count;  // per tile, then scanned in place
+  Chunk *out;
+  int *nchunks, *queue;
+  i64 maxchunks;
+};
+
+// Chunks are runs of whole sorting rows (one y-row of cells of one z-plane)
+// of about `ch` particles: a run never grows past ch, and once it holds ch/2
+// it ends at a z-plane boundary, so its cells form a compact block.  Rows
+// denser than ch are cut.  Without sorting information a tile is cut into
+// runs of ch particles.
+template <bool WRITE>
+PXR_DEV int64_t walk_chunks(const ChunkArgs &a, int t, int64_t pos) {
+  const int64_t off = a.tile_off[t], np = a.tile_np[t];
+  int64_t nch = 0;
+  auto emit = [&](int64_t b, int64_t n) {
+    if (WRITE && pos + nch < a.maxchunks) {
+      Chunk c;
+      c.begin = b;
+      c.n = (int)n;
+      c.tile = t;
+      a.out[pos + nch] = c;
+    }
+    nch++;
+  };
+  if (np <= 0) return 0;
+  if (!a.bin_start) {
+    for (int64_t b = 0; b < np; b += a.ch) emit(off + b, min((int64_t)a.ch, np - b));
+    return nch;
+  }
+  const int nx3 = a.n3[3 * t], ny3 = a.n3[3 * t + 1], nz3 = a.n3[3 * t + 2];
+  const int64_t base = a.bin_base[t];
+  int64_t cb = off, cnt = 0;
+  int zc = -1;
+  for (int z = 0; z < nz3; z++)
+    for (int y = 0; y < ny3; y++) {
+      const int64_t row = base + ((int64_t)z * ny3 + y) * nx3;
+      const int64_t rb = a.bin_start[row], rn = a.bin_start[row + nx3] - rb;
+      if (rn <= 0) continue;
+      if (cnt > 0 && (cnt + rn > a.ch || (z != zc && cnt >= a.ch / 2))) {
+        emit(cb, cnt);
+        cnt = 0;
+      }
+      if (cnt == 0) {
+        cb = rb;
+        zc = z;
+      }
+      cnt += rn;
+      while (cnt > a.ch) {
+        emit(cb, a.ch);
+        cb += a.ch;
+        cnt -= a.ch;
+      }
+    }
+  if (cnt > 0) emit(cb, cnt);
+  return nch;
+}
+
+__global__ __launch_bounds__(256) void k_chunk_count(ChunkArgs a) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < a.ntiles) a.count[t] = walk_chunks<false>(a, t, 0);
+}
+
+__global__ __launch_bounds__(256) void k_chunk_write(ChunkArgs a) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < a.ntiles) walk_chunks<true>(a, t, a.count[t]);
+  if (t == 0) {
+    // a.count[ntiles] holds the total after the scan
+    const i64 tot = a.count[a.ntiles];
+    *a.nchunks = (int)min(tot, a.maxchunks);
+    *a.queue = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// fused gather + push + deposit
+// ---------------------------------------------------------------------------
+struct FusedArgs {
+  const Chunk *chunks;
+  const int *nchunks;
+  int *queue;
+  double *x, *y, *z, *ux, *uy, *uz, *gi;
+  const double *w;
+  const double *F[6];
+  double *J[3];
+  GridGeom g;
+  double xmin, ymin, zmin, dxi, dyi, dzi, s;
+  double cst;        // Boris: q*dt/(2m); Vay: q*dt/m
+  double dt;
+  EsirkConst<double> ec;
+  double q, dx, dy, dz;  // direct deposition
+  unsigned long long *bad;
+  int lds_doubles;   // dynamic LDS size in doubles
+};
+
+PXR_DEV void wave_minmax(int *lo, int *hi) {
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      lo[d] = min(lo[d], __shfl_xor(lo[d], o));
+      hi[d] = max(hi[d], __shfl_xor(hi[d], o));
+    }
+  }
+}
+
+struct LdsAcc {
+  double *p[3];
+  int i0, j0, k0, sy, sz;
+  PXR_DEV void add(int c, int i, int j, int k, double v) const {
+    atomicAdd(p[c] + ((i - i0) + sy * (j - j0) + sz * (k - k0)), v);
+  }
+};
+
+struct GAcc {
+  double *p[3];
+  const GridGeom *g;
+  PXR_DEV void add(int c, int i, int j, int k, double v) const {
+    unsafeAtomicAdd(p[c] + g->idx(i, j, k), v);
+  }
+};
+
+template <int BS, int N, bool LO, int PUSHER, int DEPO>
+__global__ __launch_bounds__(BS) void k_fused(FusedArgs a) {
+  constexpr int M = LO ? N - 1 : N;
+  extern __shared__ double smem[];
+  __shared__ int s_chunk;
+  __shared__ int s_box[6];
+  const GridGeom &g = a.g;
+  const int glo[3] = {-g.gx, -g.gy, -g.gz};
+  const int ghi[3] = {g.nx + g.gx, g.ny + g.gy, g.nz + g.gz};
+  const int nch = *a.nchunks;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      s_chunk = atomicAdd(a.queue, 1);
+      for (int d = 0; d < 3; d++) {
+        s_box[d] = INT_MAX;
+        s_box[3 + d] = INT_MIN;
+      }
+    }
+    __syncthreads();
+    const int c = s_chunk;
+    if (c >= nch) break;
+    const Chunk ch = a.chunks[c];
+    const i64 p0 = ch.begin;
+    // ---- 1. box of the gather stencils ---------------------------------
+    int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+    for (int k = threadIdx.x; k < ch.n; k += BS) {
+      const i64 p = p0 + k;
+      GatherStencil<N, M, double> st;
+      gather_stencil<N, M, double>(st, (a.x[p] - a.xmin) * a.dxi, (a.y[p] - a.ymin) * a.dyi,
+                                   (a.z[p] - a.zmin) * a.dzi, a.s);
+      int l2[3], h2[3];
+      gather_bounds<N, M, double>(st, l2, h2);
+      for (int d = 0; d < 3; d++) {
+        lo[d] = min(lo[d], l2[d]);
+        hi[d] = max(hi[d], h2[d]);
+      }
+    }
+    wave_minmax(lo, hi);
+    if ((threadIdx.x & 63) == 0)
+      for (int d = 0; d < 3; d++) {
+        atomicMin(&s_box[d], lo[d]);
+        atomicMax(&s_box[3 + d], hi[d]);
+      }
+    __syncthreads();
+    int R0[3], R1[3], e[3];
+    for (int d = 0; d < 3; d++) {
+      R0[d] = max(s_box[d], glo[d]);
+      R1[d] = min(s_box[3 + d], ghi[d]);
+      e[d] = R1[d] >= R0[d] ? R1[d] - R0[d] + 1 : 0;
+    }
+    const i64 nb64 = (i64)e[0] * e[1] * e[2];
+    const bool lds_f = nb64 > 0 && 6 * nb64 <= a.lds_doubles;
+    const int nb = lds_f ? (int)nb64 : 0;
+    if (lds_f) {
+      for (int f = 0; f < 6; f++) {
+        const double *src = a.F[f];
+        for (int idx = threadIdx.x; idx < nb; idx += BS) {
+          const int i = R0[0] + idx % e[0];
+          const int j = R0[1] + (idx / e[0]) % e[1];
+          const int k = R0[2] + idx / (e[0] * e[1]);
+          smem[f * nb + idx] = src[g.idx(i, j, k)];
+        }
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int d = 0; d < 3; d++) {
+        s_box[d] = INT_MAX;
+        s_box[3 + d] = INT_MIN;
+      }
+    // ---- 2. gather, push, position push ---------------------------------
+    int dlo[3] = {INT_MAX, INT_MAX, INT_MAX}, dhi[3] = {INT_MIN, INT_MIN, INT_MIN};
+    for (int k = threadIdx.x; k < ch.n; k += BS) {
+      const i64 p = p0 + k;
+      GatherStencil<N, M, double> st;
+      gather_stencil<N, M, double>(st, (a.x[p] - a.xmin) * a.dxi, (a.y[p] - a.ymin) * a.dyi,
+                                   (a.z[p] - a.zmin) * a.dzi, a.s);
+      int l2[3], h2[3];
+      gather_bounds<N, M, double>(st, l2, h2);
+      if (!g.inside(l2[0], l2[1], l2[2]) || !g.inside(h2[0], h2[1], h2[2])) {
+        count_bad(a.bad);
+        continue;
+      }
+      double E[6];
+      if (lds_f) {
+        LdsView<double> v[6];
+        for (int f = 0; f < 6; f++)
+          v[f] = LdsView<double>{smem + f * nb, R0[0], R0[1], R0[2], e[0], e[0] * e[1]};
+        gather_apply<N, M, double>(st, v, E);
+      } else {
+        FieldView<double> v[6];
+        for (int f = 0; f < 6; f++) v[f] = FieldView<double>{a.F[f] + g.idx(0, 0, 0), g.NX, g.NX * g.NY};
+        gather_apply<N, M, double>(st, v, E);
+      }
+      double u[3] = {a.ux[p], a.uy[p], a.uz[p]};
+      double gi = a.gi[p];
+      if (PUSHER == 1) vay_push<double>(u, gi, E[0], E[1], E[2], E[3], E[4], E[5], a.cst);
+      else boris_push<double>(u, gi, E[0], E[1], E[2], E[3], E[4], E[5], a.cst);
+      // pxr_pushxyz (boris_3d.F90:880-882)
+      const double xn = a.x[p] + u[0] * gi * a.dt;
+      const double yn = a.y[p] + u[1] * gi * a.dt;
+      const double zn = a.z[p] + u[2] * gi * a.dt;
+      a.ux[p] = u[0];
+      a.uy[p] = u[1];
+      a.uz[p] = u[2];
+      a.gi[p] = gi;
+      a.x[p] = xn;
+      a.y[p] = yn;
+      a.z[p] = zn;
+      int b0[3], b1[3];
+      if (DEPO == 0) {
+        EsirkepovIn<double> in;
+        esirkepov_prepare<N, double>(in, xn, yn, zn, u[0], u[1], u[2], gi, a.w[p], a.ec);
+        if (!esirkepov_box<N, double>(in, b0, b1)) continue;
+      } else {
+        DirectIn<double> in;
+        direct_prepare<N, double>(in, xn, yn, zn, u[0], u[1], u[2], gi, a.w[p], a.q, a.xmin,
+                                  a.ymin, a.zmin, a.dt, a.dx, a.dy, a.dz, a.s);
+        direct_box<N>(in, b0, b1);
+      }
+      for (int d = 0; d < 3; d++) {
+        dlo[d] = min(dlo[d], b0[d]);
+        dhi[d] = max(dhi[d], b1[d]);
+      }
+    }
+    wave_minmax(dlo, dhi);
+    __syncthreads();  // field block no longer read; s_box reset visible
+    if ((threadIdx.x & 63) == 0)
+      for (int d = 0; d < 3; d++) {
+        atomicMin(&s_box[d], dlo[d]);
+        atomicMax(&s_box[3 + d], dhi[d]);
+      }
+    __syncthreads();
+    for (int d = 0; d < 3; d++) {
+      R0[d] = max(s_box[d], glo[d]);
+      R1[d] = min(s_box[3 + d], ghi[d]);
+      e[d] = R1[d] >= R0[d] ? R1[d] - R0[d] + 1 : 0;
+    }
+    const i64 nj64 = (i64)e[0] * e[1] * e[2];
+    const bool lds_j = nj64 > 0 && 3 * nj64 <= a.lds_doubles;
+    const int nj = lds_j ? (int)nj64 : 0;
+    if (lds_j)
+      for (int idx = threadIdx.x; idx < 3 * nj; idx += BS) smem[idx] = 0.0;
+    __syncthreads();
+    // ---- 3. deposit the pushed particles ---------------------------------
+    const LdsAcc lacc{{smem, smem + nj, smem + 2 * nj}, R0[0], R0[1], R0[2], e[0], e[0] * e[1]};
+    const GAcc gacc{{a.J[0], a.J[1], a.J[2]}, &a.g};
+    for (int k = threadIdx.x; k < ch.n; k += BS) {
+      const i64 p = p0 + k;
+      const double xn = a.x[p], yn = a.y[p], zn = a.z[p];
+      const double ux = a.ux[p], uy = a.uy[p], uz = a.uz[p], gi = a.gi[p], w = a.w[p];
+      int b0[3], b1[3];
+      if (DEPO == 0) {
+        EsirkepovIn<double> in;
+        esirkepov_prepare<N, double>(in, xn, yn, zn, ux, uy, uz, gi, w, a.ec);
+        if (!esirkepov_box<N, double>(in, b0, b1) || !g.inside(b0[0], b0[1], b0[2]) ||
+            !g.inside(b1[0], b1[1], b1[2])) {
+          count_bad(a.bad);
+          continue;
+        }
+        if (lds_j) esirkepov_deposit<N, double>(in, lacc);
+        else esirkepov_deposit<N, double>(in, gacc);
+      } else {
+        DirectIn<double> in;
+        direct_prepare<N, double>(in, xn, yn, zn, ux, uy, uz, gi, w, a.q, a.xmin, a.ymin, a.zmin,
+                                  a.dt, a.dx, a.dy, a.dz, a.s);
+        direct_box<N>(in, b0, b1);
+        if (!g.inside(b0[0], b0[1], b0[2]) || !g.inside(b1[0], b1[1], b1[2])) {
+          count_bad(a.bad);
+          continue;
+        }
+        if (lds_j) direct_deposit<N, double>(in, lacc);
+        else direct_deposit<N, double>(in, gacc);
+      }
+    }
+    __syncthreads();
+    if (lds_j) {
+      for (int idx = threadIdx.x; idx < nj; idx += BS) {
+        const int i = R0[0] + idx % e[0];
+        const int j = R0[1] + (idx / e[0]) % e[1];
+        const int k = R0[2] + idx / (e[0] * e[1]);
+        const i64 gi_ = g.idx(i, j, k);
+        for (int c3 = 0; c3 < 3; c3++) {
+          const double v = smem[c3 * nj + idx];
+          if (v != 0.0) unsafeAtomicAdd(a.J[c3] + gi_, v);
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace pxr
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+static const int FUSED_BS = 256;
+static const int FUSED_LDS_BYTES = 64 * 1024 - 256;
+
+extern "C" {
+
+int pxrhip_tiles_nbins(const pxrhip_grid *grid, int64_t *nbins) {
+  PXR_INIT();
+  TileTable *tt;
+  int r = get_table(grid, &tt);
+  if (r) return r;
+  if (nbins) *nbins = tt->nbins;
+  return PXRHIP_OK;
+}
+
+int pxrhip_tiles_check(void) {
+  PXR_INIT();
+  unsigned long long n = 0;
+  int r = take_bad(&n);
+  if (r) return r;
+  if (n)
+    return set_error(PXRHIP_ERR_BOUNDS, "%llu particle(s) were outside the grid arrays or sorting "
+                     "grids and were skipped", n);
+  return PXRHIP_OK;
+}
+
+static int check_species(const pxrhip_species *s, bool need_bins) {
+  if (!s) return set_error(PXRHIP_ERR_ARG, "null species");
+  if (!s->x || !s->y || !s->z || !s->ux || !s->uy || !s->uz || !s->gaminv || !s->w)
+    return set_error(PXRHIP_ERR_ARG, "null particle array");
+  if (!s->tile_off || !s->tile_np) return set_error(PXRHIP_ERR_ARG, "null tile offsets");
+  if (need_bins && !s->bin_start) return set_error(PXRHIP_ERR_ARG, "null bin_start");
+  if (s->capacity < 0 || s->capacity > ((int64_t)1 << 31) - 1)
+    return set_error(PXRHIP_ERR_ARG, "capacity must be in [0, 2^31)");
+  return PXRHIP_OK;
+}
+
+int pxrhip_tiles_rebin_sort(const pxrhip_grid *grid, pxrhip_species *src, pxrhip_species *dst) {
+  PXR_INIT();
+  TileTable *tt;
+  int r = get_table(grid, &tt);
+  if (r) return r;
+  if ((r = check_species(src, false)) || (r = check_species(dst, true))) return r;
+  if (dst->capacity < src->capacity)
+    return set_error(PXRHIP_ERR_CAPACITY, "dst capacity smaller than src capacity");
+  const i64 cap = src->capacity;
+  const i64 nbins = tt->nbins;
+  hipStream_t st = rt().stream;
+  int *key = (int *)scratch(4, sizeof(int) * (size_t)(cap > 1 ? cap : 1));
+  int *hist = (int *)scratch(5, sizeof(int) * (size_t)nbins);
+  const i64 nblk = (nbins + SCAN_B - 1) / SCAN_B;
+  i64 *bsum = (i64 *)scratch(6, sizeof(i64) * (size_t)(nblk + 1));
+  unsigned long long *cursor = (unsigned long long *)scratch(7, sizeof(i64) * (size_t)nbins);
+  if (!key || !hist || !bsum || !cursor) return set_error(PXRHIP_ERR_HIP, "scratch allocation failed");
+  RebinArgs a;
+  a.x = src->x;
+  a.y = src->y;
+  a.z = src->z;
+  a.tile_off = src->tile_off;
+  a.ntiles = tt->ntiles;
+  const double xm[3] = {grid->xmin, grid->ymin, grid->zmin}, d[3] = {grid->dx, grid->dy, grid->dz};
+  const double sd[3] = {grid->sorting_dx, grid->sorting_dy, grid->sorting_dz};
+  const int64_t nc[3] = {grid->nx, grid->ny, grid->nz};
+  for (int k = 0; k < 3; k++) {
+    a.xmin[k] = xm[k];
+    a.len[k] = (double)nc[k] * d[k];
+    a.xmax[k] = xm[k] + a.len[k];
+    a.dx[k] = d[k];
+    a.sdi[k] = 1.0 / sd[k];
+    a.nt[k] = tt->nt[k];
+    a.n0[k] = tt->n0[k];
+  }
+  a.sorg = tt->sorg;
+  a.n3 = tt->n3;
+  a.bin_base = tt->bin_base;
+  a.key = key;
+  a.hist = hist;
+  a.bad = rt().d_bad;
+  hipLaunchKernelGGL(k_fill_i32_t, dim3(grid1d(nbins, 256)), dim3(256), 0, st, hist, nbins);
+  hipLaunchKernelGGL(k_rb_keys, dim3(grid1d(cap, 256)), dim3(256), 0, st, a, cap);
+  hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nblk), dim3(256), 0, st, hist, nbins, bsum);
+  hipLaunchKernelGGL(k_scan_i64_1wg, dim3(1), dim3(1024), 0, st, bsum, nblk, (i64 *)nullptr);
+  hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nblk), dim3(256), 0, st, hist, nbins, bsum,
+                     (i64 *)dst->bin_start, (i64 *)cursor);
+  ScatterArgs sa;
+  double *S[8] = {src->x, src->y, src->z, src->ux, src->uy, src->uz, src->gaminv, src->w};
+  double *D[8] = {dst->x, dst->y, dst->z, dst->ux, dst->uy, dst->uz, dst->gaminv, dst->w};
+  for (int k = 0; k < 8; k++) {
+    sa.src[k] = S[k];
+    sa.dst[k] = D[k];
+  }
+  sa.src_off = src->tile_off;
+  sa.ntiles = tt->ntiles;
+  sa.key = key;
+  sa.cursor = cursor;
+  hipLaunchKernelGGL(k_rb_scatter, dim3(grid1d(cap, 256)), dim3(256), 0, st, sa, cap);
+  hipLaunchKernelGGL(k_rb_tiles, dim3(grid1d(tt->ntiles + 1, 256)), dim3(256), 0, st, tt->ntiles,
+                     tt->bin_base, dst->bin_start, dst->tile_off, dst->tile_np);
+  PXR_LAUNCHED();
+  return PXRHIP_OK;
+}
+
+int pxrhip_tiles_gather_push_deposit(const pxrhip_grid *grid, const pxrhip_species *sp,
+                                     const double *exg, const double *eyg, const double *ezg,
+                                     const double *bxg, const double *byg, const double *bzg,
+                                     double *jx, double *jy, double *jz, double dt, int64_t order,
+                                     int64_t l_lower_order_in_v, int64_t pusher,
+                                     int64_t depo_algo, int64_t chunk) {
+  PXR_INIT();
+  TileTable *tt;
+  int r = get_table(grid, &tt);
+  if (r) return r;
+  if ((r = check_species(sp, false))) return r;
+  if (order < 1 || order > 3) return set_error(PXRHIP_ERR_UNSUPPORTED, "shape order must be 1..3");
+  if (pusher != 0 && pusher != 1) return set_error(PXRHIP_ERR_ARG, "pusher must be 0 (Boris) or 1 (Vay)");
+  if (depo_algo != 0 && depo_algo != 3) return set_error(PXRHIP_ERR_ARG, "depo_algo must be 0 or 3");
+  if (!exg || !eyg || !ezg || !bxg || !byg || !bzg || !jx || !jy || !jz)
+    return set_error(PXRHIP_ERR_ARG, "null field array");
+  const int ch = chunk > 0 ? (int)(chunk < (1 << 20) ? chunk : (1 << 20)) : 1024;
+  hipStream_t st = rt().stream;
+  // chunk table
+  const i64 cap = sp->capacity;
+  const i64 maxchunks = tt->nrows + 3 * (cap / ch + 1) + tt->ntiles + 16;
+  int64_t *cnt = (int64_t *)scratch(8, sizeof(int64_t) * (size_t)(tt->ntiles + 1));
+  Chunk *chunks = (Chunk *)scratch(9, sizeof(Chunk) * (size_t)maxchunks);
+  int *ctl = (int *)scratch(10, 2 * sizeof(int));
+  if (!cnt || !chunks || !ctl) return set_error(PXRHIP_ERR_HIP, "scratch allocation failed");
+  ChunkArgs ca;
+  ca.ntiles = tt->ntiles;
+  ca.ch = ch;
+  ca.tile_off = sp->tile_off;
+  ca.tile_np = sp->tile_np;
+  ca.bin_start = sp->bin_start;
+  ca.n3 = tt->n3;
+  ca.bin_base = tt->bin_base;
+  ca.count = cnt;
+  ca.out = chunks;
+  ca.nchunks = ctl;
+  ca.queue = ctl + 1;
+  ca.maxchunks = maxchunks;
+  hipLaunchKernelGGL(k_chunk_count, dim3(grid1d(tt->ntiles, 256)), dim3(256), 0, st, ca);
+  hipLaunchKernelGGL(k_scan_i64_1wg, dim3(1), dim3(1024), 0, st, (i64 *)cnt, (i64)tt->ntiles + 1,
+                     (i64 *)nullptr);
+  hipLaunchKernelGGL(k_chunk_write, dim3(grid1d(tt->ntiles, 256)), dim3(256), 0, st, ca);
+  PXR_LAUNCHED();
+  // fused kernel
+  FusedArgs a;
+  a.chunks = chunks;
+  a.nchunks = ctl;
+  a.queue = ctl + 1;
+  a.x = sp->x; a.y = sp->y; a.z = sp->z;
+  a.ux = sp->ux; a.uy = sp->uy; a.uz = sp->uz; a.gi = sp->gaminv; a.w = sp->w;
+  const double *F[6] = {exg, eyg, ezg, bxg, byg, bzg};
+  for (int f = 0; f < 6; f++) a.F[f] = F[f];
+  a.J[0] = jx; a.J[1] = jy; a.J[2] = jz;
+  GridGeom g;
+  g.nx = (int)grid->nx; g.ny = (int)grid->ny; g.nz = (int)grid->nz;
+  g.gx = (int)grid->ngx; g.gy = (int)grid->ngy; g.gz = (int)grid->ngz;
+  g.NX = grid->nx + 1 + 2 * grid->ngx;
+  g.NY = grid->ny + 1 + 2 * grid->ngy;
+  a.g = g;
+  a.xmin = grid->xmin; a.ymin = grid->ymin; a.zmin = grid->zmin;
+  a.dxi = 1.0 / grid->dx; a.dyi = 1.0 / grid->dy; a.dzi = 1.0 / grid->dz;
+  a.s = 0.5;
+  a.cst = pusher == 1 ? sp->q * dt / sp->m : sp->q * dt * 0.5 / sp->m;
+  a.dt = dt;
+  a.ec = make_esirk_const<double>(sp->q, grid->xmin, grid->ymin, grid->zmin, dt, grid->dx,
+                                  grid->dy, grid->dz);
+  a.q = sp->q; a.dx = grid->dx; a.dy = grid->dy; a.dz = grid->dz;
+  a.bad = rt().d_bad;
+  a.lds_doubles = FUSED_LDS_BYTES / 8;
+  const int nwg = rt().num_cus * 2;
+  dim3 grd(nwg), blk(FUSED_BS);
+  const size_t lds = FUSED_LDS_BYTES;
+  const int key = (int)(order * 100 + (l_lower_order_in_v ? 10 : 0) + pusher * 2 + (depo_algo ? 1 : 0));
+#define PXR_FUSED_CASE(N_, LO_, P_, D_)                                                          \
+  case N_ * 100 + (LO_ ? 10 : 0) + P_ * 2 + (D_ ? 1 : 0):                                        \
+    hipLaunchKernelGGL((k_fused<FUSED_BS, N_, LO_, P_, D_>), grd, blk, lds, st, a);              \
+    break;
+  switch (key) {
+    PXR_FUSED_CASE(1, true, 0, 0) PXR_FUSED_CASE(1, true, 0, 3) PXR_FUSED_CASE(1, true, 1, 0)
+    PXR_FUSED_CASE(1, true, 1, 3) PXR_FUSED_CASE(1, false, 0, 0) PXR_FUSED_CASE(1, false, 0, 3)
+    PXR_FUSED_CASE(1, false, 1, 0) PXR_FUSED_CASE(1, false, 1, 3)
+    PXR_FUSED_CASE(2, true, 0, 0) PXR_FUSED_CASE(2, true, 0, 3) PXR_FUSED_CASE(2, true, 1, 0)
+    PXR_FUSED_CASE(2, true, 1, 3) PXR_FUSED_CASE(2, false, 0, 0) PXR_FUSED_CASE(2, false, 0, 3)
+    PXR_FUSED_CASE(2, false, 1, 0) PXR_FUSED_CASE(2, false, 1, 3)
+    PXR_FUSED_CASE(3, true, 0, 0) PXR_FUSED_CASE(3, true, 0, 3) PXR_FUSED_CASE(3, true, 1, 0)
+    PXR_FUSED_CASE(3, true, 1, 3) PXR_FUSED_CASE(3, false, 0, 0) PXR_FUSED_CASE(3, false, 0, 3)
+    PXR_FUSED_CASE(3, false, 1, 0) PXR_FUSED_CASE(3, false, 1, 3)
+    default:
+      return set_error(PXRHIP_ERR_ARG, "unsupported kernel combination");
+  }
+#undef PXR_FUSED_CASE
+  PXR_LAUNCHED();
+  return PXRHIP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,384 @@
+"""Device-resident PIC state and the reference's per-step sequence on the GPU.
+
+This is the manager layer of PICSAR (the zero-argument routines of
+``picsar_python.picsarpy.picsar`` that act on module state: push_bfield,
+push_efield, efield_bcs, bfield_bcs, current_bcs, particle_bcs,
+particle_sorting_sub, field_gathering_plus_particle_pusher,
+pxrdepose_currents_on_grid_jxjyjz) for one periodic rank.  Fields live in
+Fortran-ordered float64 tensors with guards; each species lives in SoA
+tensors packed by tile (tile geometry of set_tile_split_for_species,
+tiling.F90:94-271).
+
+The step follows submain.F90:99-199:
+
+    gather + push          field_gathering_plus_particle_pusher
+    particle boundaries    particle_bcs            (periodic wrap, re-binning)
+    sorting                particle_sorting_sub
+    current deposition     pxrdepose_currents_on_grid_jxjyjz
+    current guards         current_bcs
+    B half step, E step, B half step with their guard exchanges
+
+On the GPU the gather, push and deposition of a species are one kernel that
+deposits the pushed particles before they are wrapped; the periodic fold of
+the current guards (current_bcs) makes that identical to depositing after the
+wrap, up to rounding.  Re-binning and sorting are one counting sort that runs
+after the field update, so the particles enter the next step's gather
+wrapped, binned and cell-sorted, as in the reference.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from . import _lib
+from .picsarpy import picsar as _pxr
+
+
+def _torch():
+    import torch
+    return torch
+
+
+class PxrGrid(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in ("nx", "ny", "nz", "ngx", "ngy", "ngz", "ntx", "nty",
+                                         "ntz")] + \
+               [(n, C.c_double) for n in ("xmin", "ymin", "zmin", "dx", "dy", "dz",
+                                          "sorting_dx", "sorting_dy", "sorting_dz",
+                                          "sorting_shiftx", "sorting_shifty",
+                                          "sorting_shiftz")]
+
+
+class PxrSpecies(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("x", "y", "z", "ux", "uy", "uz", "gaminv", "w",
+                                          "tile_off", "tile_np", "bin_start")] + \
+               [("capacity", C.c_int64), ("q", C.c_double), ("m", C.c_double)]
+
+
+PUSHERS = {"boris": 0, "vay": 1}
+DEPOS = {"esirkepov": 0, "direct": 3}
+COMPONENTS = ("x", "y", "z", "ux", "uy", "uz", "gaminv", "w")
+
+
+@dataclass
+class _Buffers:
+    arrays: Dict[str, object]
+    tile_off: object
+    tile_np: object
+    bin_start: object
+
+
+@dataclass
+class Species:
+    """One particle species: charge, mass and two SoA buffers (the active one
+    and the target of the next re-binning)."""
+    name: str
+    q: float
+    m: float
+    capacity: int
+    npart: int
+    bufs: List[_Buffers] = field(default_factory=list)
+    cur: int = 0
+    lfreeze: bool = False   # particle_species%lfreeze: not pushed
+    ldodepos: bool = True   # particle_species%ldodepos: deposits current
+
+    @property
+    def active(self) -> _Buffers:
+        return self.bufs[self.cur]
+
+    def arrays(self):
+        """Device tensors of the active buffer, first npart entries."""
+        return {k: v[: self.npart] for k, v in self.active.arrays.items()}
+
+    def struct(self, which: int) -> PxrSpecies:
+        b = self.bufs[which]
+        s = PxrSpecies()
+        for k in COMPONENTS:
+            setattr(s, k, b.arrays[k].data_ptr())
+        s.tile_off = b.tile_off.data_ptr()
+        s.tile_np = b.tile_np.data_ptr()
+        s.bin_start = b.bin_start.data_ptr()
+        s.capacity = self.capacity
+        s.q = float(self.q)
+        s.m = float(self.m)
+        return s
+
+
+class Simulation:
+    """One periodic rank: Yee fields with guards and tiled particle species.
+
+    Units are the reference's -DLIBRARY normalised units (clight = mu0 =
+    eps0 = 1); momenta are gamma*v.  ``dt`` defaults to cfl/sqrt(1/dx^2 +
+    1/dy^2 + 1/dz^2) (submain.F90:581-590 with c = 1).
+    """
+
+    def __init__(self, nx, ny, nz, dx=1.0, dy=1.0, dz=1.0, xmin=0.0, ymin=0.0, zmin=0.0,
+                 ntx=1, nty=1, ntz=1, order=1, l_lower_order_in_v=True, pusher="boris",
+                 depo="esirkepov", norder=2, dt=None, cfl=0.95, nguard=None, sorting_dx=1.0,
+                 sorting_dy=1.0, sorting_dz=1.0, chunk=0, device="cuda"):
+        torch = _torch()
+        _lib.call("pxrhip_init", -1)
+        self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
+        self.dx, self.dy, self.dz = float(dx), float(dy), float(dz)
+        self.xmin, self.ymin, self.zmin = float(xmin), float(ymin), float(zmin)
+        self.order = int(order)
+        if self.order not in (1, 2, 3):
+            raise ValueError("order must be 1, 2 or 3")
+        self.l_lower_order_in_v = bool(l_lower_order_in_v)
+        self.pusher = PUSHERS[pusher]
+        self.depo = DEPOS[depo]
+        self.norder = int(norder)
+        if self.norder < 2 or self.norder % 2:
+            raise ValueError("norder must be even and >= 2")
+        # nxguards = max(nox, 2) (control_file.F90:111-114); the stencil of an
+        # order-n FDTD needs n/2 guard nodes
+        ng = max(self.order, 2, self.norder // 2) if nguard is None else int(nguard)
+        self.ng = ng
+        if dt is None:
+            dt = cfl / math.sqrt(1 / self.dx ** 2 + 1 / self.dy ** 2 + 1 / self.dz ** 2)
+        self.dt = float(dt)
+        self.chunk = int(chunk)
+        self.device = device
+        self.it = 0
+        for n, t in ((self.nx, ntx), (self.ny, nty), (self.nz, ntz)):
+            if (n + 1) // t < 4:
+                raise ValueError("tiles must be at least 4 nodes wide (tiling.F90:110-126)")
+        self.grid = PxrGrid(self.nx, self.ny, self.nz, ng, ng, ng, int(ntx), int(nty), int(ntz),
+                            self.xmin, self.ymin, self.zmin, self.dx, self.dy, self.dz,
+                            sorting_dx * self.dx, sorting_dy * self.dy, sorting_dz * self.dz,
+                            0.0, 0.0, 0.0)
+        self.ntiles = int(ntx) * int(nty) * int(ntz)
+        nb = C.c_int64(0)
+        _lib.call("pxrhip_tiles_nbins", C.byref(self.grid), C.byref(nb))
+        self.nbins = nb.value
+        from .arrays import grid as mkgrid
+        self.fields = {k: mkgrid(self.nx, self.ny, self.nz, ng, device=device)
+                       for k in ("ex", "ey", "ez", "bx", "by", "bz", "jx", "jy", "jz")}
+        if self.norder > 2:
+            c = _fd_weights(self.norder)
+            self.coefs = c
+        else:
+            self.coefs = None
+        self.species: List[Species] = []
+        self.dev = torch.device(device)
+
+    # ------------------------------------------------------------------
+    # particles
+    # ------------------------------------------------------------------
+    def add_species(self, name, q, m, x, y, z, ux, uy, uz, w, gaminv=None, capacity=None,
+                    lfreeze=False, ldodepos=True) -> Species:
+        """Add a species from host (numpy) or device arrays.  gaminv defaults
+        to 1/sqrt(1+u^2) (pxr_set_gamma)."""
+        torch = _torch()
+        n = int(len(x))
+        cap = int(capacity) if capacity is not None else max(n, 1)
+        if cap < n:
+            raise ValueError("capacity < number of particles")
+
+        def dev(a):
+            if isinstance(a, torch.Tensor):
+                return a.to(device=self.dev, dtype=torch.float64).reshape(-1)
+            return torch.as_tensor(np.asarray(a, dtype=np.float64), device=self.dev).reshape(-1)
+
+        src = {k: dev(v) for k, v in zip(("x", "y", "z", "ux", "uy", "uz", "w"),
+                                          (x, y, z, ux, uy, uz, w))}
+        if gaminv is None:
+            src["gaminv"] = 1.0 / torch.sqrt(1.0 + src["ux"] ** 2 + src["uy"] ** 2 + src["uz"] ** 2)
+        else:
+            src["gaminv"] = dev(gaminv)
+        for k, v in src.items():
+            if v.numel() != n:
+                raise ValueError(f"{k}: {v.numel()} values, expected {n}")
+        sp = Species(name=name, q=float(q), m=float(m), capacity=cap, npart=n,
+                     lfreeze=lfreeze, ldodepos=ldodepos)
+        for _ in range(2):
+            arrays = {k: torch.zeros(cap, dtype=torch.float64, device=self.dev) for k in COMPONENTS}
+            sp.bufs.append(_Buffers(
+                arrays=arrays,
+                tile_off=torch.zeros(self.ntiles + 1, dtype=torch.int64, device=self.dev),
+                tile_np=torch.zeros(self.ntiles, dtype=torch.int64, device=self.dev),
+                bin_start=torch.zeros(self.nbins + 1, dtype=torch.int64, device=self.dev)))
+        # stage into buffer 1 as one unsorted run, then bin into buffer 0
+        for k in COMPONENTS:
+            sp.bufs[1].arrays[k][:n].copy_(src[k])
+        sp.bufs[1].tile_off[-1] = n
+        sp.cur = 1
+        self.species.append(sp)
+        self._rebin(sp)
+        self.check()
+        return sp
+
+    def _rebin(self, sp: Species):
+        s_src = sp.struct(sp.cur)
+        s_dst = sp.struct(1 - sp.cur)
+        self._set_stream()
+        _lib.call("pxrhip_tiles_rebin_sort", C.byref(self.grid), C.byref(s_src), C.byref(s_dst))
+        sp.cur = 1 - sp.cur
+
+    # ------------------------------------------------------------------
+    def _set_stream(self):
+        torch = _torch()
+        _lib.call("pxrhip_set_stream", C.c_void_p(torch.cuda.current_stream().cuda_stream))
+
+    def check(self):
+        """Raise if any kernel of the tiled path skipped a particle."""
+        self._set_stream()
+        _lib.call("pxrhip_tiles_check")
+
+    # ------------------------------------------------------------------
+    # the reference's managers
+    # ------------------------------------------------------------------
+    def gather_push_deposit(self):
+        """field_gathering_plus_particle_pusher + pxrdepose_currents_on_grid_jxjyjz
+        for every species (J is zeroed first, current_deposition_manager_3d.F90:247)."""
+        for k in ("jx", "jy", "jz"):
+            self.fields[k].zero_()
+        for sp in self.species:
+            self.gather_push_deposit_species(sp)
+
+    def gather_push_deposit_species(self, sp: Species):
+        """One launch of the fused kernel for one species (J accumulates)."""
+        if sp.lfreeze or sp.npart == 0:
+            return
+        if not sp.ldodepos:
+            raise NotImplementedError("species that do not deposit are not built")
+        f = self.fields
+        self._set_stream()
+        s = sp.struct(sp.cur)
+        _lib.call("pxrhip_tiles_gather_push_deposit", C.byref(self.grid), C.byref(s),
+                  f["ex"].data_ptr(), f["ey"].data_ptr(), f["ez"].data_ptr(), f["bx"].data_ptr(),
+                  f["by"].data_ptr(), f["bz"].data_ptr(), f["jx"].data_ptr(), f["jy"].data_ptr(),
+                  f["jz"].data_ptr(), self.dt, self.order, int(self.l_lower_order_in_v),
+                  self.pusher, self.depo, self.chunk)
+
+    def particle_bcs_and_sorting(self):
+        """particle_bcs (periodic wrap + tile exchange) and particle_sorting_sub."""
+        for sp in self.species:
+            if sp.npart:
+                self._rebin(sp)
+
+    def current_bcs(self):
+        n, g = (self.nx, self.ny, self.nz), self.ng
+        for k in ("jx", "jy", "jz"):
+            _pxr.summation_bcs(self.fields[k], g, g, g, *n)
+
+    def efield_bcs(self):
+        n, g = (self.nx, self.ny, self.nz), self.ng
+        for k in ("ex", "ey", "ez"):
+            _pxr.field_bc(self.fields[k], g, g, g, *n)
+
+    def bfield_bcs(self):
+        n, g = (self.nx, self.ny, self.nz), self.ng
+        for k in ("bx", "by", "bz"):
+            _pxr.field_bc(self.fields[k], g, g, g, *n)
+
+    def push_bfield(self):
+        """push_bfield (maxwell_solver_manager.F90:44): half step, 0.5*dt/dx."""
+        f, g = self.fields, self.ng
+        h = 0.5 * self.dt
+        if self.norder == 2:
+            lo, hi = (0, 0, 0), (self.nx, self.ny, self.nz)
+            alo, ahi = (-g, -g, -g), (self.nx + g, self.ny + g, self.nz + g)
+            _pxr.pxrpush_em3d_bvec(lo, hi, lo, hi, lo, hi,
+                                   *sum(([f[k], alo, ahi] for k in ("ex", "ey", "ez", "bx", "by",
+                                                                     "bz")), []),
+                                   h / self.dx, h / self.dy, h / self.dz)
+        else:
+            c = self.coefs
+            _pxr.pxrpush_em3d_bvec_norder(f["ex"], f["ey"], f["ez"], f["bx"], f["by"], f["bz"],
+                                          h / self.dx * c, h / self.dy * c, h / self.dz * c,
+                                          self.nx, self.ny, self.nz, self.norder, self.norder,
+                                          self.norder, g, g, g, 0, 0, 0, False)
+
+    def push_efield(self):
+        """push_efield (maxwell_solver_manager.F90:393): c^2*dt/dx, mu0*c^2*dt."""
+        f, g = self.fields, self.ng
+        dt = self.dt
+        if self.norder == 2:
+            lo, hi = (0, 0, 0), (self.nx, self.ny, self.nz)
+            alo, ahi = (-g, -g, -g), (self.nx + g, self.ny + g, self.nz + g)
+            _pxr.pxrpush_em3d_evec(lo, hi, lo, hi, lo, hi,
+                                   *sum(([f[k], alo, ahi] for k in ("ex", "ey", "ez", "bx", "by",
+                                                                     "bz", "jx", "jy", "jz")), []),
+                                   dt, dt / self.dx, dt / self.dy, dt / self.dz)
+        else:
+            c = self.coefs
+            _pxr.pxrpush_em3d_evec_norder(f["ex"], f["ey"], f["ez"], f["bx"], f["by"], f["bz"],
+                                          f["jx"], f["jy"], f["jz"], dt, dt / self.dx * c,
+                                          dt / self.dy * c, dt / self.dz * c, self.nx, self.ny,
+                                          self.nz, self.norder, self.norder, self.norder, g, g, g,
+                                          0, 0, 0, False)
+
+    def field_step(self):
+        """submain.F90:160-175: B half, E, B half, each with its guard exchange."""
+        self.push_bfield()
+        self.bfield_bcs()
+        self.push_efield()
+        self.efield_bcs()
+        self.push_bfield()
+        self.bfield_bcs()
+
+    def step(self, nst=1, check=True):
+        for _ in range(int(nst)):
+            self.gather_push_deposit()
+            self.current_bcs()
+            self.field_step()
+            self.particle_bcs_and_sorting()
+            self.it += 1
+        if check:
+            self.check()
+
+    # ------------------------------------------------------------------
+    # diagnostics (diags.F90:847-1114, kin_energy.F90:33-143)
+    # ------------------------------------------------------------------
+    def kinetic_energy(self) -> float:
+        tot = 0.0
+        for sp in self.species:
+            a = sp.arrays()
+            tot += float(((1.0 / a["gaminv"] - 1.0) * a["w"]).sum().item()) * sp.m
+        return tot
+
+    def field_energy(self) -> float:
+        g = self.ng
+        vol = self.dx * self.dy * self.dz
+        e = 0.0
+        for k in ("ex", "ey", "ez", "bx", "by", "bz"):
+            v = self.fields[k][g:g + self.nx, g:g + self.ny, g:g + self.nz]
+            e += float((v * v).sum().item())
+        return 0.5 * e * vol
+
+    def total_particles(self) -> int:
+        return sum(sp.npart for sp in self.species)
+
+
+def _fd_weights(norder: int) -> np.ndarray:
+    """FD_weights (submain.F90:927-984) for a staggered grid, as the
+    reference computes them (including its column update inside the IF)."""
+    n = norder + 1
+    z = 0.5
+    m = 1
+    x = np.array([(i - n // 2 + 1) * 1.0 for i in range(n)])
+    c = np.zeros((m + 1, n))
+    c1 = 1.0
+    c4 = x[0] - z
+    c[0, 0] = 1.0
+    for i in range(1, n):
+        mn = min(i + 1, m + 1)
+        c2 = 1.0
+        c5 = c4
+        c4 = x[i] - z
+        for j in range(i):
+            c3 = x[i] - x[j]
+            c2 = c2 * c3
+            if j == i - 1:
+                for k in range(1, mn):
+                    c[k, i] = c1 * (k * c[k - 1, i - 1] - c5 * c[k, i - 1]) / c2
+                c[0, i] = -c1 * c5 * c[0, i - 1] / c2
+                for k in range(1, mn):
+                    c[k, j] = (c4 * c[k, j] - k * c[k - 1, j]) / c3
+                c[0, j] = c4 * c[0, j] / c3
+        c1 = c2
+    return np.array([c[m, norder // 2 + i - 1] for i in range(1, norder // 2 + 1)])

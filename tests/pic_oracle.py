@@ -1,0 +1,136 @@
+"""CPU oracle of one PIC step on one periodic rank (test infrastructure).
+
+Composes the oracle's restated leaf routines in the reference's step order
+(submain.F90:99-199) on global arrays: gather at the current positions, push,
+position push, deposition of the pushed particles, periodic fold of the
+current guards (current_bcs), B half step, E step, B half step with their
+guard exchanges, and the periodic wrap of the positions (particle_bcs).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+
+class OracleState:
+    def __init__(self, orc, nx, ny, nz, ng, dx, dy, dz, xmin, ymin, zmin, dt, order, lo,
+                 pusher, depo, norder=2, coefs=None):
+        self.orc = orc
+        self.nx, self.ny, self.nz, self.ng = nx, ny, nz, ng
+        self.dx, self.dy, self.dz = dx, dy, dz
+        self.xmin, self.ymin, self.zmin = xmin, ymin, zmin
+        self.dt = dt
+        self.order, self.lo, self.pusher, self.depo = order, lo, pusher, depo
+        self.norder = norder
+        self.coefs = coefs
+        shape = (nx + 1 + 2 * ng, ny + 1 + 2 * ng, nz + 1 + 2 * ng)
+        self.f = {k: np.zeros(shape, order="F") for k in
+                  ("ex", "ey", "ez", "bx", "by", "bz", "jx", "jy", "jz")}
+        self.species = []
+
+    def add_species(self, q, m, x, y, z, ux, uy, uz, w):
+        g = 1.0 / np.sqrt(1.0 + ux ** 2 + uy ** 2 + uz ** 2)
+        self.species.append(dict(q=q, m=m, x=x.copy(), y=y.copy(), z=z.copy(), ux=ux.copy(),
+                                 uy=uy.copy(), uz=uz.copy(), gaminv=g, w=w.copy()))
+
+    def geom(self):
+        return (self.xmin, self.ymin, self.zmin)
+
+    def step(self):
+        o, f = self.orc, self.f
+        nx, ny, nz, ng = self.nx, self.ny, self.nz, self.ng
+        d = (self.dx, self.dy, self.dz)
+        dt = self.dt
+        for s in self.species:
+            n = len(s["x"])
+            e = [np.zeros(n) for _ in range(6)]
+            bad = o.geteb3d_energy_conserving(s["x"], s["y"], s["z"], *e, *self.geom(), *d, nx,
+                                              ny, nz, ng, ng, ng, self.order, self.order,
+                                              self.order, f["ex"], f["ey"], f["ez"], f["bx"],
+                                              f["by"], f["bz"], self.lo, False)
+            assert bad == 0
+            u = (s["ux"], s["uy"], s["uz"])
+            if self.pusher == "vay":
+                o.ebcancelpush3d(*u, s["gaminv"], *e, s["q"], s["m"], dt, 0)
+            else:
+                o.boris_push_u_3d(*u, s["gaminv"], *e, s["q"], s["m"], dt)
+            o.pushxyz(s["x"], s["y"], s["z"], *u, s["gaminv"], dt)
+        for k in ("jx", "jy", "jz"):
+            f[k][...] = 0.0
+        for s in self.species:
+            args = (f["jx"], f["jy"], f["jz"], s["x"], s["y"], s["z"], s["ux"], s["uy"], s["uz"],
+                    s["gaminv"], s["w"], s["q"], *self.geom(), dt, *d, nx, ny, nz, ng, ng, ng,
+                    self.order)
+            bad = o.depose_esirkepov(*args) if self.depo == "esirkepov" else \
+                o.depose_direct(*args, False)
+            assert bad == 0
+        for k in ("jx", "jy", "jz"):
+            o.current_bc_periodic(f[k], nx, ny, nz, ng, ng, ng)
+        self._bhalf()
+        self._bcs(("bx", "by", "bz"))
+        self._efull()
+        self._bcs(("ex", "ey", "ez"))
+        self._bhalf()
+        self._bcs(("bx", "by", "bz"))
+        L = (nx * self.dx, ny * self.dy, nz * self.dz)
+        x0 = self.geom()
+        for s in self.species:
+            for a, k in enumerate(("x", "y", "z")):
+                v = s[k]
+                lo = v < x0[a]
+                hi = v >= x0[a] + L[a]
+                v[lo] = v[lo] + L[a]
+                v[hi] = v[hi] - L[a]
+
+    def _bcs(self, keys):
+        for k in keys:
+            self.orc.field_bc_periodic(self.f[k], self.nx, self.ny, self.nz, self.ng, self.ng,
+                                       self.ng)
+
+    def _bhalf(self):
+        f, h = self.f, 0.5 * self.dt
+        if self.norder == 2:
+            lo, hi = (0, 0, 0), (self.nx, self.ny, self.nz)
+            self.orc.push_em3d_bvec(lo, hi, f["ex"], f["ey"], f["ez"], f["bx"], f["by"], f["bz"],
+                                    h / self.dx, h / self.dy, h / self.dz, self.nx, self.ny,
+                                    self.nz, self.ng, self.ng, self.ng)
+        else:
+            c = self.coefs
+            self.orc.push_em3d_bvec_norder(f["ex"], f["ey"], f["ez"], f["bx"], f["by"], f["bz"],
+                                           h / self.dx * c, h / self.dy * c, h / self.dz * c,
+                                           self.nx, self.ny, self.nz, self.norder, self.norder,
+                                           self.norder, self.ng, self.ng, self.ng, 0, 0, 0)
+
+    def _efull(self):
+        f, dt = self.f, self.dt
+        if self.norder == 2:
+            lo, hi = (0, 0, 0), (self.nx, self.ny, self.nz)
+            self.orc.push_em3d_evec(lo, hi, f["ex"], f["ey"], f["ez"], f["bx"], f["by"], f["bz"],
+                                    f["jx"], f["jy"], f["jz"], dt, dt / self.dx, dt / self.dy,
+                                    dt / self.dz, self.nx, self.ny, self.nz, self.ng, self.ng,
+                                    self.ng)
+        else:
+            c = self.coefs
+            self.orc.push_em3d_evec_norder(f["ex"], f["ey"], f["ez"], f["bx"], f["by"], f["bz"],
+                                           f["jx"], f["jy"], f["jz"], dt, dt / self.dx * c,
+                                           dt / self.dy * c, dt / self.dz * c, self.nx, self.ny,
+                                           self.nz, self.norder, self.norder, self.norder,
+                                           self.ng, self.ng, self.ng, 0, 0, 0)
+
+
+def thermal_plasma(rng, nx, ny, nz, ppc, dx, dy, dz, xmin, ymin, zmin, uth, id_offset=0):
+    """Uniform random positions, Maxwellian momenta (normal per component);
+    weights carry a unique id in their low digits so that particles can be
+    matched after re-ordering."""
+    n = nx * ny * nz * ppc
+    x = xmin + rng.uniform(0, nx, n) * dx
+    y = ymin + rng.uniform(0, ny, n) * dy
+    z = zmin + rng.uniform(0, nz, n) * dz
+    ux, uy, uz = (rng.standard_normal(n) * uth for _ in range(3))
+    w = 1.0 + (np.arange(n) + id_offset) * 1e-9
+    return x, y, z, ux, uy, uz, w
+
+
+def default_dt(dx, dy, dz, cfl=0.95):
+    return cfl / math.sqrt(1 / dx ** 2 + 1 / dy ** 2 + 1 / dz ** 2)

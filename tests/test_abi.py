@@ -41,6 +41,23 @@ def test_python_binding_covers_header(lib_path):
     assert set(declared()) <= bound
 
 
+def declared_arity():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    out = {}
+    for m in re.finditer(r"\b(pxrhip_\w+)\s*\(([^)]*)\)\s*;", txt):
+        args = m.group(2).strip()
+        out[m.group(1)] = 0 if args in ("", "void") else args.count(",") + 1
+    return out
+
+
+def test_python_argument_counts_match_header(lib_path):
+    from picsar_amd import _lib
+    for name, n in declared_arity().items():
+        if name in _lib.SIGNATURES:
+            assert len(_lib.SIGNATURES[name]) == n, (name, len(_lib.SIGNATURES[name]), n)
+
+
 def test_library_is_gfx950(lib_path):
     data = open(lib_path, "rb").read()
     assert b"gfx950" in data
