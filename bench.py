@@ -4,7 +4,9 @@
 Workload (the plain FP64 GPU configuration): a periodic thermal plasma on a
 128^3 grid, dx = dy = dz = 1, dt = 0.95 CFL, electrons and ions at 16
 particles per cell each (67,108,864 macro-particles), Maxwellian momenta
-(u_th = 0.01 c and 0.001 c), order-3 energy-conserving gather, Vay pusher,
+(u_th = 0.01 c and 0.001 c), weights giving an electron plasma frequency
+0.02 (Debye length 0.5 cell, w_pe*dt = 0.011), order-3 energy-conserving
+gather, Vay pusher,
 order-3 Esirkepov deposition, 2nd-order Yee FDTD, 16^3-node tiles, FP64.
 Synthetic data generated on the device from a fixed seed.
 
@@ -33,85 +35,88 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_PARTICLE_FP64 = 120  # read x,y,z,ux,uy,uz,gaminv,w; write x..gaminv
 BYTES_PER_CELL = 72            # read E, B (48 B); write J (24 B)
+OMEGA_PE = 0.02                # electron plasma frequency (c = dx = 1)
 
 
 def build_workload(args):
-    import torch
     from picsar_amd.sim import Simulation
     n = args.n
     sim = Simulation(n, n, n, 1.0, 1.0, 1.0, 0.0, 0.0, 0.0, args.tiles, args.tiles, args.tiles,
                      order=3, l_lower_order_in_v=True, pusher="vay", depo="esirkepov", norder=2,
                      cfl=0.95, chunk=args.chunk)
-    gen = torch.Generator(device="cuda")
-    gen.manual_seed(12345)
-    ncell = n ** 3
+    rng = np.random.default_rng(12345)
+    npart = n ** 3 * args.ppc
     for name, q, m, uth in (("electrons", -1.0, 1.0, 0.01), ("ions", 1.0, 1836.0, 0.001)):
-        npart = ncell * args.ppc
-        pos = [torch.rand(npart, generator=gen, device="cuda", dtype=torch.float64) * n
-               for _ in range(3)]
-        mom = [torch.randn(npart, generator=gen, device="cuda", dtype=torch.float64) * uth
-               for _ in range(3)]
-        w = torch.ones(npart, device="cuda", dtype=torch.float64)
+        pos = [rng.random(npart) * n for _ in range(3)]
+        mom = [rng.standard_normal(npart) * uth for _ in range(3)]
+        w = np.full(npart, OMEGA_PE ** 2 / args.ppc)
         sim.add_species(name, q, m, *pos, *mom, w)
         del pos, mom, w
-    torch.cuda.synchronize()
     return sim
 
 
 def run_gpu(args):
-    import torch
+    from picsar_amd import _lib
+    from picsar_amd.devmem import synchronize
+    import ctypes
+
+    def elapsed(a, b):
+        v = ctypes.c_double(0.0)
+        _lib.call("pxrhip_timer_elapsed", a, b, ctypes.byref(v))
+        return v.value
+
     sim = build_workload(args)
     N = sim.total_particles()
     for _ in range(args.warmup):
         sim.step(1, check=False)
     sim.check()
-    torch.cuda.synchronize()
-    ev = lambda: torch.cuda.Event(enable_timing=True)  # noqa: E731
-    stage_ms = {"gather_push_deposit": 0.0, "current_bcs": 0.0, "fdtd": 0.0, "rebin_sort": 0.0}
-    fused_ms = []
+    synchronize()
+    ke0, fe0 = sim.kinetic_energy(), sim.field_energy()
+    K = args.steps
+    nsp = len(sim.species)
+    # timer slots per step: 0 start, 1..2*nsp fused launches, then stage ends
+    per = 2 * nsp + 5
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        e = [ev() for _ in range(6)]
-        e[0].record()
-        f = sim.fields
+    for s in range(K):
+        b = s * per
+        _lib.call("pxrhip_timer_record", b)
         for k in ("jx", "jy", "jz"):
-            f[k].zero_()
-        for sp in sim.species:
-            a, b = ev(), ev()
-            a.record()
+            sim.fields[k].fill(0.0)
+        for i, sp in enumerate(sim.species):
+            _lib.call("pxrhip_timer_record", b + 1 + 2 * i)
             sim.gather_push_deposit_species(sp)
-            b.record()
-            fused_ms.append((a, b))
-        e[1].record()
+            _lib.call("pxrhip_timer_record", b + 2 + 2 * i)
+        _lib.call("pxrhip_timer_record", b + 2 * nsp + 1)
         sim.current_bcs()
-        e[2].record()
+        _lib.call("pxrhip_timer_record", b + 2 * nsp + 2)
         sim.field_step()
-        e[3].record()
+        _lib.call("pxrhip_timer_record", b + 2 * nsp + 3)
         sim.particle_bcs_and_sorting()
-        e[4].record()
+        _lib.call("pxrhip_timer_record", b + 2 * nsp + 4)
         sim.it += 1
-        stage_ms_step = e
-        stage_ms.setdefault("_events", []).append(stage_ms_step)
-    torch.cuda.synchronize()
+    synchronize()
     wall = time.perf_counter() - t0
     sim.check()
-    evs = stage_ms.pop("_events")
-    for e in evs:
-        stage_ms["gather_push_deposit"] += e[0].elapsed_time(e[1])
-        stage_ms["current_bcs"] += e[1].elapsed_time(e[2])
-        stage_ms["fdtd"] += e[2].elapsed_time(e[3])
-        stage_ms["rebin_sort"] += e[3].elapsed_time(e[4])
-    gpu_ms = sum(stage_ms.values())
-    launches = [a.elapsed_time(b) for a, b in fused_ms]
+    stage = {"gather_push_deposit": 0.0, "current_bcs": 0.0, "fdtd": 0.0, "rebin_sort": 0.0}
+    launches = []
+    total_ms = 0.0
+    for s in range(K):
+        b = s * per
+        stage["gather_push_deposit"] += elapsed(b, b + 2 * nsp + 1)
+        stage["current_bcs"] += elapsed(b + 2 * nsp + 1, b + 2 * nsp + 2)
+        stage["fdtd"] += elapsed(b + 2 * nsp + 2, b + 2 * nsp + 3)
+        stage["rebin_sort"] += elapsed(b + 2 * nsp + 3, b + 2 * nsp + 4)
+        total_ms += elapsed(b, b + 2 * nsp + 4)
+        for i in range(nsp):
+            launches.append(elapsed(b + 1 + 2 * i, b + 2 + 2 * i))
     per_launch_ms = float(np.mean(launches))
-    npart_launch = N / len(sim.species)
+    npart_launch = N / nsp
     ncells = sim.nx * sim.ny * sim.nz
     bytes_launch = npart_launch * BYTES_PER_PARTICLE_FP64 + ncells * BYTES_PER_CELL
     achieved = bytes_launch / (per_launch_ms * 1e-3) / 1e9
-    K = args.steps
-    return dict(sim=sim, N=N, wall=wall, gpu_ms=gpu_ms, stage_ms={k: v / K for k, v in stage_ms.items()},
+    return dict(sim=sim, N=N, wall=wall, gpu_ms=total_ms, stage_ms={k: v / K for k, v in stage.items()},
                 per_launch_ms=per_launch_ms, achieved=achieved, bytes_launch=bytes_launch,
-                ke=sim.kinetic_energy(), fe=sim.field_energy())
+                ke0=ke0, fe0=fe0, ke=sim.kinetic_energy(), fe=sim.field_energy())
 
 
 # ---------------------------------------------------------------------------
@@ -196,12 +201,28 @@ def main():
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
+                    help="launch parameter of the tiled kernels (pxrhip_set_tuning)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="after the timed region, copy 1 GiB with one plain device copy "
+                         "kernel (the reference point of HBM traffic counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(this build runs the path on one MI355X; see DESIGN.md for how it would shard)",
               file=sys.stderr)
         sys.exit(2)
+    from picsar_amd import _lib
+    _lib.call("pxrhip_init", 0)
+    tuning = {}
+    for kv in args.tune:
+        k, v = kv.split("=")
+        _lib.call("pxrhip_set_tuning", k.encode(), int(v))
+    for k in ("fused.lds_bytes", "fused.wg_per_cu", "fused.spread", "fused.chunk"):
+        import ctypes
+        v = ctypes.c_int64(0)
+        _lib.call("pxrhip_get_tuning", k.encode(), ctypes.byref(v))
+        tuning[k] = v.value
     r = run_gpu(args)
     K = args.steps
     value = r["N"] * K / (r["gpu_ms"] * 1e-3)
@@ -222,7 +243,7 @@ def main():
         "config": {"workload": f"thermal plasma {args.n}^3 periodic, {args.ppc} ppc electrons + "
                                f"ions, order-3 gather, Vay, order-3 Esirkepov, Yee, "
                                f"{(args.n + 1) // args.tiles}^3-node tiles, FP64",
-                   "particles": r["N"], "cells": args.n ** 3},
+                   "particles": r["N"], "cells": args.n ** 3, "tuning": tuning},
         "stages": {"ms_per_step": r["stage_ms"],
                    "gather_push_deposit_particle_steps_per_s": gpd,
                    "wall_s_timed_region": r["wall"]},
@@ -231,8 +252,21 @@ def main():
                      "kernel": "k_fused (gather+push+deposit, one species per launch)",
                      "launch_ms": r["per_launch_ms"],
                      "bytes_per_launch": r["bytes_launch"]},
-        "energy": {"kinetic": r["ke"], "field": r["fe"]},
+        "energy": {"kinetic_start": r["ke0"], "field_start": r["fe0"], "kinetic_end": r["ke"],
+                   "field_end": r["fe"],
+                   "relative_drift": (r["ke"] + r["fe"] - r["ke0"] - r["fe0"]) /
+                   (r["ke0"] + r["fe0"])},
     }
+    if args.calibrate_copy:
+        from picsar_amd.devmem import DeviceArray, synchronize
+        a = DeviceArray(1 << 27)
+        b = DeviceArray(1 << 27)
+        a.fill(1.0)
+        synchronize()
+        b.copy_from(a)
+        synchronize()
+        out["calibration_copy_bytes"] = 2 * a.nbytes
+        del a, b
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(out))

@@ -53,6 +53,32 @@ int pxrhip_version(void);
 /* number of particles skipped as out of range since the last reset */
 int pxrhip_bad_count(long long *count, int reset);
 
+/* ---- device memory, timers (no counterpart in the reference) ------------- */
+/* The library runs on its own HIP runtime and stream.  Hosts that hold no
+ * other GPU runtime allocate and move data here; copies to the host and
+ * from pageable host memory synchronise the stream. */
+int pxrhip_malloc(void **ptr, int64_t bytes);
+int pxrhip_free(void *ptr);
+int pxrhip_memcpy_htod(void *dst, const void *src, int64_t bytes);
+int pxrhip_memcpy_dtoh(void *dst, const void *src, int64_t bytes);
+int pxrhip_memcpy_dtod(void *dst, const void *src, int64_t bytes);
+int pxrhip_fill_f64(double *p, int64_t n, double value);
+int pxrhip_fill_i64(int64_t *p, int64_t n, int64_t value);
+/* HIP events on the library stream, the device counterpart of the
+ * reference's localtimes() wall-clock slots (modules.F90:918-941) */
+int pxrhip_timer_record(int slot);
+int pxrhip_timer_elapsed(int start, int stop, double *ms);
+
+/* ---- diagnostics --------------------------------------------------------- */
+/* sum over np particles of (1/gaminv - 1)*w; times m*c^2 this is
+ * compute_kin_energy (kin_energy.F90:33-143) for one species */
+int pxrhip_kinetic_energy(int64_t np, const double *gaminv, const double *w,
+                          double *out);
+/* sum of f^2 over the nodes 0..n-1 of each axis; times dV/2 this is one
+ * component of the field energy of diags.F90:1072-1114 */
+int pxrhip_field_energy(const double *f, int64_t nx, int64_t ny, int64_t nz,
+                        int64_t ngx, int64_t ngy, int64_t ngz, double *out);
+
 /* ---- field gathering ---------------------------------------------------- */
 /* geteb3d_energy_conserving, field_gathering_manager_3d.F90:295-351.
  * nox = noy = noz in {1,2,3}; adds the Yee-staggered E and B at each
@@ -232,6 +258,18 @@ int pxrhip_tiles_gather_push_deposit(
  * Particles of one cell keep no particular order among themselves. */
 int pxrhip_tiles_rebin_sort(const pxrhip_grid *grid, pxrhip_species *src,
                             pxrhip_species *dst);
+
+/* Launch parameters of the tiled kernels.  No counterpart in the reference
+ * (its CPU blocking factors lvec_* play this role).  Keys:
+ * "fused.lds_bytes" (dynamic LDS per workgroup), "fused.wg_per_cu",
+ * "fused.spread" (0/1, lane-spread particle order in the deposition phase),
+ * "fused.still" (0/1, fixed-box Esirkepov path for particles that stay in
+ * their cell), "fused.chunk" (default particles per work item),
+ * "fused.j_lds_bytes" (largest current block accumulated in LDS; negative:
+ * all of fused.lds_bytes).  Work items whose field or current block exceeds
+ * its LDS budget are completed from global memory by a second kernel. */
+int pxrhip_set_tuning(const char *key, int64_t value);
+int pxrhip_get_tuning(const char *key, int64_t *value);
 
 /* Reads the device-side error counter of the tiled path (particles found
  * outside the arrays and skipped) and resets it; a non-zero count is

@@ -11,7 +11,9 @@ import os
 import threading
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libpxrhip.so")
+# PXR_LIB overrides the location of the shared library (a build placed
+# elsewhere); the default is the in-tree build next to this file
+LIB_PATH = os.environ.get("PXR_LIB") or os.path.join(HERE, "libpxrhip.so")
 
 _i64 = C.c_int64
 _f64 = C.c_double
@@ -31,6 +33,17 @@ SIGNATURES = {
     "pxrhip_set_checks": [_i32],
     "pxrhip_version": [],
     "pxrhip_bad_count": [C.POINTER(C.c_longlong), _i32],
+    "pxrhip_malloc": [C.POINTER(C.c_void_p), _I],
+    "pxrhip_free": [_P],
+    "pxrhip_memcpy_htod": [_P, _P, _I],
+    "pxrhip_memcpy_dtoh": [_P, _P, _I],
+    "pxrhip_memcpy_dtod": [_P, _P, _I],
+    "pxrhip_fill_f64": [_P, _I, _F],
+    "pxrhip_fill_i64": [_P, _I, _I],
+    "pxrhip_timer_record": [_i32],
+    "pxrhip_timer_elapsed": [_i32, _i32, C.POINTER(C.c_double)],
+    "pxrhip_kinetic_energy": [_I, _P, _P, C.POINTER(C.c_double)],
+    "pxrhip_field_energy": [_P] + [_I] * 6 + [C.POINTER(C.c_double)],
     "pxrhip_geteb3d_energy_conserving": [_I] + [_P] * 9 + [_F] * 6 + [_I] * 9 + [_P] * 6 + [_I] * 5,
     "pxrhip_boris_push_u_3d": [_I] + [_P] * 10 + [_F] * 3,
     "pxrhip_ebcancelpush3d": [_I] + [_P] * 10 + [_F] * 3 + [_I],
@@ -46,6 +59,8 @@ SIGNATURES = {
     "pxrhip_particle_bin_sorting": [_I] + [_P] * 8 + [_I] + [_F] * 9,
     "pxrhip_tiles_nbins": [_P, _P],
     "pxrhip_tiles_check": [],
+    "pxrhip_set_tuning": [C.c_char_p, _I],
+    "pxrhip_get_tuning": [C.c_char_p, C.POINTER(C.c_int64)],
     "pxrhip_tiles_gather_push_deposit": [_P, _P] + [_P] * 6 + [_P] * 3 + [_F] + [_I] * 5,
     "pxrhip_tiles_rebin_sort": [_P, _P, _P],
 }

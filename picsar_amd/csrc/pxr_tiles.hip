@@ -373,25 +373,25 @@ struct Chunk {
 };
 
 struct ChunkArgs {
-  int ntiles, ch;
+  int ntiles, ch, nzmax;  // nzmax: largest number of sorting planes of a tile
   const int64_t *tile_off, *tile_np, *bin_start;
   const int *n3;
   const int64_t *bin_base;
-  int64_t *count;  // per tile, then scanned in place
+  int *count;       // chunks per unit (tile, plane)
+  i64 *start;       // exclusive scan of count, nunits+1 entries
   Chunk *out;
   int *nchunks, *queue;
   i64 maxchunks;
 };
 
-// Chunks are runs of whole sorting rows (one y-row of cells of one z-plane)
-// of about `ch` particles: a run never grows past ch, and once it holds ch/2
-// it ends at a z-plane boundary, so its cells form a compact block.  Rows
-// denser than ch are cut.  Without sorting information a tile is cut into
-// runs of ch particles.
+// A unit is one z-plane of the sorting grid of one tile.  Its chunks are
+// runs of whole rows (cells of one y-row) of at most `ch` particles; rows
+// denser than that are cut.  A chunk therefore spans a compact block of
+// cells: the tile's width in x, a few rows in y, one plane in z.  Without
+// sorting information the tile is one unit cut into runs of ch particles.
 template <bool WRITE>
-PXR_DEV int64_t walk_chunks(const ChunkArgs &a, int t, int64_t pos) {
-  const int64_t off = a.tile_off[t], np = a.tile_np[t];
-  int64_t nch = 0;
+PXR_DEV int walk_unit(const ChunkArgs &a, int t, int z, i64 pos) {
+  int nch = 0;
   auto emit = [&](int64_t b, int64_t n) {
     if (WRITE && pos + nch < a.maxchunks) {
       Chunk c;
@@ -402,52 +402,51 @@ PXR_DEV int64_t walk_chunks(const ChunkArgs &a, int t, int64_t pos) {
     }
     nch++;
   };
+  const int64_t off = a.tile_off[t], np = a.tile_np[t];
   if (np <= 0) return 0;
   if (!a.bin_start) {
+    if (z != 0) return 0;
     for (int64_t b = 0; b < np; b += a.ch) emit(off + b, min((int64_t)a.ch, np - b));
     return nch;
   }
   const int nx3 = a.n3[3 * t], ny3 = a.n3[3 * t + 1], nz3 = a.n3[3 * t + 2];
-  const int64_t base = a.bin_base[t];
-  int64_t cb = off, cnt = 0;
-  int zc = -1;
-  for (int z = 0; z < nz3; z++)
-    for (int y = 0; y < ny3; y++) {
-      const int64_t row = base + ((int64_t)z * ny3 + y) * nx3;
-      const int64_t rb = a.bin_start[row], rn = a.bin_start[row + nx3] - rb;
-      if (rn <= 0) continue;
-      if (cnt > 0 && (cnt + rn > a.ch || (z != zc && cnt >= a.ch / 2))) {
-        emit(cb, cnt);
-        cnt = 0;
-      }
-      if (cnt == 0) {
-        cb = rb;
-        zc = z;
-      }
-      cnt += rn;
-      while (cnt > a.ch) {
-        emit(cb, a.ch);
-        cb += a.ch;
-        cnt -= a.ch;
-      }
+  if (z >= nz3) return 0;
+  const int64_t base = a.bin_base[t] + (int64_t)z * ny3 * nx3;
+  int64_t cb = 0, cnt = 0;
+  for (int y = 0; y < ny3; y++) {
+    const int64_t row = base + (int64_t)y * nx3;
+    const int64_t rb = a.bin_start[row], rn = a.bin_start[row + nx3] - rb;
+    if (rn <= 0) continue;
+    if (cnt > 0 && cnt + rn > a.ch) {
+      emit(cb, cnt);
+      cnt = 0;
     }
+    if (cnt == 0) cb = rb;
+    cnt += rn;
+    while (cnt > a.ch) {
+      emit(cb, a.ch);
+      cb += a.ch;
+      cnt -= a.ch;
+    }
+  }
   if (cnt > 0) emit(cb, cnt);
   return nch;
 }
 
 __global__ __launch_bounds__(256) void k_chunk_count(ChunkArgs a) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < a.ntiles) a.count[t] = walk_chunks<false>(a, t, 0);
+  const i64 u = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u < (i64)a.ntiles * a.nzmax) a.count[u] = walk_unit<false>(a, (int)(u / a.nzmax), (int)(u % a.nzmax), 0);
 }
 
 __global__ __launch_bounds__(256) void k_chunk_write(ChunkArgs a) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < a.ntiles) walk_chunks<true>(a, t, a.count[t]);
-  if (t == 0) {
-    // a.count[ntiles] holds the total after the scan
-    const i64 tot = a.count[a.ntiles];
+  const i64 nu = (i64)a.ntiles * a.nzmax;
+  const i64 u = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u < nu) walk_unit<true>(a, (int)(u / a.nzmax), (int)(u % a.nzmax), a.start[u]);
+  if (u == 0) {
+    const i64 tot = a.start[nu];
     *a.nchunks = (int)min(tot, a.maxchunks);
     *a.queue = 0;
+    a.queue[1] = 0;  // overflow count (ctl[2])
   }
 }
 
@@ -469,17 +468,57 @@ struct FusedArgs {
   EsirkConst<double> ec;
   double q, dx, dy, dz;  // direct deposition
   unsigned long long *bad;
-  int lds_doubles;   // dynamic LDS size in doubles
+  int lds_doubles;   // dynamic LDS size in doubles (field blocks)
+  int j_doubles;     // largest current block kept in LDS, in doubles
+  int spread;        // deposit phase: lanes take particles S apart
+  int still;         // deposit phase: fixed-box path for particles that stay in their cell
+  int *ovf_count;    // chunks whose blocks do not fit in LDS ...
+  int2 *ovf;         // ... (chunk, stage): stage 0 untouched, 1 pushed
+  int ovf_cap;
 };
 
-PXR_DEV void wave_minmax(int *lo, int *hi) {
+struct Box6 {
+  int lo[3], hi[3];
+};
+
+PXR_DEV void box_reset(Box6 &b) {
+  for (int d = 0; d < 3; d++) {
+    b.lo[d] = INT_MAX;
+    b.hi[d] = INT_MIN;
+  }
+}
+
+PXR_DEV void box_add(Box6 &b, const int *lo, const int *hi) {
+  for (int d = 0; d < 3; d++) {
+    b.lo[d] = min(b.lo[d], lo[d]);
+    b.hi[d] = max(b.hi[d], hi[d]);
+  }
+}
+
+PXR_DEV void wave_minmax(Box6 &b) {
 #pragma unroll
   for (int d = 0; d < 3; d++) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
-      lo[d] = min(lo[d], __shfl_xor(lo[d], o));
-      hi[d] = max(hi[d], __shfl_xor(hi[d], o));
+      b.lo[d] = min(b.lo[d], __shfl_xor(b.lo[d], o));
+      b.hi[d] = max(b.hi[d], __shfl_xor(b.hi[d], o));
     }
+  }
+}
+
+// node index bounds of the gather stencil, without the weights
+template <int N, int M>
+PXR_DEV void gather_index_bounds(double x, double y, double z, double s, int *lo, int *hi) {
+  const double p[3] = {x, y, z};
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const double v = p[d];
+    const int j = (N == 2 ? f_nint(v) : f_floor(v)) - (N == 1 ? 0 : 1);
+    int j0;
+    if (M == 0 || M == 2) j0 = f_floor((v + 0.5) - s) - (M == 2 ? 1 : 0);
+    else j0 = f_floor(v - s) - (M == 3 ? 1 : 0);
+    lo[d] = min(j, j0);
+    hi[d] = max(j + N, j0 + M);
   }
 }
 
@@ -493,18 +532,93 @@ struct LdsAcc {
 
 struct GAcc {
   double *p[3];
-  const GridGeom *g;
+  GridGeom g;
   PXR_DEV void add(int c, int i, int j, int k, double v) const {
-    unsafeAtomicAdd(p[c] + g->idx(i, j, k), v);
+    unsafeAtomicAdd(p[c] + g.idx(i, j, k), v);
   }
 };
 
+// Gather, push and position push of particle p with the field views v;
+// extends the box of the current it will deposit.
+template <int N, bool LO, int PUSHER, int DEPO, typename V>
+PXR_DEV void gpd_push_particle(const FusedArgs &a, i64 p, const V *v, Box6 &dep) {
+  constexpr int M = LO ? N - 1 : N;
+  const GridGeom &g = a.g;
+  GatherStencil<N, M, double> st;
+  gather_stencil<N, M, double>(st, (a.x[p] - a.xmin) * a.dxi, (a.y[p] - a.ymin) * a.dyi,
+                               (a.z[p] - a.zmin) * a.dzi, a.s);
+  int l2[3], h2[3];
+  gather_bounds<N, M, double>(st, l2, h2);
+  if (!g.inside(l2[0], l2[1], l2[2]) || !g.inside(h2[0], h2[1], h2[2])) {
+    count_bad(a.bad);
+    return;
+  }
+  double E[6];
+  gather_apply<N, M, double>(st, v, E);
+  double u[3] = {a.ux[p], a.uy[p], a.uz[p]};
+  double gi = a.gi[p];
+  if (PUSHER == 1) vay_push<double>(u, gi, E[0], E[1], E[2], E[3], E[4], E[5], a.cst);
+  else boris_push<double>(u, gi, E[0], E[1], E[2], E[3], E[4], E[5], a.cst);
+  // pxr_pushxyz (boris_3d.F90:880-882)
+  const double xn = a.x[p] + u[0] * gi * a.dt;
+  const double yn = a.y[p] + u[1] * gi * a.dt;
+  const double zn = a.z[p] + u[2] * gi * a.dt;
+  a.ux[p] = u[0];
+  a.uy[p] = u[1];
+  a.uz[p] = u[2];
+  a.gi[p] = gi;
+  a.x[p] = xn;
+  a.y[p] = yn;
+  a.z[p] = zn;
+  int b0[3], b1[3];
+  if (DEPO == 0) {
+    EsirkepovIn<double> in;
+    esirkepov_prepare<N, double>(in, xn, yn, zn, u[0], u[1], u[2], gi, a.w[p], a.ec);
+    if (!esirkepov_box<N, double>(in, b0, b1)) return;
+  } else {
+    DirectIn<double> in;
+    direct_prepare<N, double>(in, xn, yn, zn, u[0], u[1], u[2], gi, a.w[p], a.q, a.xmin, a.ymin,
+                              a.zmin, a.dt, a.dx, a.dy, a.dz, a.s);
+    direct_box<N>(in, b0, b1);
+  }
+  box_add(dep, b0, b1);
+}
+
+// Deposit particle p (its pushed state is read back from memory).  Returns
+// false when the particle left the arrays (counted, skipped).
+template <int N, int DEPO, typename Acc>
+PXR_DEV bool gpd_deposit_general(const FusedArgs &a, i64 p, const Acc &acc) {
+  const GridGeom &g = a.g;
+  int b0[3], b1[3];
+  if (DEPO == 0) {
+    EsirkepovIn<double> in;
+    esirkepov_prepare<N, double>(in, a.x[p], a.y[p], a.z[p], a.ux[p], a.uy[p], a.uz[p], a.gi[p],
+                                 a.w[p], a.ec);
+    if (!esirkepov_box<N, double>(in, b0, b1) || !g.inside(b0[0], b0[1], b0[2]) ||
+        !g.inside(b1[0], b1[1], b1[2]))
+      return false;
+    esirkepov_deposit<N, double>(in, acc);
+  } else {
+    DirectIn<double> in;
+    direct_prepare<N, double>(in, a.x[p], a.y[p], a.z[p], a.ux[p], a.uy[p], a.uz[p], a.gi[p],
+                              a.w[p], a.q, a.xmin, a.ymin, a.zmin, a.dt, a.dx, a.dy, a.dz, a.s);
+    direct_box<N>(in, b0, b1);
+    if (!g.inside(b0[0], b0[1], b0[2]) || !g.inside(b1[0], b1[1], b1[2])) return false;
+    direct_deposit<N, double>(in, acc);
+  }
+  return true;
+}
+
+#define SLOW_CAP 512  // particles per chunk deferred to the general Esirkepov path
+
 template <int BS, int N, bool LO, int PUSHER, int DEPO>
-__global__ __launch_bounds__(BS) void k_fused(FusedArgs a) {
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void k_fused(FusedArgs a) {
   constexpr int M = LO ? N - 1 : N;
   extern __shared__ double smem[];
   __shared__ int s_chunk;
   __shared__ int s_box[6];
+  __shared__ int s_nslow;
+  __shared__ int s_slow[SLOW_CAP];
   const GridGeom &g = a.g;
   const int glo[3] = {-g.gx, -g.gy, -g.gz};
   const int ghi[3] = {g.nx + g.gx, g.ny + g.gy, g.nz + g.gz};
@@ -516,6 +630,7 @@ __global__ __launch_bounds__(BS) void k_fused(FusedArgs a) {
         s_box[d] = INT_MAX;
         s_box[3 + d] = INT_MIN;
       }
+      s_nslow = 0;
     }
     __syncthreads();
     const int c = s_chunk;
@@ -523,31 +638,27 @@ __global__ __launch_bounds__(BS) void k_fused(FusedArgs a) {
     const Chunk ch = a.chunks[c];
     const i64 p0 = ch.begin;
     // ---- 1. box of the gather stencils ---------------------------------
-    int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+    Box6 b;
+    box_reset(b);
     for (int k = threadIdx.x; k < ch.n; k += BS) {
       const i64 p = p0 + k;
-      GatherStencil<N, M, double> st;
-      gather_stencil<N, M, double>(st, (a.x[p] - a.xmin) * a.dxi, (a.y[p] - a.ymin) * a.dyi,
-                                   (a.z[p] - a.zmin) * a.dzi, a.s);
       int l2[3], h2[3];
-      gather_bounds<N, M, double>(st, l2, h2);
-      for (int d = 0; d < 3; d++) {
-        lo[d] = min(lo[d], l2[d]);
-        hi[d] = max(hi[d], h2[d]);
-      }
+      gather_index_bounds<N, M>((a.x[p] - a.xmin) * a.dxi, (a.y[p] - a.ymin) * a.dyi,
+                                (a.z[p] - a.zmin) * a.dzi, a.s, l2, h2);
+      box_add(b, l2, h2);
     }
-    wave_minmax(lo, hi);
+    wave_minmax(b);
     if ((threadIdx.x & 63) == 0)
       for (int d = 0; d < 3; d++) {
-        atomicMin(&s_box[d], lo[d]);
-        atomicMax(&s_box[3 + d], hi[d]);
+        atomicMin(&s_box[d], b.lo[d]);
+        atomicMax(&s_box[3 + d], b.hi[d]);
       }
     __syncthreads();
-    int R0[3], R1[3], e[3];
+    int R0[3], e[3];
     for (int d = 0; d < 3; d++) {
       R0[d] = max(s_box[d], glo[d]);
-      R1[d] = min(s_box[3 + d], ghi[d]);
-      e[d] = R1[d] >= R0[d] ? R1[d] - R0[d] + 1 : 0;
+      const int R1 = min(s_box[3 + d], ghi[d]);
+      e[d] = R1 >= R0[d] ? R1 - R0[d] + 1 : 0;
     }
     const i64 nb64 = (i64)e[0] * e[1] * e[2];
     const bool lds_f = nb64 > 0 && 6 * nb64 <= a.lds_doubles;
@@ -569,124 +680,131 @@ __global__ __launch_bounds__(BS) void k_fused(FusedArgs a) {
         s_box[d] = INT_MAX;
         s_box[3 + d] = INT_MIN;
       }
-    // ---- 2. gather, push, position push ---------------------------------
-    int dlo[3] = {INT_MAX, INT_MAX, INT_MAX}, dhi[3] = {INT_MIN, INT_MIN, INT_MIN};
-    for (int k = threadIdx.x; k < ch.n; k += BS) {
-      const i64 p = p0 + k;
-      GatherStencil<N, M, double> st;
-      gather_stencil<N, M, double>(st, (a.x[p] - a.xmin) * a.dxi, (a.y[p] - a.ymin) * a.dyi,
-                                   (a.z[p] - a.zmin) * a.dzi, a.s);
-      int l2[3], h2[3];
-      gather_bounds<N, M, double>(st, l2, h2);
-      if (!g.inside(l2[0], l2[1], l2[2]) || !g.inside(h2[0], h2[1], h2[2])) {
-        count_bad(a.bad);
-        continue;
+    if (!lds_f) {
+      // block too large for LDS: left to the overflow kernel
+      if (threadIdx.x == 0) {
+        const int slot = atomicAdd(a.ovf_count, 1);
+        if (slot < a.ovf_cap) a.ovf[slot] = make_int2(c, 0);
       }
-      double E[6];
-      if (lds_f) {
-        LdsView<double> v[6];
-        for (int f = 0; f < 6; f++)
-          v[f] = LdsView<double>{smem + f * nb, R0[0], R0[1], R0[2], e[0], e[0] * e[1]};
-        gather_apply<N, M, double>(st, v, E);
-      } else {
-        FieldView<double> v[6];
-        for (int f = 0; f < 6; f++) v[f] = FieldView<double>{a.F[f] + g.idx(0, 0, 0), g.NX, g.NX * g.NY};
-        gather_apply<N, M, double>(st, v, E);
-      }
-      double u[3] = {a.ux[p], a.uy[p], a.uz[p]};
-      double gi = a.gi[p];
-      if (PUSHER == 1) vay_push<double>(u, gi, E[0], E[1], E[2], E[3], E[4], E[5], a.cst);
-      else boris_push<double>(u, gi, E[0], E[1], E[2], E[3], E[4], E[5], a.cst);
-      // pxr_pushxyz (boris_3d.F90:880-882)
-      const double xn = a.x[p] + u[0] * gi * a.dt;
-      const double yn = a.y[p] + u[1] * gi * a.dt;
-      const double zn = a.z[p] + u[2] * gi * a.dt;
-      a.ux[p] = u[0];
-      a.uy[p] = u[1];
-      a.uz[p] = u[2];
-      a.gi[p] = gi;
-      a.x[p] = xn;
-      a.y[p] = yn;
-      a.z[p] = zn;
-      int b0[3], b1[3];
-      if (DEPO == 0) {
-        EsirkepovIn<double> in;
-        esirkepov_prepare<N, double>(in, xn, yn, zn, u[0], u[1], u[2], gi, a.w[p], a.ec);
-        if (!esirkepov_box<N, double>(in, b0, b1)) continue;
-      } else {
-        DirectIn<double> in;
-        direct_prepare<N, double>(in, xn, yn, zn, u[0], u[1], u[2], gi, a.w[p], a.q, a.xmin,
-                                  a.ymin, a.zmin, a.dt, a.dx, a.dy, a.dz, a.s);
-        direct_box<N>(in, b0, b1);
-      }
-      for (int d = 0; d < 3; d++) {
-        dlo[d] = min(dlo[d], b0[d]);
-        dhi[d] = max(dhi[d], b1[d]);
-      }
+      __syncthreads();
+      continue;
     }
-    wave_minmax(dlo, dhi);
+    // ---- 2. gather, push, position push ---------------------------------
+    Box6 dep;
+    box_reset(dep);
+    {
+      LdsView<double> v[6];
+      for (int f = 0; f < 6; f++)
+        v[f] = LdsView<double>{smem + f * nb, R0[0], R0[1], R0[2], e[0], e[0] * e[1]};
+      for (int k = threadIdx.x; k < ch.n; k += BS)
+        gpd_push_particle<N, LO, PUSHER, DEPO>(a, p0 + k, v, dep);
+    }
+    wave_minmax(dep);
     __syncthreads();  // field block no longer read; s_box reset visible
     if ((threadIdx.x & 63) == 0)
       for (int d = 0; d < 3; d++) {
-        atomicMin(&s_box[d], dlo[d]);
-        atomicMax(&s_box[3 + d], dhi[d]);
+        atomicMin(&s_box[d], dep.lo[d]);
+        atomicMax(&s_box[3 + d], dep.hi[d]);
       }
     __syncthreads();
     for (int d = 0; d < 3; d++) {
       R0[d] = max(s_box[d], glo[d]);
-      R1[d] = min(s_box[3 + d], ghi[d]);
-      e[d] = R1[d] >= R0[d] ? R1[d] - R0[d] + 1 : 0;
+      const int R1 = min(s_box[3 + d], ghi[d]);
+      e[d] = R1 >= R0[d] ? R1 - R0[d] + 1 : 0;
     }
     const i64 nj64 = (i64)e[0] * e[1] * e[2];
-    const bool lds_j = nj64 > 0 && 3 * nj64 <= a.lds_doubles;
-    const int nj = lds_j ? (int)nj64 : 0;
-    if (lds_j)
-      for (int idx = threadIdx.x; idx < 3 * nj; idx += BS) smem[idx] = 0.0;
+    const bool lds_j = nj64 > 0 && 3 * nj64 <= a.j_doubles;
+    if (!lds_j) {
+      // pushed, but the current block does not fit: deposit in the overflow kernel
+      if (threadIdx.x == 0) {
+        const int slot = atomicAdd(a.ovf_count, 1);
+        if (slot < a.ovf_cap) a.ovf[slot] = make_int2(c, 1);
+      }
+      __syncthreads();
+      continue;
+    }
+    const int nj = (int)nj64;
+    for (int idx = threadIdx.x; idx < 3 * nj; idx += BS) smem[idx] = 0.0;
     __syncthreads();
     // ---- 3. deposit the pushed particles ---------------------------------
-    const LdsAcc lacc{{smem, smem + nj, smem + 2 * nj}, R0[0], R0[1], R0[2], e[0], e[0] * e[1]};
-    const GAcc gacc{{a.J[0], a.J[1], a.J[2]}, &a.g};
-    for (int k = threadIdx.x; k < ch.n; k += BS) {
-      const i64 p = p0 + k;
-      const double xn = a.x[p], yn = a.y[p], zn = a.z[p];
-      const double ux = a.ux[p], uy = a.uy[p], uz = a.uz[p], gi = a.gi[p], w = a.w[p];
-      int b0[3], b1[3];
-      if (DEPO == 0) {
-        EsirkepovIn<double> in;
-        esirkepov_prepare<N, double>(in, xn, yn, zn, ux, uy, uz, gi, w, a.ec);
-        if (!esirkepov_box<N, double>(in, b0, b1) || !g.inside(b0[0], b0[1], b0[2]) ||
-            !g.inside(b1[0], b1[1], b1[2])) {
-          count_bad(a.bad);
-          continue;
+    // lanes of a wave take particles S apart (one or more cells apart in a
+    // cell-sorted chunk) so that their LDS atomics rarely hit one address
+    const int S = (ch.n + 63) >> 6;
+    {
+      const LdsAcc lacc{{smem, smem + nj, smem + 2 * nj}, R0[0], R0[1], R0[2], e[0], e[0] * e[1]};
+      for (int kk = threadIdx.x; kk < S * 64; kk += BS) {
+        const int k = a.spread ? (kk & 63) * S + (kk >> 6) : kk;
+        if (k >= ch.n) continue;
+        const i64 p = p0 + k;
+        if (DEPO == 0 && a.still) {
+          // particles that stay in their cell take the fixed-box path; the
+          // others are deferred and deposited together below
+          EsirkepovIn<double> in;
+          esirkepov_prepare<N, double>(in, a.x[p], a.y[p], a.z[p], a.ux[p], a.uy[p], a.uz[p],
+                                       a.gi[p], a.w[p], a.ec);
+          int i0[3];
+          const double pos[3] = {in.x, in.y, in.z}, old[3] = {in.xo, in.yo, in.zo};
+          bool still = true;
+          for (int q = 0; q < 3; q++) {
+            i0[q] = eindex<N, double>(pos[q]);
+            still = still && eindex<N, double>(old[q]) == i0[q];
+          }
+          still = still && g.inside(i0[0] + ESup<N>::lo, i0[1] + ESup<N>::lo, i0[2] + ESup<N>::lo) &&
+                  g.inside(i0[0] + ESup<N>::hi, i0[1] + ESup<N>::hi, i0[2] + ESup<N>::hi);
+          if (still) {
+            esirkepov_deposit_still<N, double>(in, i0, lacc);
+          } else {
+            const int slot = atomicAdd(&s_nslow, 1);
+            if (slot < SLOW_CAP) s_slow[slot] = k;
+            else if (!gpd_deposit_general<N, DEPO>(a, p, lacc)) count_bad(a.bad);
+          }
+        } else {
+          if (!gpd_deposit_general<N, DEPO>(a, p, lacc)) count_bad(a.bad);
         }
-        if (lds_j) esirkepov_deposit<N, double>(in, lacc);
-        else esirkepov_deposit<N, double>(in, gacc);
-      } else {
-        DirectIn<double> in;
-        direct_prepare<N, double>(in, xn, yn, zn, ux, uy, uz, gi, w, a.q, a.xmin, a.ymin, a.zmin,
-                                  a.dt, a.dx, a.dy, a.dz, a.s);
-        direct_box<N>(in, b0, b1);
-        if (!g.inside(b0[0], b0[1], b0[2]) || !g.inside(b1[0], b1[1], b1[2])) {
-          count_bad(a.bad);
-          continue;
-        }
-        if (lds_j) direct_deposit<N, double>(in, lacc);
-        else direct_deposit<N, double>(in, gacc);
+      }
+      if (DEPO == 0 && a.still) {
+        __syncthreads();
+        const int ns = min(s_nslow, SLOW_CAP);
+        for (int q = threadIdx.x; q < ns; q += BS)
+          if (!gpd_deposit_general<N, DEPO>(a, p0 + s_slow[q], lacc)) count_bad(a.bad);
       }
     }
     __syncthreads();
-    if (lds_j) {
-      for (int idx = threadIdx.x; idx < nj; idx += BS) {
-        const int i = R0[0] + idx % e[0];
-        const int j = R0[1] + (idx / e[0]) % e[1];
-        const int k = R0[2] + idx / (e[0] * e[1]);
-        const i64 gi_ = g.idx(i, j, k);
-        for (int c3 = 0; c3 < 3; c3++) {
-          const double v = smem[c3 * nj + idx];
-          if (v != 0.0) unsafeAtomicAdd(a.J[c3] + gi_, v);
-        }
+    for (int idx = threadIdx.x; idx < nj; idx += BS) {
+      const int i = R0[0] + idx % e[0];
+      const int j = R0[1] + (idx / e[0]) % e[1];
+      const int k = R0[2] + idx / (e[0] * e[1]);
+      const i64 gi_ = g.idx(i, j, k);
+      for (int c3 = 0; c3 < 3; c3++) {
+        const double v = smem[c3 * nj + idx];
+        if (v != 0.0) unsafeAtomicAdd(a.J[c3] + gi_, v);
       }
     }
+    __syncthreads();
+  }
+}
+
+// Chunks whose field or current block did not fit in LDS: gather from
+// global memory and deposit with global atomics, one workgroup per chunk.
+template <int BS, int N, bool LO, int PUSHER, int DEPO>
+__global__ __launch_bounds__(BS) void k_fused_overflow(FusedArgs a) {
+  const int n = min(*a.ovf_count, a.ovf_cap);
+  const GridGeom &g = a.g;
+  for (int o = blockIdx.x; o < n; o += gridDim.x) {
+    const int2 e = a.ovf[o];
+    const Chunk ch = a.chunks[e.x];
+    if (e.y == 0) {
+      FieldView<double> v[6];
+      for (int f = 0; f < 6; f++) v[f] = FieldView<double>{a.F[f] + g.idx(0, 0, 0), g.NX, g.NX * g.NY};
+      Box6 dep;
+      box_reset(dep);
+      for (int k = threadIdx.x; k < ch.n; k += BS)
+        gpd_push_particle<N, LO, PUSHER, DEPO>(a, ch.begin + k, v, dep);
+    }
+    __syncthreads();
+    const GAcc acc{{a.J[0], a.J[1], a.J[2]}, a.g};
+    for (int k = threadIdx.x; k < ch.n; k += BS)
+      if (!gpd_deposit_general<N, DEPO>(a, ch.begin + k, acc)) count_bad(a.bad);
     __syncthreads();
   }
 }
@@ -697,7 +815,19 @@ __global__ __launch_bounds__(BS) void k_fused(FusedArgs a) {
 // C ABI
 // ===========================================================================
 static const int FUSED_BS = 256;
-static const int FUSED_LDS_BYTES = 64 * 1024 - 256;
+
+namespace pxr {
+struct Tuning {
+  int lds_bytes = 79616;  // dynamic LDS per workgroup: two workgroups (with their 2 KiB
+                          // of static LDS) fit in a CU's 160 KiB
+  int wg_per_cu = 2;      // persistent workgroups per CU
+  int spread = 1;         // lane-spread mapping in the deposit phase
+  int still = 1;          // fixed-box Esirkepov path for particles that stay in their cell
+  int chunk = 2048;       // default particles per work item
+  int j_lds_bytes = -1;   // cap on the LDS current block (negative: all of lds_bytes)
+};
+static Tuning g_tune;
+}  // namespace pxr
 
 extern "C" {
 
@@ -707,6 +837,42 @@ int pxrhip_tiles_nbins(const pxrhip_grid *grid, int64_t *nbins) {
   int r = get_table(grid, &tt);
   if (r) return r;
   if (nbins) *nbins = tt->nbins;
+  return PXRHIP_OK;
+}
+
+int pxrhip_set_tuning(const char *key, int64_t value) {
+  if (!key) return set_error(PXRHIP_ERR_ARG, "null tuning key");
+  if (!strcmp(key, "fused.lds_bytes")) {
+    if (value < 4096 || value > 160 * 1024 - 1024)
+      return set_error(PXRHIP_ERR_ARG, "fused.lds_bytes must be in [4096, 159744]");
+    g_tune.lds_bytes = (int)(value & ~7);
+  } else if (!strcmp(key, "fused.wg_per_cu")) {
+    if (value < 1 || value > 16) return set_error(PXRHIP_ERR_ARG, "fused.wg_per_cu must be in [1, 16]");
+    g_tune.wg_per_cu = (int)value;
+  } else if (!strcmp(key, "fused.spread")) {
+    g_tune.spread = value != 0;
+  } else if (!strcmp(key, "fused.still")) {
+    g_tune.still = value != 0;
+  } else if (!strcmp(key, "fused.j_lds_bytes")) {
+    g_tune.j_lds_bytes = value < 0 ? -1 : (value > (1 << 30) ? (1 << 30) : (int)value);
+  } else if (!strcmp(key, "fused.chunk")) {
+    if (value < 1 || value > (1 << 20)) return set_error(PXRHIP_ERR_ARG, "fused.chunk must be in [1, 2^20]");
+    g_tune.chunk = (int)value;
+  } else {
+    return set_error(PXRHIP_ERR_ARG, "unknown tuning key '%s'", key);
+  }
+  return PXRHIP_OK;
+}
+
+int pxrhip_get_tuning(const char *key, int64_t *value) {
+  if (!key || !value) return set_error(PXRHIP_ERR_ARG, "null argument");
+  if (!strcmp(key, "fused.lds_bytes")) *value = g_tune.lds_bytes;
+  else if (!strcmp(key, "fused.wg_per_cu")) *value = g_tune.wg_per_cu;
+  else if (!strcmp(key, "fused.spread")) *value = g_tune.spread;
+  else if (!strcmp(key, "fused.still")) *value = g_tune.still;
+  else if (!strcmp(key, "fused.chunk")) *value = g_tune.chunk;
+  else if (!strcmp(key, "fused.j_lds_bytes")) *value = g_tune.j_lds_bytes;
+  else return set_error(PXRHIP_ERR_ARG, "unknown tuning key '%s'", key);
   return PXRHIP_OK;
 }
 
@@ -813,32 +979,57 @@ int pxrhip_tiles_gather_push_deposit(const pxrhip_grid *grid, const pxrhip_speci
   if (depo_algo != 0 && depo_algo != 3) return set_error(PXRHIP_ERR_ARG, "depo_algo must be 0 or 3");
   if (!exg || !eyg || !ezg || !bxg || !byg || !bzg || !jx || !jy || !jz)
     return set_error(PXRHIP_ERR_ARG, "null field array");
-  const int ch = chunk > 0 ? (int)(chunk < (1 << 20) ? chunk : (1 << 20)) : 1024;
+  const int ch = chunk > 0 ? (int)(chunk < (1 << 20) ? chunk : (1 << 20)) : g_tune.chunk;
   hipStream_t st = rt().stream;
   // chunk table
   const i64 cap = sp->capacity;
-  const i64 maxchunks = tt->nrows + 3 * (cap / ch + 1) + tt->ntiles + 16;
-  int64_t *cnt = (int64_t *)scratch(8, sizeof(int64_t) * (size_t)(tt->ntiles + 1));
+  int nzmax = 1;
+  {
+    // largest number of sorting planes of a tile (host copy of the table)
+    static int cached_nzmax = 0;
+    static const TileTable *cached_for = nullptr;
+    static pxrhip_grid cached_key;
+    if (cached_for != tt || memcmp(&cached_key, &tt->key, sizeof(pxrhip_grid)) != 0) {
+      std::vector<int> n3(3 * (size_t)tt->ntiles);
+      PXR_HIP(hipMemcpy(n3.data(), tt->n3, sizeof(int) * n3.size(), hipMemcpyDeviceToHost));
+      cached_nzmax = 1;
+      for (int t = 0; t < tt->ntiles; t++) cached_nzmax = max(cached_nzmax, n3[3 * t + 2]);
+      cached_for = tt;
+      cached_key = tt->key;
+    }
+    nzmax = cached_nzmax;
+  }
+  const i64 nunits = (i64)tt->ntiles * nzmax;
+  const i64 maxchunks = tt->nrows + (cap / ch + 1) + nunits + 16;
+  const i64 nblk = (nunits + SCAN_B - 1) / SCAN_B;
+  int *cnt = (int *)scratch(8, sizeof(int) * (size_t)nunits);
+  i64 *cstart = (i64 *)scratch(12, sizeof(i64) * (size_t)(nunits + 1));
+  i64 *bsum = (i64 *)scratch(13, sizeof(i64) * (size_t)(nblk + 1));
   Chunk *chunks = (Chunk *)scratch(9, sizeof(Chunk) * (size_t)maxchunks);
-  int *ctl = (int *)scratch(10, 2 * sizeof(int));
-  if (!cnt || !chunks || !ctl) return set_error(PXRHIP_ERR_HIP, "scratch allocation failed");
+  int *ctl = (int *)scratch(10, 4 * sizeof(int));
+  int2 *ovf = (int2 *)scratch(14, sizeof(int2) * (size_t)maxchunks);
+  if (!cnt || !cstart || !bsum || !chunks || !ctl || !ovf) return set_error(PXRHIP_ERR_HIP, "scratch allocation failed");
   ChunkArgs ca;
   ca.ntiles = tt->ntiles;
   ca.ch = ch;
+  ca.nzmax = nzmax;
   ca.tile_off = sp->tile_off;
   ca.tile_np = sp->tile_np;
   ca.bin_start = sp->bin_start;
   ca.n3 = tt->n3;
   ca.bin_base = tt->bin_base;
   ca.count = cnt;
+  ca.start = cstart;
   ca.out = chunks;
   ca.nchunks = ctl;
   ca.queue = ctl + 1;
   ca.maxchunks = maxchunks;
-  hipLaunchKernelGGL(k_chunk_count, dim3(grid1d(tt->ntiles, 256)), dim3(256), 0, st, ca);
-  hipLaunchKernelGGL(k_scan_i64_1wg, dim3(1), dim3(1024), 0, st, (i64 *)cnt, (i64)tt->ntiles + 1,
-                     (i64 *)nullptr);
-  hipLaunchKernelGGL(k_chunk_write, dim3(grid1d(tt->ntiles, 256)), dim3(256), 0, st, ca);
+  hipLaunchKernelGGL(k_chunk_count, dim3(grid1d(nunits, 256)), dim3(256), 0, st, ca);
+  hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nblk), dim3(256), 0, st, cnt, nunits, bsum);
+  hipLaunchKernelGGL(k_scan_i64_1wg, dim3(1), dim3(1024), 0, st, bsum, nblk, (i64 *)nullptr);
+  hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nblk), dim3(256), 0, st, cnt, nunits, bsum,
+                     cstart, (i64 *)nullptr);
+  hipLaunchKernelGGL(k_chunk_write, dim3(grid1d(nunits, 256)), dim3(256), 0, st, ca);
   PXR_LAUNCHED();
   // fused kernel
   FusedArgs a;
@@ -865,14 +1056,26 @@ int pxrhip_tiles_gather_push_deposit(const pxrhip_grid *grid, const pxrhip_speci
                                   grid->dy, grid->dz);
   a.q = sp->q; a.dx = grid->dx; a.dy = grid->dy; a.dz = grid->dz;
   a.bad = rt().d_bad;
-  a.lds_doubles = FUSED_LDS_BYTES / 8;
-  const int nwg = rt().num_cus * 2;
+  a.lds_doubles = g_tune.lds_bytes / 8;
+  a.j_doubles = g_tune.j_lds_bytes < 0 || g_tune.j_lds_bytes / 8 > a.lds_doubles ? a.lds_doubles
+                                                                         : g_tune.j_lds_bytes / 8;
+  a.spread = g_tune.spread;
+  a.still = g_tune.still;
+  a.ovf_count = ctl + 2;
+  a.ovf = ovf;
+  a.ovf_cap = maxchunks < INT_MAX ? (int)maxchunks : INT_MAX;
+  const int nwg = rt().num_cus * g_tune.wg_per_cu;
   dim3 grd(nwg), blk(FUSED_BS);
-  const size_t lds = FUSED_LDS_BYTES;
+  const size_t lds = (size_t)g_tune.lds_bytes;
   const int key = (int)(order * 100 + (l_lower_order_in_v ? 10 : 0) + pusher * 2 + (depo_algo ? 1 : 0));
 #define PXR_FUSED_CASE(N_, LO_, P_, D_)                                                          \
   case N_ * 100 + (LO_ ? 10 : 0) + P_ * 2 + (D_ ? 1 : 0):                                        \
+    if (lds > 65536)                                                                             \
+      PXR_HIP(hipFuncSetAttribute((const void *)&k_fused<FUSED_BS, N_, LO_, P_, D_>,             \
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));       \
     hipLaunchKernelGGL((k_fused<FUSED_BS, N_, LO_, P_, D_>), grd, blk, lds, st, a);              \
+    hipLaunchKernelGGL((k_fused_overflow<FUSED_BS, N_, LO_, P_, D_>), dim3(rt().num_cus), blk, 0, \
+                       st, a);                                                                   \
     break;
   switch (key) {
     PXR_FUSED_CASE(1, true, 0, 0) PXR_FUSED_CASE(1, true, 0, 3) PXR_FUSED_CASE(1, true, 1, 0)

@@ -13,12 +13,14 @@ the HIP kernels of libpxrhip.so:
                                   ll4symtry, l_lower_order_in_v, lvect,
                                   field_gathe_algo)
 
-Arrays may be torch ROCm tensors (used in place, zero copy) or numpy arrays
-(copied to the GPU for the call and written back, like Forthon's in-place
-update).  Grid arrays are Fortran-ordered with bounds (-ng:n+ng) on each
-axis: a numpy array with order='F' and shape (nx+1+2ng, ny+1+2ng, nz+1+2ng),
-or a tensor with that shape and strides (1, NX, NX*NY) (see
-``picsar_amd.arrays.grid``), or any contiguous tensor of that many elements.
+Arrays may be numpy arrays (copied to the GPU for the call and written
+back, like Forthon's in-place update), device arrays of the library
+(``picsar_amd.devmem.DeviceArray`` / ``DeviceGrid``, used in place) or torch
+ROCm tensors (used in place).  Grid arrays are Fortran-ordered with bounds
+(-ng:n+ng) on each axis: a numpy array with order='F' and shape
+(nx+1+2ng, ny+1+2ng, nz+1+2ng), a DeviceGrid of that shape, or a tensor with
+that shape and strides (1, NX, NX*NY) (or any contiguous tensor of that many
+elements).
 
 Module variables the reference kernels read (fields%l_nodalgrid) are
 attributes of ``picsar``.
@@ -31,13 +33,9 @@ import numpy as np
 
 from . import _lib
 from ._lib import PxrHipError  # noqa: F401
+from .devmem import DeviceArray, DeviceGrid
 
 _i64arr3 = C.c_int64 * 3
-
-
-def _torch():
-    import torch
-    return torch
 
 
 class _Staged:
@@ -45,19 +43,28 @@ class _Staged:
 
     def __init__(self, a, order, write):
         self.a, self.order, self.write = a, order, write
-        self.t = None
+        self.d = None
 
 
 class _Call:
     """Collects the array arguments of one call.  Every argument is checked
-    before anything is copied; numpy arrays are then staged on the device
-    and the modified ones written back after the call."""
+    before anything is copied.  numpy arrays are staged in device memory of
+    the library and the modified ones written back after the call; device
+    arrays of the library (``picsar_amd.devmem``) are used in place.  torch
+    tensors are used in place too; since torch runs its own HIP runtime, the
+    call is then bracketed by a device synchronisation on both sides."""
 
     def __init__(self):
         self.staged = []
+        self.torch = False
+
+    @staticmethod
+    def _is_torch(a):
+        mod = type(a).__module__
+        return mod.startswith("torch")
 
     def _tensor_ptr(self, t, need, name, fortran_shape=None):
-        torch = _torch()
+        import torch
         if t.dtype != torch.float64:
             raise TypeError(f"{name}: expected float64, got {t.dtype}")
         if not t.is_cuda:
@@ -73,9 +80,19 @@ class _Call:
                                  f"{tuple(fortran_shape)}")
         elif not t.is_contiguous():
             raise ValueError(f"{name}: tensor must be contiguous")
+        self.torch = True
         return t.data_ptr()
 
     def arr(self, a, need, name, write=False, fortran_shape=None):
+        if isinstance(a, DeviceArray):
+            if a.dtype != np.float64:
+                raise TypeError(f"{name}: expected float64")
+            if a.n < need:
+                raise ValueError(f"{name}: {a.n} elements, need {need}")
+            if fortran_shape is not None and isinstance(a, DeviceGrid) and \
+                    tuple(a.shape) != tuple(fortran_shape):
+                raise ValueError(f"{name}: grid shape {a.shape} != {tuple(fortran_shape)}")
+            return a.ptr
         if isinstance(a, np.ndarray):
             if a.dtype != np.float64:
                 raise TypeError(f"{name}: expected float64, got {a.dtype}")
@@ -87,40 +104,39 @@ class _Call:
             h = _Staged(a, order, write)
             self.staged.append(h)
             return h
-        torch = _torch()
-        if isinstance(a, torch.Tensor):
+        if self._is_torch(a):
             return self._tensor_ptr(a, need, name, fortran_shape)
-        raise TypeError(f"{name}: expected a torch tensor or numpy array")
+        raise TypeError(f"{name}: expected a numpy array, a picsar_amd device array or a torch "
+                        "tensor")
 
     def ptrs(self, handles):
         """Copy the staged numpy arrays to the device; device pointers."""
-        torch = _torch()
         out = []
         for h in handles:
             if isinstance(h, _Staged):
-                if h.t is None:
-                    flat = h.a.ravel(order="F" if h.order == "F" else "K")
-                    h.t = torch.from_numpy(np.ascontiguousarray(flat)).to("cuda")
-                out.append(h.t.data_ptr())
+                if h.d is None:
+                    h.d = DeviceArray.from_numpy(h.a.ravel(order="F" if h.order == "F" else "K"))
+                out.append(h.d.ptr)
             else:
                 out.append(h)
         return out
 
+    def before(self):
+        if self.torch:
+            import torch
+            torch.cuda.synchronize()
+
     def finish(self):
+        if self.torch:
+            _lib.call("pxrhip_synchronize")
         for h in self.staged:
-            if h.write and h.t is not None:
-                v = h.t.cpu().numpy()
+            if h.write and h.d is not None:
+                v = h.d.numpy()
                 if h.order == "F":
                     h.a[...] = v.reshape(h.a.shape, order="F")
                 else:
                     h.a.reshape(-1)[...] = v
         self.staged.clear()
-
-
-def _stream():
-    torch = _torch()
-    s = torch.cuda.current_stream().cuda_stream
-    _lib.call("pxrhip_set_stream", C.c_void_p(s))
 
 
 def _gshape(nx, ny, nz, gx, gy, gz):
@@ -157,7 +173,7 @@ class _Picsar:
              ((exg, "exg"), (eyg, "eyg"), (ezg, "ezg"), (bxg, "bxg"), (byg, "byg"),
               (bzg, "bzg"))]
         P, O, G = c.ptrs(P), c.ptrs(O), c.ptrs(G)
-        _stream()
+        c.before()
         _lib.call("pxrhip_geteb3d_energy_conserving", n, *P, *O, float(xmin), float(ymin),
                   float(zmin), float(dx), float(dy), float(dz), int(nx), int(ny), int(nz),
                   int(nxguard), int(nyguard), int(nzguard), int(nox), int(noy), int(noz),
@@ -174,7 +190,7 @@ class _Picsar:
         E = [c.arr(a, n, nm) for a, nm in
              ((ex, "ex"), (ey, "ey"), (ez, "ez"), (bx, "bx"), (by, "by"), (bz, "bz"))]
         U, E = c.ptrs(U), c.ptrs(E)
-        _stream()
+        c.before()
         _lib.call("pxrhip_boris_push_u_3d", n, *U, *E, float(q), float(m), float(dt))
         c.finish()
 
@@ -188,7 +204,7 @@ class _Picsar:
         E = [c.arr(a, n, nm) for a, nm in
              ((ex, "ex"), (ey, "ey"), (ez, "ez"), (bx, "bx"), (by, "by"), (bz, "bz"))]
         U, E = c.ptrs(U), c.ptrs(E)
-        _stream()
+        c.before()
         _lib.call("pxrhip_ebcancelpush3d", n, *U, *E, float(q), float(m), float(dt), int(which))
         c.finish()
 
@@ -200,7 +216,7 @@ class _Picsar:
         U = [c.arr(a, n, nm) for a, nm in
              ((uxp, "uxp"), (uyp, "uyp"), (uzp, "uzp"), (gaminv, "gaminv"))]
         X, U = c.ptrs(X), c.ptrs(U)
-        _stream()
+        c.before()
         _lib.call("pxrhip_pushxyz", n, *X, *U, float(dt))
         c.finish()
 
@@ -211,7 +227,7 @@ class _Picsar:
         U = [c.arr(a, n, nm) for a, nm in ((uxp, "uxp"), (uyp, "uyp"), (uzp, "uzp"))]
         g = c.arr(gaminv, n, "gaminv", write=True)
         U, (g,) = c.ptrs(U), c.ptrs([g])
-        _stream()
+        c.before()
         _lib.call("pxrhip_set_gamma", n, *U, g)
         c.finish()
 
@@ -229,7 +245,7 @@ class _Picsar:
              ((xp, "xp"), (yp, "yp"), (zp, "zp"), (uxp, "uxp"), (uyp, "uyp"), (uzp, "uzp"),
               (gaminv, "gaminv"), (w, "w"))]
         J, P = c.ptrs(J), c.ptrs(P)
-        _stream()
+        c.before()
         _lib.call("pxrhip_depose_jxjyjz", *J, n, *P, float(q), float(xmin), float(ymin),
                   float(zmin), float(dt), float(dx), float(dy), float(dz), int(nx), int(ny),
                   int(nz), int(nxguard), int(nyguard), int(nzguard), int(nox), int(noy), int(noz),
@@ -253,7 +269,7 @@ class _Picsar:
             B += [lo_, hi_]
             args += [c.arr(a, shp[0] * shp[1] * shp[2], nm, write=wr, fortran_shape=shp), lo_, hi_]
         args = c.ptrs(args)
-        _stream()
+        c.before()
         _lib.call("pxrhip_push_em3d_evec", *B[:6], *args, float(mudt), float(dtsdx),
                   float(dtsdy), float(dtsdz))
         c.finish()
@@ -273,7 +289,7 @@ class _Picsar:
             B += [lo_, hi_]
             args += [c.arr(a, shp[0] * shp[1] * shp[2], nm, write=wr, fortran_shape=shp), lo_, hi_]
         args = c.ptrs(args)
-        _stream()
+        c.before()
         _lib.call("pxrhip_push_em3d_bvec", *B[:6], *args, float(dtsdx), float(dtsdy),
                   float(dtsdz))
         c.finish()
@@ -299,7 +315,7 @@ class _Picsar:
                                                       (dtsdz, norderz)))
         nod = self.l_nodalgrid if l_nodalgrid is None else l_nodalgrid
         F = c.ptrs(F)
-        _stream()
+        c.before()
         _lib.call("pxrhip_push_em3d_evec_norder", *F, float(mudt), cx.ctypes.data, cy.ctypes.data,
                   cz.ctypes.data, int(nx), int(ny), int(nz), int(norderx), int(nordery),
                   int(norderz), int(nxguard), int(nyguard), int(nzguard), int(nxs), int(nys),
@@ -319,7 +335,7 @@ class _Picsar:
                                                       (dtsdz, norderz)))
         nod = self.l_nodalgrid if l_nodalgrid is None else l_nodalgrid
         F = c.ptrs(F)
-        _stream()
+        c.before()
         _lib.call("pxrhip_push_em3d_bvec_norder", *F, cx.ctypes.data, cy.ctypes.data,
                   cz.ctypes.data, int(nx), int(ny), int(nz), int(norderx), int(nordery),
                   int(norderz), int(nxguard), int(nyguard), int(nzguard), int(nxs), int(nys),
@@ -335,7 +351,7 @@ class _Picsar:
              ((xp, "xp"), (yp, "yp"), (zp, "zp"), (ux, "ux"), (uy, "uy"), (uz, "uz"),
               (gam, "gam"), (pid, "pid"))]
         A = c.ptrs(A)
-        _stream()
+        c.before()
         _lib.call("pxrhip_particle_bin_sorting", n, *A, int(wpid), float(xmin2), float(ymin2),
                   float(zmin2), float(xmax2), float(ymax2), float(zmax2), float(dxf), float(dyf),
                   float(dzf))
@@ -347,7 +363,7 @@ class _Picsar:
         c = _Call()
         p = c.arr(field, shp[0] * shp[1] * shp[2], "field", write=True, fortran_shape=shp)
         (p,) = c.ptrs([p])
-        _stream()
+        c.before()
         _lib.call("pxrhip_field_bc_periodic", p, int(nx), int(ny), int(nz), int(nxg), int(nyg),
                   int(nzg))
         c.finish()
@@ -358,7 +374,7 @@ class _Picsar:
         c = _Call()
         p = c.arr(array, shp[0] * shp[1] * shp[2], "array", write=True, fortran_shape=shp)
         (p,) = c.ptrs([p])
-        _stream()
+        c.before()
         _lib.call("pxrhip_current_bc_periodic", p, int(nx), int(ny), int(nz), int(nxg), int(nyg),
                   int(nzg))
         c.finish()

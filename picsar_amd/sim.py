@@ -35,12 +35,8 @@ from typing import Dict, List, Optional
 import numpy as np
 
 from . import _lib
+from .devmem import DeviceArray, DeviceGrid
 from .picsarpy import picsar as _pxr
-
-
-def _torch():
-    import torch
-    return torch
 
 
 class PxrGrid(C.Structure):
@@ -90,17 +86,17 @@ class Species:
         return self.bufs[self.cur]
 
     def arrays(self):
-        """Device tensors of the active buffer, first npart entries."""
-        return {k: v[: self.npart] for k, v in self.active.arrays.items()}
+        """Host copies of the active buffer's particle arrays (npart values)."""
+        return {k: v.numpy(self.npart) for k, v in self.active.arrays.items()}
 
     def struct(self, which: int) -> PxrSpecies:
         b = self.bufs[which]
         s = PxrSpecies()
         for k in COMPONENTS:
-            setattr(s, k, b.arrays[k].data_ptr())
-        s.tile_off = b.tile_off.data_ptr()
-        s.tile_np = b.tile_np.data_ptr()
-        s.bin_start = b.bin_start.data_ptr()
+            setattr(s, k, b.arrays[k].ptr)
+        s.tile_off = b.tile_off.ptr
+        s.tile_np = b.tile_np.ptr
+        s.bin_start = b.bin_start.ptr
         s.capacity = self.capacity
         s.q = float(self.q)
         s.m = float(self.m)
@@ -118,8 +114,7 @@ class Simulation:
     def __init__(self, nx, ny, nz, dx=1.0, dy=1.0, dz=1.0, xmin=0.0, ymin=0.0, zmin=0.0,
                  ntx=1, nty=1, ntz=1, order=1, l_lower_order_in_v=True, pusher="boris",
                  depo="esirkepov", norder=2, dt=None, cfl=0.95, nguard=None, sorting_dx=1.0,
-                 sorting_dy=1.0, sorting_dz=1.0, chunk=0, device="cuda"):
-        torch = _torch()
+                 sorting_dy=1.0, sorting_dz=1.0, chunk=0):
         _lib.call("pxrhip_init", -1)
         self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
         self.dx, self.dy, self.dz = float(dx), float(dy), float(dz)
@@ -141,7 +136,6 @@ class Simulation:
             dt = cfl / math.sqrt(1 / self.dx ** 2 + 1 / self.dy ** 2 + 1 / self.dz ** 2)
         self.dt = float(dt)
         self.chunk = int(chunk)
-        self.device = device
         self.it = 0
         for n, t in ((self.nx, ntx), (self.ny, nty), (self.nz, ntz)):
             if (n + 1) // t < 4:
@@ -154,8 +148,7 @@ class Simulation:
         nb = C.c_int64(0)
         _lib.call("pxrhip_tiles_nbins", C.byref(self.grid), C.byref(nb))
         self.nbins = nb.value
-        from .arrays import grid as mkgrid
-        self.fields = {k: mkgrid(self.nx, self.ny, self.nz, ng, device=device)
+        self.fields = {k: DeviceGrid(self.nx, self.ny, self.nz, ng)
                        for k in ("ex", "ey", "ez", "bx", "by", "bz", "jx", "jy", "jz")}
         if self.norder > 2:
             c = _fd_weights(self.norder)
@@ -163,48 +156,45 @@ class Simulation:
         else:
             self.coefs = None
         self.species: List[Species] = []
-        self.dev = torch.device(device)
 
     # ------------------------------------------------------------------
     # particles
     # ------------------------------------------------------------------
     def add_species(self, name, q, m, x, y, z, ux, uy, uz, w, gaminv=None, capacity=None,
                     lfreeze=False, ldodepos=True) -> Species:
-        """Add a species from host (numpy) or device arrays.  gaminv defaults
-        to 1/sqrt(1+u^2) (pxr_set_gamma)."""
-        torch = _torch()
+        """Add a species from host arrays.  gaminv defaults to
+        1/sqrt(1+u^2) (pxr_set_gamma)."""
         n = int(len(x))
         cap = int(capacity) if capacity is not None else max(n, 1)
         if cap < n:
             raise ValueError("capacity < number of particles")
-
-        def dev(a):
-            if isinstance(a, torch.Tensor):
-                return a.to(device=self.dev, dtype=torch.float64).reshape(-1)
-            return torch.as_tensor(np.asarray(a, dtype=np.float64), device=self.dev).reshape(-1)
-
-        src = {k: dev(v) for k, v in zip(("x", "y", "z", "ux", "uy", "uz", "w"),
-                                          (x, y, z, ux, uy, uz, w))}
+        src = {k: np.ascontiguousarray(np.asarray(v, dtype=np.float64).ravel())
+               for k, v in zip(("x", "y", "z", "ux", "uy", "uz", "w"), (x, y, z, ux, uy, uz, w))}
         if gaminv is None:
-            src["gaminv"] = 1.0 / torch.sqrt(1.0 + src["ux"] ** 2 + src["uy"] ** 2 + src["uz"] ** 2)
+            u = src
+            src["gaminv"] = 1.0 / np.sqrt(1.0 + u["ux"] ** 2 + u["uy"] ** 2 + u["uz"] ** 2)
         else:
-            src["gaminv"] = dev(gaminv)
+            src["gaminv"] = np.ascontiguousarray(np.asarray(gaminv, dtype=np.float64).ravel())
         for k, v in src.items():
-            if v.numel() != n:
-                raise ValueError(f"{k}: {v.numel()} values, expected {n}")
+            if v.size != n:
+                raise ValueError(f"{k}: {v.size} values, expected {n}")
         sp = Species(name=name, q=float(q), m=float(m), capacity=cap, npart=n,
                      lfreeze=lfreeze, ldodepos=ldodepos)
         for _ in range(2):
-            arrays = {k: torch.zeros(cap, dtype=torch.float64, device=self.dev) for k in COMPONENTS}
-            sp.bufs.append(_Buffers(
-                arrays=arrays,
-                tile_off=torch.zeros(self.ntiles + 1, dtype=torch.int64, device=self.dev),
-                tile_np=torch.zeros(self.ntiles, dtype=torch.int64, device=self.dev),
-                bin_start=torch.zeros(self.nbins + 1, dtype=torch.int64, device=self.dev)))
+            arrays = {k: DeviceArray(cap) for k in COMPONENTS}
+            b = _Buffers(arrays=arrays, tile_off=DeviceArray(self.ntiles + 1, np.int64),
+                         tile_np=DeviceArray(self.ntiles, np.int64),
+                         bin_start=DeviceArray(self.nbins + 1, np.int64))
+            b.tile_off.fill(0)
+            b.tile_np.fill(0)
+            b.bin_start.fill(0)
+            sp.bufs.append(b)
         # stage into buffer 1 as one unsorted run, then bin into buffer 0
         for k in COMPONENTS:
-            sp.bufs[1].arrays[k][:n].copy_(src[k])
-        sp.bufs[1].tile_off[-1] = n
+            sp.bufs[1].arrays[k].upload(src[k])
+        off = np.zeros(self.ntiles + 1, dtype=np.int64)
+        off[-1] = n
+        sp.bufs[1].tile_off.upload(off)
         sp.cur = 1
         self.species.append(sp)
         self._rebin(sp)
@@ -214,18 +204,12 @@ class Simulation:
     def _rebin(self, sp: Species):
         s_src = sp.struct(sp.cur)
         s_dst = sp.struct(1 - sp.cur)
-        self._set_stream()
         _lib.call("pxrhip_tiles_rebin_sort", C.byref(self.grid), C.byref(s_src), C.byref(s_dst))
         sp.cur = 1 - sp.cur
 
     # ------------------------------------------------------------------
-    def _set_stream(self):
-        torch = _torch()
-        _lib.call("pxrhip_set_stream", C.c_void_p(torch.cuda.current_stream().cuda_stream))
-
     def check(self):
         """Raise if any kernel of the tiled path skipped a particle."""
-        self._set_stream()
         _lib.call("pxrhip_tiles_check")
 
     # ------------------------------------------------------------------
@@ -235,7 +219,7 @@ class Simulation:
         """field_gathering_plus_particle_pusher + pxrdepose_currents_on_grid_jxjyjz
         for every species (J is zeroed first, current_deposition_manager_3d.F90:247)."""
         for k in ("jx", "jy", "jz"):
-            self.fields[k].zero_()
+            self.fields[k].fill(0.0)
         for sp in self.species:
             self.gather_push_deposit_species(sp)
 
@@ -246,12 +230,10 @@ class Simulation:
         if not sp.ldodepos:
             raise NotImplementedError("species that do not deposit are not built")
         f = self.fields
-        self._set_stream()
         s = sp.struct(sp.cur)
         _lib.call("pxrhip_tiles_gather_push_deposit", C.byref(self.grid), C.byref(s),
-                  f["ex"].data_ptr(), f["ey"].data_ptr(), f["ez"].data_ptr(), f["bx"].data_ptr(),
-                  f["by"].data_ptr(), f["bz"].data_ptr(), f["jx"].data_ptr(), f["jy"].data_ptr(),
-                  f["jz"].data_ptr(), self.dt, self.order, int(self.l_lower_order_in_v),
+                  f["ex"].ptr, f["ey"].ptr, f["ez"].ptr, f["bx"].ptr, f["by"].ptr, f["bz"].ptr,
+                  f["jx"].ptr, f["jy"].ptr, f["jz"].ptr, self.dt, self.order, int(self.l_lower_order_in_v),
                   self.pusher, self.depo, self.chunk)
 
     def particle_bcs_and_sorting(self):
@@ -335,19 +317,25 @@ class Simulation:
     # diagnostics (diags.F90:847-1114, kin_energy.F90:33-143)
     # ------------------------------------------------------------------
     def kinetic_energy(self) -> float:
+        """Sum over species of m c^2 sum (1/gaminv - 1) w (kin_energy.F90:33-143)."""
         tot = 0.0
         for sp in self.species:
-            a = sp.arrays()
-            tot += float(((1.0 / a["gaminv"] - 1.0) * a["w"]).sum().item()) * sp.m
+            out = C.c_double(0.0)
+            b = sp.active
+            _lib.call("pxrhip_kinetic_energy", sp.npart, b.arrays["gaminv"].ptr, b.arrays["w"].ptr,
+                      C.byref(out))
+            tot += out.value * sp.m
         return tot
 
     def field_energy(self) -> float:
-        g = self.ng
+        """(1/2) sum (E^2 + B^2) dV over nodes 0..n-1 (diags.F90:1072-1114)."""
         vol = self.dx * self.dy * self.dz
         e = 0.0
         for k in ("ex", "ey", "ez", "bx", "by", "bz"):
-            v = self.fields[k][g:g + self.nx, g:g + self.ny, g:g + self.nz]
-            e += float((v * v).sum().item())
+            out = C.c_double(0.0)
+            _lib.call("pxrhip_field_energy", self.fields[k].ptr, self.nx, self.ny, self.nz,
+                      self.ng, self.ng, self.ng, C.byref(out))
+            e += out.value
         return 0.5 * e * vol
 
     def total_particles(self) -> int:

@@ -18,12 +18,10 @@ REL = 1e-12
 
 @pytest.fixture(scope="module")
 def dev():
-    import torch
-    assert torch.cuda.is_available(), "GPU test run without a GPU"
     from picsar_amd import _lib
     _lib.load()
-    _lib.call("pxrhip_init", 0)
-    return torch
+    _lib.call("pxrhip_init", 0)  # raises unless a gfx950 device is present
+    return _lib
 
 
 @pytest.fixture(scope="module")
@@ -34,21 +32,23 @@ def pxr(dev):
 
 
 def T(dev, a):
-    return dev.from_numpy(np.ascontiguousarray(a)).to("cuda")
+    from picsar_amd.devmem import DeviceArray
+    return DeviceArray.from_numpy(np.asarray(a, dtype=np.float64))
 
 
 def TG(dev, a):
-    from picsar_amd.arrays import from_numpy_grid
-    return from_numpy_grid(a)
+    from picsar_amd.devmem import DeviceGrid
+    g = DeviceGrid(a.shape[0] - 1, a.shape[1] - 1, a.shape[2] - 1, 0)
+    g.upload(a)
+    return g
 
 
 def H(t):
-    return t.cpu().numpy()
+    return t.numpy()
 
 
 def HG(t):
-    from picsar_amd.arrays import to_numpy_grid
-    return to_numpy_grid(t)
+    return t.numpy()
 
 
 def rel(a, b):
@@ -300,3 +300,25 @@ def test_errors_are_reported(dev, pxr):
     np.testing.assert_array_equal(H(P[7]), [1.0, 2.0, 3.0])
     # np = 0 is a no-op
     pxr.pxr_pushxyz(0, *[T(dev, np.zeros(1)) for _ in range(7)], 0.1)
+
+
+def test_torch_tensors_interoperate(dev, pxr, orc):
+    """torch ROCm tensors are accepted in place; the call synchronises the
+    device on both sides (torch runs its own HIP runtime)."""
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("torch sees no GPU")
+    rng = np.random.default_rng(12)
+    n = 5000
+    x = [rng.uniform(1, 10, n) for _ in range(3)]
+    F = [np.asfortranarray(rng.standard_normal(SHAPE)) for _ in range(6)]
+    e_ref = [np.zeros(n) for _ in range(6)]
+    orc.geteb3d_energy_conserving(*x, *e_ref, 0, 0, 0, 1, 1, 1, NX, NY, NZ, NG, NG, NG, 2, 2, 2,
+                                  *F, True, False)
+    tx = [torch.tensor(a, device="cuda") for a in x]
+    te = [torch.zeros(n, dtype=torch.float64, device="cuda") for _ in range(6)]
+    tF = [torch.tensor(np.ascontiguousarray(f.T), device="cuda").permute(2, 1, 0) for f in F]
+    pxr.geteb3d_energy_conserving(n, *tx, *te, 0, 0, 0, 1, 1, 1, NX, NY, NZ, NG, NG, NG, 2, 2, 2,
+                                  *tF, False, True, 8, 0)
+    for a, b in zip(te, e_ref):
+        assert rel(a.cpu().numpy(), b) < REL
