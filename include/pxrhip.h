@@ -263,8 +263,8 @@ int pxrhip_tiles_rebin_sort(const pxrhip_grid *grid, pxrhip_species *src,
  * (its CPU blocking factors lvec_* play this role).  Keys:
  * "fused.lds_bytes" (dynamic LDS per workgroup), "fused.wg_per_cu",
  * "fused.spread" (0/1, lane-spread particle order in the deposition phase),
- * "fused.still" (0/1, fixed-box Esirkepov path for particles that stay in
- * their cell), "fused.chunk" (default particles per work item),
+ * "fused.chunk" (default particles per work item; a work item also holds no
+ * more sorting-grid rows than its LDS blocks allow),
  * "fused.j_lds_bytes" (largest current block accumulated in LDS; negative:
  * all of fused.lds_bytes).  Work items whose field or current block exceeds
  * its LDS budget are completed from global memory by a second kernel. */

@@ -336,61 +336,6 @@ PXR_DEV void esirkepov_deposit(const EsirkepovIn<T> &in, const Acc &acc) {
   }
 }
 
-// Current of a particle whose old and new positions have the same node
-// index on every axis (d = 0, the common case): the box is the fixed
-// (N+1)^3 support, so every loop has compile-time bounds.  Same arithmetic
-// as esirkepov_deposit with d = 0.
-template <int N, typename T, typename Acc>
-PXR_DEV void esirkepov_deposit_still(const EsirkepovIn<T> &in, const int *i0, const Acc &acc) {
-  T wn[3][N + 1], ds[3][N + 1], c[3][N];
-  const T pos[3] = {in.x, in.y, in.z}, old[3] = {in.xo, in.yo, in.zo};
-  const T wq[3] = {in.wqx, in.wqy, in.wqz};
-#pragma unroll
-  for (int a = 0; a < 3; a++) {
-    T wo[N + 1];
-    eweights<N, T>(pos[a], i0[a], wn[a]);
-    eweights<N, T>(old[a], i0[a], wo);
-    T sum = T(0);
-#pragma unroll
-    for (int r = 0; r <= N; r++) {
-      ds[a][r] = wo[r] - wn[a][r];
-      if (r < N) {
-        sum += ds[a][r];
-        c[a][r] = wq[a] * sum;
-      }
-    }
-  }
-  const int bx = i0[0] + ESup<N>::lo, by = i0[1] + ESup<N>::lo, bz = i0[2] + ESup<N>::lo;
-  const T third = Cst<T>::onethird;
-#pragma unroll
-  for (int rk = 0; rk <= N; rk++)
-#pragma unroll
-    for (int rj = 0; rj <= N; rj++) {
-      const T wx = (wn[1][rj] + T(0.5) * ds[1][rj]) * wn[2][rk] +
-                   (T(0.5) * wn[1][rj] + third * ds[1][rj]) * ds[2][rk];
-#pragma unroll
-      for (int r = 0; r < N; r++) acc.add(0, bx + r, by + rj, bz + rk, c[0][r] * wx);
-    }
-#pragma unroll
-  for (int rk = 0; rk <= N; rk++)
-#pragma unroll
-    for (int ri = 0; ri <= N; ri++) {
-      const T wy = (wn[2][rk] + T(0.5) * ds[2][rk]) * wn[0][ri] +
-                   (T(0.5) * wn[2][rk] + third * ds[2][rk]) * ds[0][ri];
-#pragma unroll
-      for (int r = 0; r < N; r++) acc.add(1, bx + ri, by + r, bz + rk, c[1][r] * wy);
-    }
-#pragma unroll
-  for (int rj = 0; rj <= N; rj++)
-#pragma unroll
-    for (int ri = 0; ri <= N; ri++) {
-      const T wz = (wn[0][ri] + T(0.5) * ds[0][ri]) * wn[1][rj] +
-                   (T(0.5) * wn[0][ri] + third * ds[0][ri]) * ds[1][rj];
-#pragma unroll
-      for (int r = 0; r < N; r++) acc.add(2, bx + ri, by + rj, bz + r, c[2][r] * wz);
-    }
-}
-
 // ---------------------------------------------------------------------------
 // direct deposition
 // ---------------------------------------------------------------------------

@@ -84,6 +84,7 @@ struct TileTable {
   int nt[3] = {1, 1, 1}, n0[3] = {1, 1, 1};
   int64_t nbins = 0, nrows = 0;
   int max_bins_tile = 0;
+  int nxmax = 1, nzmax = 1;   // largest sorting-grid extents of a tile in x and z
   // device arrays, per tile
   double *sorg = nullptr;     // 3*ntiles: sorting-grid origin
   int *n3 = nullptr;          // 3*ntiles: sorting cells per axis
@@ -136,7 +137,7 @@ static int get_table(const pxrhip_grid *g, TileTable **out) {
   std::vector<int> n3(3 * (size_t)nt);
   std::vector<int64_t> base(nt + 1);
   int64_t nb = 0, nrows = 0;
-  int maxb = 0;
+  int maxb = 0, nxmax = 1, nzmax = 1;
   for (int iz = 0; iz < ax[2].nt; iz++)
     for (int iy = 0; iy < ax[1].nt; iy++)
       for (int ix = 0; ix < ax[0].nt; ix++) {
@@ -159,6 +160,8 @@ static int get_table(const pxrhip_grid *g, TileTable **out) {
         nb += cells;
         nrows += (int64_t)n3[3 * t + 1] * n3[3 * t + 2];
         if (cells > maxb) maxb = (int)cells;
+        nxmax = max(nxmax, n3[3 * t]);
+        nzmax = max(nzmax, n3[3 * t + 2]);
       }
   base[nt] = nb;
   if (nb >= ((int64_t)1 << 31)) return set_error(PXRHIP_ERR_ARG, "too many sorting cells");
@@ -178,6 +181,8 @@ static int get_table(const pxrhip_grid *g, TileTable **out) {
   g_tt.nbins = nb;
   g_tt.nrows = nrows;
   g_tt.max_bins_tile = maxb;
+  g_tt.nxmax = nxmax;
+  g_tt.nzmax = nzmax;
   g_tt.valid = true;
   *out = &g_tt;
   return PXRHIP_OK;
@@ -374,6 +379,7 @@ struct Chunk {
 
 struct ChunkArgs {
   int ntiles, ch, nzmax;  // nzmax: largest number of sorting planes of a tile
+  int maxrows;            // rows per chunk whose field and current blocks fit in LDS
   const int64_t *tile_off, *tile_np, *bin_start;
   const int *n3;
   const int64_t *bin_base;
@@ -385,8 +391,8 @@ struct ChunkArgs {
 };
 
 // A unit is one z-plane of the sorting grid of one tile.  Its chunks are
-// runs of whole rows (cells of one y-row) of at most `ch` particles; rows
-// denser than that are cut.  A chunk therefore spans a compact block of
+// runs of at most `maxrows` whole rows (cells of one y-row) and at most `ch`
+// particles; rows denser than that are cut.  A chunk therefore spans a compact block of
 // cells: the tile's width in x, a few rows in y, one plane in z.  Without
 // sorting information the tile is one unit cut into runs of ch particles.
 template <bool WRITE>
@@ -413,15 +419,19 @@ PXR_DEV int walk_unit(const ChunkArgs &a, int t, int z, i64 pos) {
   if (z >= nz3) return 0;
   const int64_t base = a.bin_base[t] + (int64_t)z * ny3 * nx3;
   int64_t cb = 0, cnt = 0;
+  int y0 = 0;  // first row of the open chunk
   for (int y = 0; y < ny3; y++) {
     const int64_t row = base + (int64_t)y * nx3;
     const int64_t rb = a.bin_start[row], rn = a.bin_start[row + nx3] - rb;
     if (rn <= 0) continue;
-    if (cnt > 0 && cnt + rn > a.ch) {
+    if (cnt > 0 && (cnt + rn > a.ch || y - y0 >= a.maxrows)) {
       emit(cb, cnt);
       cnt = 0;
     }
-    if (cnt == 0) cb = rb;
+    if (cnt == 0) {
+      cb = rb;
+      y0 = y;
+    }
     cnt += rn;
     while (cnt > a.ch) {
       emit(cb, a.ch);
@@ -471,7 +481,6 @@ struct FusedArgs {
   int lds_doubles;   // dynamic LDS size in doubles (field blocks)
   int j_doubles;     // largest current block kept in LDS, in doubles
   int spread;        // deposit phase: lanes take particles S apart
-  int still;         // deposit phase: fixed-box path for particles that stay in their cell
   int *ovf_count;    // chunks whose blocks do not fit in LDS ...
   int2 *ovf;         // ... (chunk, stage): stage 0 untouched, 1 pushed
   int ovf_cap;
@@ -609,16 +618,12 @@ PXR_DEV bool gpd_deposit_general(const FusedArgs &a, i64 p, const Acc &acc) {
   return true;
 }
 
-#define SLOW_CAP 512  // particles per chunk deferred to the general Esirkepov path
-
 template <int BS, int N, bool LO, int PUSHER, int DEPO>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void k_fused(FusedArgs a) {
   constexpr int M = LO ? N - 1 : N;
   extern __shared__ double smem[];
   __shared__ int s_chunk;
   __shared__ int s_box[6];
-  __shared__ int s_nslow;
-  __shared__ int s_slow[SLOW_CAP];
   const GridGeom &g = a.g;
   const int glo[3] = {-g.gx, -g.gy, -g.gz};
   const int ghi[3] = {g.nx + g.gx, g.ny + g.gy, g.nz + g.gz};
@@ -630,7 +635,6 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void k_
         s_box[d] = INT_MAX;
         s_box[3 + d] = INT_MIN;
       }
-      s_nslow = 0;
     }
     __syncthreads();
     const int c = s_chunk;
@@ -735,38 +739,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void k_
       for (int kk = threadIdx.x; kk < S * 64; kk += BS) {
         const int k = a.spread ? (kk & 63) * S + (kk >> 6) : kk;
         if (k >= ch.n) continue;
-        const i64 p = p0 + k;
-        if (DEPO == 0 && a.still) {
-          // particles that stay in their cell take the fixed-box path; the
-          // others are deferred and deposited together below
-          EsirkepovIn<double> in;
-          esirkepov_prepare<N, double>(in, a.x[p], a.y[p], a.z[p], a.ux[p], a.uy[p], a.uz[p],
-                                       a.gi[p], a.w[p], a.ec);
-          int i0[3];
-          const double pos[3] = {in.x, in.y, in.z}, old[3] = {in.xo, in.yo, in.zo};
-          bool still = true;
-          for (int q = 0; q < 3; q++) {
-            i0[q] = eindex<N, double>(pos[q]);
-            still = still && eindex<N, double>(old[q]) == i0[q];
-          }
-          still = still && g.inside(i0[0] + ESup<N>::lo, i0[1] + ESup<N>::lo, i0[2] + ESup<N>::lo) &&
-                  g.inside(i0[0] + ESup<N>::hi, i0[1] + ESup<N>::hi, i0[2] + ESup<N>::hi);
-          if (still) {
-            esirkepov_deposit_still<N, double>(in, i0, lacc);
-          } else {
-            const int slot = atomicAdd(&s_nslow, 1);
-            if (slot < SLOW_CAP) s_slow[slot] = k;
-            else if (!gpd_deposit_general<N, DEPO>(a, p, lacc)) count_bad(a.bad);
-          }
-        } else {
-          if (!gpd_deposit_general<N, DEPO>(a, p, lacc)) count_bad(a.bad);
-        }
-      }
-      if (DEPO == 0 && a.still) {
-        __syncthreads();
-        const int ns = min(s_nslow, SLOW_CAP);
-        for (int q = threadIdx.x; q < ns; q += BS)
-          if (!gpd_deposit_general<N, DEPO>(a, p0 + s_slow[q], lacc)) count_bad(a.bad);
+        if (!gpd_deposit_general<N, DEPO>(a, p0 + k, lacc)) count_bad(a.bad);
       }
     }
     __syncthreads();
@@ -786,6 +759,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void k_
 
 // Chunks whose field or current block did not fit in LDS: gather from
 // global memory and deposit with global atomics, one workgroup per chunk.
+// Its particles were already pushed when the stage is 1.
 template <int BS, int N, bool LO, int PUSHER, int DEPO>
 __global__ __launch_bounds__(BS) void k_fused_overflow(FusedArgs a) {
   const int n = min(*a.ovf_count, a.ovf_cap);
@@ -803,8 +777,11 @@ __global__ __launch_bounds__(BS) void k_fused_overflow(FusedArgs a) {
     }
     __syncthreads();
     const GAcc acc{{a.J[0], a.J[1], a.J[2]}, a.g};
-    for (int k = threadIdx.x; k < ch.n; k += BS)
-      if (!gpd_deposit_general<N, DEPO>(a, ch.begin + k, acc)) count_bad(a.bad);
+    const int S = (ch.n + 63) >> 6;
+    for (int kk = threadIdx.x; kk < S * 64; kk += BS) {
+      const int k = a.spread ? (kk & 63) * S + (kk >> 6) : kk;
+      if (k < ch.n && !gpd_deposit_general<N, DEPO>(a, ch.begin + k, acc)) count_bad(a.bad);
+    }
     __syncthreads();
   }
 }
@@ -822,7 +799,6 @@ struct Tuning {
                           // of static LDS) fit in a CU's 160 KiB
   int wg_per_cu = 2;      // persistent workgroups per CU
   int spread = 1;         // lane-spread mapping in the deposit phase
-  int still = 1;          // fixed-box Esirkepov path for particles that stay in their cell
   int chunk = 2048;       // default particles per work item
   int j_lds_bytes = -1;   // cap on the LDS current block (negative: all of lds_bytes)
 };
@@ -851,8 +827,6 @@ int pxrhip_set_tuning(const char *key, int64_t value) {
     g_tune.wg_per_cu = (int)value;
   } else if (!strcmp(key, "fused.spread")) {
     g_tune.spread = value != 0;
-  } else if (!strcmp(key, "fused.still")) {
-    g_tune.still = value != 0;
   } else if (!strcmp(key, "fused.j_lds_bytes")) {
     g_tune.j_lds_bytes = value < 0 ? -1 : (value > (1 << 30) ? (1 << 30) : (int)value);
   } else if (!strcmp(key, "fused.chunk")) {
@@ -869,7 +843,6 @@ int pxrhip_get_tuning(const char *key, int64_t *value) {
   if (!strcmp(key, "fused.lds_bytes")) *value = g_tune.lds_bytes;
   else if (!strcmp(key, "fused.wg_per_cu")) *value = g_tune.wg_per_cu;
   else if (!strcmp(key, "fused.spread")) *value = g_tune.spread;
-  else if (!strcmp(key, "fused.still")) *value = g_tune.still;
   else if (!strcmp(key, "fused.chunk")) *value = g_tune.chunk;
   else if (!strcmp(key, "fused.j_lds_bytes")) *value = g_tune.j_lds_bytes;
   else return set_error(PXRHIP_ERR_ARG, "unknown tuning key '%s'", key);
@@ -983,21 +956,25 @@ int pxrhip_tiles_gather_push_deposit(const pxrhip_grid *grid, const pxrhip_speci
   hipStream_t st = rt().stream;
   // chunk table
   const i64 cap = sp->capacity;
-  int nzmax = 1;
+  const int nzmax = tt->nzmax;
+  // rows of a chunk: cells spanned on each axis by particles of r rows of
+  // the sorting grid (one cell more for the half-cell offset of the sorting
+  // grid), plus the order-N stencil (field block) and one cell of motion
+  // (current block)
+  int maxrows = 1;
   {
-    // largest number of sorting planes of a tile (host copy of the table)
-    static int cached_nzmax = 0;
-    static const TileTable *cached_for = nullptr;
-    static pxrhip_grid cached_key;
-    if (cached_for != tt || memcmp(&cached_key, &tt->key, sizeof(pxrhip_grid)) != 0) {
-      std::vector<int> n3(3 * (size_t)tt->ntiles);
-      PXR_HIP(hipMemcpy(n3.data(), tt->n3, sizeof(int) * n3.size(), hipMemcpyDeviceToHost));
-      cached_nzmax = 1;
-      for (int t = 0; t < tt->ntiles; t++) cached_nzmax = max(cached_nzmax, n3[3 * t + 2]);
-      cached_for = tt;
-      cached_key = tt->key;
+    const double sx = grid->sorting_dx / grid->dx, sy = grid->sorting_dy / grid->dy;
+    const double sz = grid->sorting_dz / grid->dz;
+    const double wx = ceil((tt->nxmax - 1) * sx) + 1, wz = ceil(sz) + 1;
+    const double fl = g_tune.lds_bytes / 8.0;
+    const double jl = g_tune.j_lds_bytes < 0 ? fl : g_tune.j_lds_bytes / 8.0;
+    for (int r = 2; r <= 4096; r++) {
+      const double wy = ceil(r * sy) + 1;
+      const double fb = 6.0 * (wx + order + 1) * (wy + order + 1) * (wz + order + 1);
+      const double jb = 3.0 * (wx + order + 2) * (wy + order + 2) * (wz + order + 2);
+      if (fb > fl || jb > jl) break;
+      maxrows = r;
     }
-    nzmax = cached_nzmax;
   }
   const i64 nunits = (i64)tt->ntiles * nzmax;
   const i64 maxchunks = tt->nrows + (cap / ch + 1) + nunits + 16;
@@ -1013,6 +990,7 @@ int pxrhip_tiles_gather_push_deposit(const pxrhip_grid *grid, const pxrhip_speci
   ca.ntiles = tt->ntiles;
   ca.ch = ch;
   ca.nzmax = nzmax;
+  ca.maxrows = maxrows;
   ca.tile_off = sp->tile_off;
   ca.tile_np = sp->tile_np;
   ca.bin_start = sp->bin_start;
@@ -1060,7 +1038,6 @@ int pxrhip_tiles_gather_push_deposit(const pxrhip_grid *grid, const pxrhip_speci
   a.j_doubles = g_tune.j_lds_bytes < 0 || g_tune.j_lds_bytes / 8 > a.lds_doubles ? a.lds_doubles
                                                                          : g_tune.j_lds_bytes / 8;
   a.spread = g_tune.spread;
-  a.still = g_tune.still;
   a.ovf_count = ctl + 2;
   a.ovf = ovf;
   a.ovf_cap = maxchunks < INT_MAX ? (int)maxchunks : INT_MAX;
@@ -1074,7 +1051,7 @@ int pxrhip_tiles_gather_push_deposit(const pxrhip_grid *grid, const pxrhip_speci
       PXR_HIP(hipFuncSetAttribute((const void *)&k_fused<FUSED_BS, N_, LO_, P_, D_>,             \
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));       \
     hipLaunchKernelGGL((k_fused<FUSED_BS, N_, LO_, P_, D_>), grd, blk, lds, st, a);              \
-    hipLaunchKernelGGL((k_fused_overflow<FUSED_BS, N_, LO_, P_, D_>), dim3(rt().num_cus), blk, 0, \
+    hipLaunchKernelGGL((k_fused_overflow<FUSED_BS, N_, LO_, P_, D_>), dim3(2 * rt().num_cus), blk, 0, \
                        st, a);                                                                   \
     break;
   switch (key) {

@@ -128,7 +128,7 @@ def test_chunking_does_not_change_results(torch, orc, chunk):
     {"fused.lds_bytes": 4096},        # no field block fits: gather and deposit from global memory
     {"fused.j_lds_bytes": 0},         # fields in LDS, every current block deposited globally
     {"fused.j_lds_bytes": 6000},      # some current blocks in LDS, the others globally
-    {"fused.still": 0, "fused.spread": 0},
+    {"fused.spread": 0},
 ])
 @pytest.mark.parametrize("depo", ["esirkepov", "direct"])
 def test_overflow_and_variants_match(torch, orc, tune, depo):
