@@ -33,9 +33,30 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
 BYTES_PER_PARTICLE_FP64 = 120  # read x,y,z,ux,uy,uz,gaminv,w; write x..gaminv
 BYTES_PER_CELL = 72            # read E, B (48 B); write J (24 B)
 OMEGA_PE = 0.02                # electron plasma frequency (c = dx = 1)
+
+
+def profiled_traffic(args):
+    """HBM bytes per fused launch from the committed counter summary
+    (FETCH_SIZE and WRITE_SIZE passes scaled by a copy-kernel calibration),
+    valid for the default workload and launch parameters only."""
+    if (args.n, args.ppc, args.tiles, args.chunk) != (128, 16, 8, 0) or args.tune:
+        return None
+    path = os.path.join(ROOT, "profiles", "fused_c2_hbm_traffic.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch ="):
+                return float(line.split("=")[1])
+    except OSError:
+        pass
+    return None
 
 
 def build_workload(args):
@@ -67,10 +88,12 @@ def run_gpu(args):
 
     sim = build_workload(args)
     N = sim.total_particles()
+    log(f"workload ready: {N} particles, {sim.nx}x{sim.ny}x{sim.nz} cells")
     for _ in range(args.warmup):
         sim.step(1, check=False)
     sim.check()
     synchronize()
+    log(f"{args.warmup} warmup steps done")
     ke0, fe0 = sim.kinetic_energy(), sim.field_energy()
     K = args.steps
     nsp = len(sim.species)
@@ -97,6 +120,7 @@ def run_gpu(args):
     synchronize()
     wall = time.perf_counter() - t0
     sim.check()
+    log(f"{K} timed steps done in {wall:.2f} s")
     stage = {"gather_push_deposit": 0.0, "current_bcs": 0.0, "fdtd": 0.0, "rebin_sort": 0.0}
     launches = []
     total_ms = 0.0
@@ -193,8 +217,8 @@ def cpu_baseline(args):
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=128, help="cells per axis")
     ap.add_argument("--ppc", type=int, default=16)
     ap.add_argument("--tiles", type=int, default=8, help="tiles per axis")
@@ -249,7 +273,10 @@ def main():
                    "gather_push_deposit_particle_steps_per_s": gpd,
                    "wall_s_timed_region": r["wall"]},
         "roofline": {"bound": "hbm", "achieved": r["achieved"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": r["achieved"] / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": r["achieved"] / HBM_PEAK_GBS,
+                     "traffic": profiled_traffic(args),
+                     "traffic_source": "profiles/fused_c2_hbm_traffic.md (rocprofv3 FETCH_SIZE, "
+                                       "WRITE_SIZE, copy-calibrated)",
                      "kernel": "k_fused (gather+push+deposit, one species per launch)",
                      "launch_ms": r["per_launch_ms"],
                      "bytes_per_launch": r["bytes_launch"]},
@@ -269,6 +296,7 @@ def main():
         out["calibration_copy_bytes"] = 2 * a.nbytes
         del a, b
     if not args.no_cpu_baseline:
+        log(f"timing the CPU baseline for about {args.cpu_seconds:.0f} s")
         out["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(out))
 

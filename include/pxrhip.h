@@ -64,8 +64,9 @@ int pxrhip_memcpy_dtoh(void *dst, const void *src, int64_t bytes);
 int pxrhip_memcpy_dtod(void *dst, const void *src, int64_t bytes);
 int pxrhip_fill_f64(double *p, int64_t n, double value);
 int pxrhip_fill_i64(int64_t *p, int64_t n, int64_t value);
-/* HIP events on the library stream, the device counterpart of the
- * reference's localtimes() wall-clock slots (modules.F90:918-941) */
+/* HIP events on the library stream (slots 0..65535), the device
+ * counterpart of the reference's localtimes() wall-clock slots
+ * (modules.F90:918-941) */
 int pxrhip_timer_record(int slot);
 int pxrhip_timer_elapsed(int start, int stop, double *ms);
 

@@ -5,8 +5,8 @@ This is the manager layer of PICSAR (the zero-argument routines of
 push_efield, efield_bcs, bfield_bcs, current_bcs, particle_bcs,
 particle_sorting_sub, field_gathering_plus_particle_pusher,
 pxrdepose_currents_on_grid_jxjyjz) for one periodic rank.  Fields live in
-Fortran-ordered float64 tensors with guards; each species lives in SoA
-tensors packed by tile (tile geometry of set_tile_split_for_species,
+Fortran-ordered float64 device grids with guards; each species lives in SoA
+device arrays packed by tile (tile geometry of set_tile_split_for_species,
 tiling.F90:94-271).
 
 The step follows submain.F90:99-199:
