@@ -243,7 +243,7 @@ def main():
         k, v = kv.split("=")
         _lib.call("pxrhip_set_tuning", k.encode(), int(v))
     for k in ("fused.lds_bytes", "fused.wg_per_cu", "fused.spread", "fused.chunk",
-              "fused.j_lds_bytes"):
+              "fused.j_lds_bytes", "fused.jpad", "fused.margin_pct"):
         import ctypes
         v = ctypes.c_int64(0)
         _lib.call("pxrhip_get_tuning", k.encode(), ctypes.byref(v))

@@ -133,6 +133,17 @@ int pxrhip_depose_jxjyjz(double *jx, double *jy, double *jz, int64_t np,
                          int64_t nox, int64_t noy, int64_t noz,
                          int64_t current_depo_algo, int64_t l_nodal);
 
+/* pxr_depose_rho_n, charge_deposition_3d.F90:2081-2228: charge density of
+ * order nox = noy = noz in {1,2,3} at the nodes (floor for odd, nint for
+ * even orders), ADDED into rho.  l_particles_weight = 0 uses w[0] for every
+ * particle; l4symtry folds x and y to their absolute values. */
+int pxrhip_depose_rho_n(double *rho, int64_t np, const double *xp, const double *yp,
+                        const double *zp, const double *w, double q, double xmin,
+                        double ymin, double zmin, double dx, double dy, double dz,
+                        int64_t nx, int64_t ny, int64_t nz, int64_t nxguard,
+                        int64_t nyguard, int64_t nzguard, int64_t nox, int64_t noy,
+                        int64_t noz, int64_t l_particles_weight, int64_t l4symtry);
+
 /* ---- FDTD ---------------------------------------------------------------- */
 /* pxrpush_em3d_evec, yee.F90:637-750.  lo/hi triples are host int64[3];
  * each array carries its own bounds, as in the reference. */
@@ -267,7 +278,9 @@ int pxrhip_tiles_rebin_sort(const pxrhip_grid *grid, pxrhip_species *src,
  * "fused.chunk" (default particles per work item; a work item also holds no
  * more sorting-grid rows than its LDS blocks allow),
  * "fused.j_lds_bytes" (largest current block accumulated in LDS; negative:
- * all of fused.lds_bytes).  Work items whose field or current block exceeds
+ * all of fused.lds_bytes), "fused.jpad" (0/1, odd row and plane pitches of
+ * the LDS current block), "fused.margin_pct" (motion margin, in 1/100 cell,
+ * of the current block estimated when work items are cut).  Work items whose field or current block exceeds
  * its LDS budget are completed from global memory by a second kernel. */
 int pxrhip_set_tuning(const char *key, int64_t value);
 int pxrhip_get_tuning(const char *key, int64_t *value);

@@ -54,6 +54,7 @@ for f in field_gathering/energy_conserving/field_gathering_o1_3d.F90 \
          particle_pushers/boris_pusher/boris_3d.F90 \
          particle_pushers/vay_pusher/vay_3d.F90 \
          particle_deposition/current_deposition/esirkepov/esirkepov_3d.F90 \
+         particle_deposition/charge_deposition/charge_deposition_3d.F90 \
          field_solvers/Maxwell/yee_solver/yee.F90; do
   o=$(basename "$f" .F90).o
   $FC $FFLAGS -c "$SRC/$f" -o "$o"

@@ -480,11 +480,11 @@ i64 orc_depose_esirkepov(double *jx, double *jy, double *jz, i64 np,
 }
 
 /* ------------------------------------------------------------------------
- * Charge density, nodal shape of order n at the given positions.  Used by
- * the tests to form the discrete continuity residual of the Esirkepov
- * current; follows the node convention of the Esirkepov routines above
- * (floor for orders 1/3, nint for order 2) so that rho and J share one
- * shape.  rho is ADDED into.                                           */
+ * Charge density, pxr_depose_rho_n (charge_deposition_3d.F90:2081-2228)
+ * with nox = noy = noz = n, l_particles_weight, no 4-fold symmetry: nodal
+ * shape of order n at floor (n odd) / nint (n even), the node convention of
+ * the Esirkepov routines above, so that rho and J share one shape (the
+ * discrete continuity residual of the tests).  rho is ADDED into.      */
 i64 orc_depose_rho(double *rho, i64 np, const double *xp, const double *yp,
                    const double *zp, const double *w, double q, double xmin,
                    double ymin, double zmin, double dx, double dy, double dz,

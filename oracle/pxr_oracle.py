@@ -312,6 +312,22 @@ class Reference:
         else:
             raise ValueError("algo must be 0 or 3")
 
+    def depose_rho(self, rho, xp, yp, zp, w, q, xmin, ymin, zmin, dx, dy, dz, nx, ny, nz, ngx,
+                   ngy, ngz, order, scalar=False):
+        """pxr_depose_rho_n (charge_deposition_3d.F90:2081), or with scalar=True
+        depose_rho_scalar_n_n_n (:77/:174/:324)."""
+        _chk_grid(rho)
+        _chk_part(xp, yp, zp, w)
+        head = [_ptr(rho), _ri(len(xp)), _ptr(xp), _ptr(yp), _ptr(zp), _ptr(w), _rf(q),
+                _rf(xmin), _rf(ymin), _rf(zmin), _rf(dx), _rf(dy), _rf(dz), _ri(nx), _ri(ny),
+                _ri(nz), _ri(ngx), _ri(ngy), _ri(ngz)]
+        if scalar:
+            o = f"{order}_{order}_{order}"
+            self._f(f"depose_rho_scalar_{o}_")(*head, _ri(64))
+        else:
+            self._f("pxr_depose_rho_n_")(*head, _ri(order), _ri(order), _ri(order), _ri(1),
+                                         _ri(0))
+
     def push_em3d_evec(self, lo, hi, ex, ey, ez, bx, by, bz, jx, jy, jz, mudt, dtsdx, dtsdy,
                        dtsdz, nx, ny, nz, ngx, ngy, ngz):
         _chk_grid(ex, ey, ez, bx, by, bz, jx, jy, jz)

@@ -50,6 +50,7 @@ SIGNATURES = {
     "pxrhip_pushxyz": [_I] + [_P] * 7 + [_F],
     "pxrhip_set_gamma": [_I] + [_P] * 4,
     "pxrhip_depose_jxjyjz": [_P] * 3 + [_I] + [_P] * 8 + [_F] * 8 + [_I] * 6 + [_I] * 3 + [_I, _I],
+    "pxrhip_depose_rho_n": [_P, _I] + [_P] * 4 + [_F] * 7 + [_I] * 6 + [_I] * 3 + [_I, _I],
     "pxrhip_push_em3d_evec": [_P] * 6 + [_P] * 27 + [_F] * 4,
     "pxrhip_push_em3d_bvec": [_P] * 6 + [_P] * 18 + [_F] * 3,
     "pxrhip_push_em3d_evec_norder": [_P] * 9 + [_F] + [_P] * 3 + [_I] * 13,

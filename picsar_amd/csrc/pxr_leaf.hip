@@ -168,6 +168,49 @@ __global__ __launch_bounds__(256) void k_direct(DepoArgs a) {
   direct_deposit<N, double>(in, acc);
 }
 
+// charge density, pxr_depose_rho_n (charge_deposition_3d.F90:2081-2228):
+// nodal shape of order N at floor (N odd) / nint (N even), wq = q w / vol,
+// rho(j+a, k+b, l+c) += sx(a) sy(b) sz(c) wq
+struct RhoArgs {
+  i64 np;
+  const double *xp, *yp, *zp, *w;
+  double q, xmin, ymin, zmin, dxi, dyi, dzi, invvol;
+  int per_particle_weight, l4symtry;
+  GridGeom g;
+  double *rho;
+  unsigned long long *bad;
+};
+
+template <int N>
+__global__ __launch_bounds__(256) void k_rho(RhoArgs a) {
+  const i64 ip = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= a.np) return;
+  double x = (a.xp[ip] - a.xmin) * a.dxi;
+  double y = (a.yp[ip] - a.ymin) * a.dyi;
+  const double z = (a.zp[ip] - a.zmin) * a.dzi;
+  if (a.l4symtry) {
+    x = fabs(x);
+    y = fabs(y);
+  }
+  double sx[N + 1], sy[N + 1], sz[N + 1];
+  const int j = shape_nodal<N, double>(x, sx);
+  const int k = shape_nodal<N, double>(y, sy);
+  const int l = shape_nodal<N, double>(z, sz);
+  if (!a.g.inside(j, k, l) || !a.g.inside(j + N, k + N, l + N)) {
+    count_bad(a.bad);
+    return;
+  }
+  const double wq = a.per_particle_weight ? a.q * a.w[ip] * a.invvol : a.q * a.invvol * a.w[0];
+  double *r = a.rho + a.g.idx(j, k, l);
+#pragma unroll
+  for (int c = 0; c <= N; c++)
+#pragma unroll
+    for (int b = 0; b <= N; b++)
+#pragma unroll
+      for (int e = 0; e <= N; e++)
+        unsafeAtomicAdd(r + e + a.g.NX * (b + (i64)a.g.NY * c), sx[e] * sy[b] * sz[c] * wq);
+}
+
 // ---------------------------------------------------------------------------
 // FDTD
 // ---------------------------------------------------------------------------
@@ -675,6 +718,41 @@ int pxrhip_depose_jxjyjz(double *jx, double *jy, double *jz, int64_t np, const d
   }
   PXR_LAUNCHED();
   return finish_checked("depose_jxjyjz");
+}
+
+int pxrhip_depose_rho_n(double *rho, int64_t np, const double *xp, const double *yp,
+                        const double *zp, const double *w, double q, double xmin, double ymin,
+                        double zmin, double dx, double dy, double dz, int64_t nx, int64_t ny,
+                        int64_t nz, int64_t nxguard, int64_t nyguard, int64_t nzguard, int64_t nox,
+                        int64_t noy, int64_t noz, int64_t l_particles_weight, int64_t l4symtry) {
+  PXR_INIT();
+  if (np < 0) return set_error(PXRHIP_ERR_ARG, "np < 0");
+  if (np == 0) return PXRHIP_OK;
+  int r = check_grid(nx, ny, nz, nxguard, nyguard, nzguard);
+  if (r) return r;
+  if (!rho || !xp || !yp || !zp || !w) return set_error(PXRHIP_ERR_ARG, "null array");
+  if (nox != noy || nox != noz)
+    return set_error(PXRHIP_ERR_UNSUPPORTED, "unequal shape orders: only nox=noy=noz is built");
+  if (nox < 1 || nox > 3)
+    return set_error(PXRHIP_ERR_UNSUPPORTED, "shape order %lld: orders 1..3 are built", (long long)nox);
+  RhoArgs a;
+  a.np = np;
+  a.xp = xp; a.yp = yp; a.zp = zp; a.w = w;
+  a.q = q; a.xmin = xmin; a.ymin = ymin; a.zmin = zmin;
+  a.dxi = 1.0 / dx; a.dyi = 1.0 / dy; a.dzi = 1.0 / dz;
+  a.invvol = a.dxi * a.dyi * a.dzi;
+  a.per_particle_weight = l_particles_weight != 0;
+  a.l4symtry = l4symtry != 0;
+  a.g = mkgeom(nx, ny, nz, nxguard, nyguard, nzguard);
+  a.rho = rho;
+  a.bad = rt().d_bad;
+  dim3 grd(grid1d(np, 256)), blk(256);
+  hipStream_t st = rt().stream;
+  if (nox == 1) hipLaunchKernelGGL(k_rho<1>, grd, blk, 0, st, a);
+  else if (nox == 2) hipLaunchKernelGGL(k_rho<2>, grd, blk, 0, st, a);
+  else hipLaunchKernelGGL(k_rho<3>, grd, blk, 0, st, a);
+  PXR_LAUNCHED();
+  return finish_checked("pxr_depose_rho_n");
 }
 
 static int mk_arr3(Arr3 &a, const double *p, const int64_t *lo, const int64_t *hi) {

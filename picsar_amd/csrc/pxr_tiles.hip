@@ -87,6 +87,7 @@ struct TileTable {
   int nxmax = 1, nzmax = 1;   // largest sorting-grid extents of a tile in x and z
   // device arrays, per tile
   double *sorg = nullptr;     // 3*ntiles: sorting-grid origin
+  double *tbox = nullptr;     // 6*ntiles: particle membership [tmin, tmax) per axis
   int *n3 = nullptr;          // 3*ntiles: sorting cells per axis
   int64_t *bin_base = nullptr;  // ntiles+1
 };
@@ -94,11 +95,13 @@ struct TileTable {
 static TileTable g_tt;
 
 static void free_table(TileTable &t) {
-  if (t.sorg || t.n3 || t.bin_base) (void)hipStreamSynchronize(rt().stream);
+  if (t.sorg || t.n3 || t.bin_base || t.tbox) (void)hipStreamSynchronize(rt().stream);
   if (t.sorg) (void)hipFree(t.sorg);
+  if (t.tbox) (void)hipFree(t.tbox);
   if (t.n3) (void)hipFree(t.n3);
   if (t.bin_base) (void)hipFree(t.bin_base);
   t.sorg = nullptr;
+  t.tbox = nullptr;
   t.n3 = nullptr;
   t.bin_base = nullptr;
   t.valid = false;
@@ -133,7 +136,7 @@ static int get_table(const pxrhip_grid *g, TileTable **out) {
   for (int a = 0; a < 3; a++)
     if ((r = split_axis(nc[a], ntl[a], x0[a], d[a], ax[a]))) return r;
   const int nt = ax[0].nt * ax[1].nt * ax[2].nt;
-  std::vector<double> sorg(3 * (size_t)nt);
+  std::vector<double> sorg(3 * (size_t)nt), tbox(6 * (size_t)nt);
   std::vector<int> n3(3 * (size_t)nt);
   std::vector<int64_t> base(nt + 1);
   int64_t nb = 0, nrows = 0;
@@ -152,6 +155,8 @@ static int get_table(const pxrhip_grid *g, TileTable **out) {
           const int64_t n = (int64_t)ceil((hi - lo) * sdi);
           if (n < 1 || n > (1 << 20)) return set_error(PXRHIP_ERR_ARG, "bad sorting grid");
           sorg[3 * t + a] = lo;
+          tbox[6 * t + a] = ax[a].tmin[ii[a]];
+          tbox[6 * t + 3 + a] = ax[a].tmax[ii[a]];
           n3[3 * t + a] = (int)n;
           cells *= n;
         }
@@ -167,6 +172,8 @@ static int get_table(const pxrhip_grid *g, TileTable **out) {
   if (nb >= ((int64_t)1 << 31)) return set_error(PXRHIP_ERR_ARG, "too many sorting cells");
   free_table(g_tt);
   PXR_HIP(hipMalloc(&g_tt.sorg, sizeof(double) * sorg.size()));
+  PXR_HIP(hipMalloc(&g_tt.tbox, sizeof(double) * tbox.size()));
+  PXR_HIP(hipMemcpy(g_tt.tbox, tbox.data(), sizeof(double) * tbox.size(), hipMemcpyHostToDevice));
   PXR_HIP(hipMalloc(&g_tt.n3, sizeof(int) * n3.size()));
   PXR_HIP(hipMalloc(&g_tt.bin_base, sizeof(int64_t) * base.size()));
   PXR_HIP(hipMemcpy(g_tt.sorg, sorg.data(), sizeof(double) * sorg.size(), hipMemcpyHostToDevice));
@@ -371,15 +378,90 @@ __global__ __launch_bounds__(256) void k_rb_tiles(int ntiles, const int64_t *bin
 // ---------------------------------------------------------------------------
 // chunk table of the fused kernel
 // ---------------------------------------------------------------------------
+// Node blocks of a work item.  The particles of a work item sit in known
+// cells of the sorting grid (tile t, plane z, rows y0..y1), so the index
+// ranges of their gather stencils and of their currents follow from the
+// positions bounding those cells; the index functions of the reference's
+// shapes are non-decreasing in the position.
+struct BoxGeom {
+  double x0[3], di[3];   // grid origin and inverse cell size
+  double sd[3];          // sorting cell size
+  double mv[3];          // motion margin of the current block, in cells
+  double s;              // staggering shift (1/2)
+  int N, M, depo;        // shape orders (nodal, staggered) and deposition (0 Esirkepov, 3 direct)
+  int glo[3], ghi[3];    // node range of the grid arrays
+  const double *sorg;    // sorting-grid origin of every tile
+  const double *tbox;    // membership [tmin, tmax) of every tile
+  const int *n3;         // sorting cells of every tile
+};
+
+PXR_DEV void gather_range(const BoxGeom &b, double ulo, double uhi, int &lo, int &hi) {
+  const int N = b.N, M = b.M;
+  auto jn = [&](double v) { return (N == 2 ? f_nint(v) : f_floor(v)) - (N == 1 ? 0 : 1); };
+  auto js = [&](double v) {
+    return (M == 0 || M == 2) ? f_floor((v + 0.5) - b.s) - (M == 2 ? 1 : 0)
+                              : f_floor(v - b.s) - (M == 3 ? 1 : 0);
+  };
+  lo = min(jn(ulo), js(ulo));
+  hi = max(jn(uhi) + N, js(uhi) + M);
+}
+
+PXR_DEV void deposit_range(const BoxGeom &b, double ulo, double uhi, double m, int &lo, int &hi) {
+  const int N = b.N;
+  if (b.depo == 0) {  // Esirkepov: old and new node index, support ESup<N>
+    auto ei = [&](double v) { return N == 2 ? f_nint(v) : f_floor(v); };
+    lo = ei(ulo - m) + (N == 1 ? 0 : -1);
+    hi = ei(uhi + m) + (N == 3 ? 2 : 1);
+  } else {  // direct: nodal and staggered shapes at the mid position
+    auto dj = [&](double v) { return N == 2 ? f_nint(v) - 1 : f_floor(v) - (N == 1 ? 0 : 1); };
+    lo = min(dj(ulo - m), dj((ulo - m) - b.s));
+    hi = max(dj(uhi + m), dj((uhi + m) - b.s)) + N;
+  }
+}
+
+// normalized coordinate range of the particles of tile t in sorting rows
+// y0..y1 of plane z (intersected with the tile's membership)
+PXR_DEV void rows_range(const BoxGeom &b, int t, int z, int y0, int y1, double *ulo, double *uhi) {
+  const double *so = b.sorg + 3 * t, *tb = b.tbox + 6 * t;
+  const int c0[3] = {0, y0, z};
+  const int c1[3] = {b.n3[3 * t] - 1, y1, z};
+  for (int d = 0; d < 3; d++) {
+    const double plo = fmax(so[d] + b.sd[d] * c0[d], tb[d]);
+    const double phi = fmin(so[d] + b.sd[d] * (c1[d] + 1), tb[3 + d]);
+    ulo[d] = (plo - b.x0[d]) * b.di[d] - 1e-7;
+    uhi[d] = (phi - b.x0[d]) * b.di[d] + 1e-7;
+  }
+}
+
+// field block (FR0, fe) and current block (JR0, je) for that range, clamped
+// to the arrays; returns the doubles they take in LDS (6 + 3 arrays)
+PXR_DEV i64 blocks_for(const BoxGeom &b, const double *ulo, const double *uhi, int *FR0, int *fe,
+                       int *JR0, int *je) {
+  for (int d = 0; d < 3; d++) {
+    int lo, hi;
+    gather_range(b, ulo[d], uhi[d], lo, hi);
+    FR0[d] = max(lo, b.glo[d]);
+    fe[d] = max(min(hi, b.ghi[d]) - FR0[d] + 1, 0);
+    deposit_range(b, ulo[d], uhi[d], b.mv[d], lo, hi);
+    JR0[d] = max(lo, b.glo[d]);
+    je[d] = max(min(hi, b.ghi[d]) - JR0[d] + 1, 0);
+  }
+  return 6 * (i64)fe[0] * fe[1] * fe[2] + 3 * (i64)je[0] * je[1] * je[2];
+}
+
 struct Chunk {
   long long begin;
   int n;
   int tile;
+  int z, y0, y1;  // sorting-grid plane and rows of the particles; z < 0: unknown
+  int pad;
 };
 
 struct ChunkArgs {
   int ntiles, ch, nzmax;  // nzmax: largest number of sorting planes of a tile
-  int maxrows;            // rows per chunk whose field and current blocks fit in LDS
+  BoxGeom bg;
+  i64 lds_doubles;        // LDS budget of a work item's field + current blocks
+  i64 j_doubles;          // budget of its current block
   const int64_t *tile_off, *tile_np, *bin_start;
   const int *n3;
   const int64_t *bin_base;
@@ -391,19 +473,24 @@ struct ChunkArgs {
 };
 
 // A unit is one z-plane of the sorting grid of one tile.  Its chunks are
-// runs of at most `maxrows` whole rows (cells of one y-row) and at most `ch`
-// particles; rows denser than that are cut.  A chunk therefore spans a compact block of
+// runs of whole rows (cells of one y-row) of at most `ch` particles whose
+// field and current blocks fit the LDS budget; rows denser than that are
+// cut.  A chunk therefore spans a compact block of
 // cells: the tile's width in x, a few rows in y, one plane in z.  Without
 // sorting information the tile is one unit cut into runs of ch particles.
 template <bool WRITE>
 PXR_DEV int walk_unit(const ChunkArgs &a, int t, int z, i64 pos) {
   int nch = 0;
-  auto emit = [&](int64_t b, int64_t n) {
+  auto emit = [&](int64_t b, int64_t n, int zz, int y0, int y1) {
     if (WRITE && pos + nch < a.maxchunks) {
       Chunk c;
       c.begin = b;
       c.n = (int)n;
       c.tile = t;
+      c.z = zz;
+      c.y0 = y0;
+      c.y1 = y1;
+      c.pad = 0;
       a.out[pos + nch] = c;
     }
     nch++;
@@ -412,20 +499,30 @@ PXR_DEV int walk_unit(const ChunkArgs &a, int t, int z, i64 pos) {
   if (np <= 0) return 0;
   if (!a.bin_start) {
     if (z != 0) return 0;
-    for (int64_t b = 0; b < np; b += a.ch) emit(off + b, min((int64_t)a.ch, np - b));
+    for (int64_t b = 0; b < np; b += a.ch) emit(off + b, min((int64_t)a.ch, np - b), -1, 0, 0);
     return nch;
   }
   const int nx3 = a.n3[3 * t], ny3 = a.n3[3 * t + 1], nz3 = a.n3[3 * t + 2];
   if (z >= nz3) return 0;
   const int64_t base = a.bin_base[t] + (int64_t)z * ny3 * nx3;
   int64_t cb = 0, cnt = 0;
-  int y0 = 0;  // first row of the open chunk
+  int y0 = 0, ylast = 0;  // first and last row of the open chunk
   for (int y = 0; y < ny3; y++) {
     const int64_t row = base + (int64_t)y * nx3;
     const int64_t rb = a.bin_start[row], rn = a.bin_start[row + nx3] - rb;
     if (rn <= 0) continue;
-    if (cnt > 0 && (cnt + rn > a.ch || y - y0 >= a.maxrows)) {
-      emit(cb, cnt);
+    bool fits = true;
+    if (cnt > 0) {
+      double ulo[3], uhi[3];
+      int FR0[3], fe[3], JR0[3], je[3];
+      rows_range(a.bg, t, z, y0, y, ulo, uhi);
+      blocks_for(a.bg, ulo, uhi, FR0, fe, JR0, je);
+      // current block, with room for the odd pitches of k_fused
+      const i64 jpz = ((i64)(je[0] | 1) * je[1]) | 1;
+      fits = 6 * (i64)fe[0] * fe[1] * fe[2] <= a.lds_doubles && 3 * jpz * je[2] <= a.j_doubles;
+    }
+    if (cnt > 0 && (cnt + rn > a.ch || !fits)) {
+      emit(cb, cnt, z, y0, ylast);
       cnt = 0;
     }
     if (cnt == 0) {
@@ -433,13 +530,15 @@ PXR_DEV int walk_unit(const ChunkArgs &a, int t, int z, i64 pos) {
       y0 = y;
     }
     cnt += rn;
-    while (cnt > a.ch) {
-      emit(cb, a.ch);
+    ylast = y;
+    while (cnt > a.ch) {  // a row denser than a chunk: cut it
+      emit(cb, a.ch, z, y0, y);
       cb += a.ch;
       cnt -= a.ch;
+      y0 = y;
     }
   }
-  if (cnt > 0) emit(cb, cnt);
+  if (cnt > 0) emit(cb, cnt, z, y0, ylast);
   return nch;
 }
 
@@ -478,11 +577,13 @@ struct FusedArgs {
   EsirkConst<double> ec;
   double q, dx, dy, dz;  // direct deposition
   unsigned long long *bad;
-  int lds_doubles;   // dynamic LDS size in doubles (field blocks)
-  int j_doubles;     // largest current block kept in LDS, in doubles
-  int spread;        // deposit phase: lanes take particles S apart
-  int *ovf_count;    // chunks whose blocks do not fit in LDS ...
-  int2 *ovf;         // ... (chunk, stage): stage 0 untouched, 1 pushed
+  BoxGeom bg;            // work-item block geometry
+  int lds_doubles;       // dynamic LDS size in doubles (field and current blocks)
+  int j_doubles;         // largest current block kept in LDS, in doubles
+  int spread;            // lanes take particles S apart
+  int jpad;              // odd pitches for the LDS current block
+  int *ovf_count;        // work items whose blocks do not fit in LDS ...
+  int2 *ovf;             // ... (work item, stage): 0 untouched, 1 pushed
   int ovf_cap;
 };
 
@@ -497,13 +598,6 @@ PXR_DEV void box_reset(Box6 &b) {
   }
 }
 
-PXR_DEV void box_add(Box6 &b, const int *lo, const int *hi) {
-  for (int d = 0; d < 3; d++) {
-    b.lo[d] = min(b.lo[d], lo[d]);
-    b.hi[d] = max(b.hi[d], hi[d]);
-  }
-}
-
 PXR_DEV void wave_minmax(Box6 &b) {
 #pragma unroll
   for (int d = 0; d < 3; d++) {
@@ -512,22 +606,6 @@ PXR_DEV void wave_minmax(Box6 &b) {
       b.lo[d] = min(b.lo[d], __shfl_xor(b.lo[d], o));
       b.hi[d] = max(b.hi[d], __shfl_xor(b.hi[d], o));
     }
-  }
-}
-
-// node index bounds of the gather stencil, without the weights
-template <int N, int M>
-PXR_DEV void gather_index_bounds(double x, double y, double z, double s, int *lo, int *hi) {
-  const double p[3] = {x, y, z};
-#pragma unroll
-  for (int d = 0; d < 3; d++) {
-    const double v = p[d];
-    const int j = (N == 2 ? f_nint(v) : f_floor(v)) - (N == 1 ? 0 : 1);
-    int j0;
-    if (M == 0 || M == 2) j0 = f_floor((v + 0.5) - s) - (M == 2 ? 1 : 0);
-    else j0 = f_floor(v - s) - (M == 3 ? 1 : 0);
-    lo[d] = min(j, j0);
-    hi[d] = max(j + N, j0 + M);
   }
 }
 
@@ -547,31 +625,40 @@ struct GAcc {
   }
 };
 
-// Gather, push and position push of particle p with the field views v;
-// extends the box of the current it will deposit.
-template <int N, bool LO, int PUSHER, int DEPO, typename V>
-PXR_DEV void gpd_push_particle(const FusedArgs &a, i64 p, const V *v, Box6 &dep) {
+PXR_DEV bool in_block(const int *lo, const int *hi, const int *R0, const int *e) {
+  return lo[0] >= R0[0] && lo[1] >= R0[1] && lo[2] >= R0[2] && hi[0] < R0[0] + e[0] &&
+         hi[1] < R0[1] + e[1] && hi[2] < R0[2] + e[2];
+}
+
+// Gather, push and position push of particle p.  Fields are read from the
+// LDS block when the stencil lies inside it (views fl), else from global
+// memory (fg).  Extends the box of the current the particle will deposit.
+template <int N, bool LO, int PUSHER, int DEPO, typename VL, typename VG>
+PXR_DEV void gpd_push(const FusedArgs &a, i64 p, const VL *fl, const VG *fg, const int *FR0,
+                      const int *fe, Box6 &dep) {
   constexpr int M = LO ? N - 1 : N;
   const GridGeom &g = a.g;
+  const double x = a.x[p], y = a.y[p], z = a.z[p];
   GatherStencil<N, M, double> st;
-  gather_stencil<N, M, double>(st, (a.x[p] - a.xmin) * a.dxi, (a.y[p] - a.ymin) * a.dyi,
-                               (a.z[p] - a.zmin) * a.dzi, a.s);
+  gather_stencil<N, M, double>(st, (x - a.xmin) * a.dxi, (y - a.ymin) * a.dyi,
+                               (z - a.zmin) * a.dzi, a.s);
   int l2[3], h2[3];
   gather_bounds<N, M, double>(st, l2, h2);
   if (!g.inside(l2[0], l2[1], l2[2]) || !g.inside(h2[0], h2[1], h2[2])) {
-    count_bad(a.bad);
+    count_bad(a.bad);  // outside the arrays: skipped, as the reference cannot hold it
     return;
   }
   double E[6];
-  gather_apply<N, M, double>(st, v, E);
+  if (in_block(l2, h2, FR0, fe)) gather_apply<N, M, double>(st, fl, E);
+  else gather_apply<N, M, double>(st, fg, E);
   double u[3] = {a.ux[p], a.uy[p], a.uz[p]};
   double gi = a.gi[p];
   if (PUSHER == 1) vay_push<double>(u, gi, E[0], E[1], E[2], E[3], E[4], E[5], a.cst);
   else boris_push<double>(u, gi, E[0], E[1], E[2], E[3], E[4], E[5], a.cst);
   // pxr_pushxyz (boris_3d.F90:880-882)
-  const double xn = a.x[p] + u[0] * gi * a.dt;
-  const double yn = a.y[p] + u[1] * gi * a.dt;
-  const double zn = a.z[p] + u[2] * gi * a.dt;
+  const double xn = x + u[0] * gi * a.dt;
+  const double yn = y + u[1] * gi * a.dt;
+  const double zn = z + u[2] * gi * a.dt;
   a.ux[p] = u[0];
   a.uy[p] = u[1];
   a.uz[p] = u[2];
@@ -590,13 +677,16 @@ PXR_DEV void gpd_push_particle(const FusedArgs &a, i64 p, const V *v, Box6 &dep)
                               a.zmin, a.dt, a.dx, a.dy, a.dz, a.s);
     direct_box<N>(in, b0, b1);
   }
-  box_add(dep, b0, b1);
+  for (int d = 0; d < 3; d++) {
+    dep.lo[d] = min(dep.lo[d], b0[d]);
+    dep.hi[d] = max(dep.hi[d], b1[d]);
+  }
 }
 
-// Deposit particle p (its pushed state is read back from memory).  Returns
+// Deposit pushed particle p (its state is read back from memory).  Returns
 // false when the particle left the arrays (counted, skipped).
 template <int N, int DEPO, typename Acc>
-PXR_DEV bool gpd_deposit_general(const FusedArgs &a, i64 p, const Acc &acc) {
+PXR_DEV bool gpd_deposit(const FusedArgs &a, i64 p, const Acc &acc) {
   const GridGeom &g = a.g;
   int b0[3], b1[3];
   if (DEPO == 0) {
@@ -618,6 +708,22 @@ PXR_DEV bool gpd_deposit_general(const FusedArgs &a, i64 p, const Acc &acc) {
   return true;
 }
 
+// Persistent kernel over the work items.  Per work item:
+//   1. the node box of its field block, from the sorting-grid cells that
+//      hold its particles (or, without sorting information, from one pass
+//      over the positions); the six field blocks are staged into LDS;
+//   2. gather (from LDS; a particle whose stencil leaves the block, e.g.
+//      after a push without re-sorting, reads global memory), push and
+//      position push, lanes on consecutive particles; the box of the
+//      currents is reduced across the workgroup;
+//   3. the current block is zeroed in LDS and the pushed particles are
+//      deposited with LDS atomics; lanes of a wave take particles S apart
+//      (one or more cells apart in a cell-sorted run) so that their atomics
+//      rarely meet;
+//   4. the current block is added to global J.
+// A work item whose field block does not fit in LDS is left whole to
+// k_fused_overflow; one whose current block does not fit is left to it
+// after the push.
 template <int BS, int N, bool LO, int PUSHER, int DEPO>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void k_fused(FusedArgs a) {
   constexpr int M = LO ? N - 1 : N;
@@ -627,6 +733,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void k_
   const GridGeom &g = a.g;
   const int glo[3] = {-g.gx, -g.gy, -g.gz};
   const int ghi[3] = {g.nx + g.gx, g.ny + g.gy, g.nz + g.gz};
+  const double x0[3] = {a.xmin, a.ymin, a.zmin}, di[3] = {a.dxi, a.dyi, a.dzi};
   const int nch = *a.nchunks;
   for (;;) {
     if (threadIdx.x == 0) {
@@ -641,21 +748,88 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void k_
     if (c >= nch) break;
     const Chunk ch = a.chunks[c];
     const i64 p0 = ch.begin;
-    // ---- 1. box of the gather stencils ---------------------------------
-    Box6 b;
-    box_reset(b);
-    for (int k = threadIdx.x; k < ch.n; k += BS) {
-      const i64 p = p0 + k;
-      int l2[3], h2[3];
-      gather_index_bounds<N, M>((a.x[p] - a.xmin) * a.dxi, (a.y[p] - a.ymin) * a.dyi,
-                                (a.z[p] - a.zmin) * a.dzi, a.s, l2, h2);
-      box_add(b, l2, h2);
+    // ---- 1. field block -------------------------------------------------
+    double ulo[3], uhi[3];
+    if (ch.z >= 0) {
+      rows_range(a.bg, ch.tile, ch.z, ch.y0, ch.y1, ulo, uhi);
+    } else {
+      // no sorting information: one pass over the positions, in whole cells
+      Box6 b;
+      box_reset(b);
+      for (int k = threadIdx.x; k < ch.n; k += BS) {
+        const i64 p = p0 + k;
+        const double pp[3] = {a.x[p], a.y[p], a.z[p]};
+        for (int d = 0; d < 3; d++) {
+          const int cc = f_floor((pp[d] - x0[d]) * di[d]);
+          b.lo[d] = min(b.lo[d], cc);
+          b.hi[d] = max(b.hi[d], cc);
+        }
+      }
+      wave_minmax(b);
+      if ((threadIdx.x & 63) == 0)
+        for (int d = 0; d < 3; d++) {
+          atomicMin(&s_box[d], b.lo[d]);
+          atomicMax(&s_box[3 + d], b.hi[d]);
+        }
+      __syncthreads();
+      for (int d = 0; d < 3; d++) {
+        const int lo = max(s_box[d], glo[d] - 4), hi = min(s_box[3 + d], ghi[d] + 4);
+        ulo[d] = lo - 1e-7;
+        uhi[d] = hi + 1.0 + 1e-7;
+      }
+      __syncthreads();
+      if (threadIdx.x == 0)
+        for (int d = 0; d < 3; d++) {
+          s_box[d] = INT_MAX;
+          s_box[3 + d] = INT_MIN;
+        }
     }
-    wave_minmax(b);
+    int FR0[3], fe[3];
+    for (int d = 0; d < 3; d++) {
+      int lo, hi;
+      gather_range(a.bg, ulo[d], uhi[d], lo, hi);
+      FR0[d] = max(lo, glo[d]);
+      fe[d] = max(min(hi, ghi[d]) - FR0[d] + 1, 0);
+    }
+    const i64 nb64 = (i64)fe[0] * fe[1] * fe[2];
+    if (nb64 <= 0 || 6 * nb64 > a.lds_doubles) {
+      if (threadIdx.x == 0) {
+        const int slot = atomicAdd(a.ovf_count, 1);
+        if (slot < a.ovf_cap) a.ovf[slot] = make_int2(c, 0);
+      }
+      __syncthreads();
+      continue;
+    }
+    const int nb = (int)nb64;
+    for (int f = 0; f < 6; f++) {
+      const double *src = a.F[f];
+      for (int idx = threadIdx.x; idx < nb; idx += BS) {
+        const int i = FR0[0] + idx % fe[0];
+        const int j = FR0[1] + (idx / fe[0]) % fe[1];
+        const int k = FR0[2] + idx / (fe[0] * fe[1]);
+        smem[f * nb + idx] = src[g.idx(i, j, k)];
+      }
+    }
+    __syncthreads();
+    // ---- 2. gather, push, position push ---------------------------------
+    Box6 dep;
+    box_reset(dep);
+    {
+      LdsView<double> fl[6];
+      FieldView<double> fg[6];
+      for (int f = 0; f < 6; f++) {
+        fl[f] = LdsView<double>{smem + f * nb, FR0[0], FR0[1], FR0[2], fe[0], fe[0] * fe[1]};
+        fg[f] = FieldView<double>{a.F[f] + g.idx(0, 0, 0), g.NX, g.NX * g.NY};
+      }
+      for (int k = threadIdx.x; k < ch.n; k += BS)
+        gpd_push<N, LO, PUSHER, DEPO>(a, p0 + k, fl, fg, FR0, fe, dep);
+    }
+    wave_minmax(dep);
+    __syncthreads();  // field block no longer read
     if ((threadIdx.x & 63) == 0)
       for (int d = 0; d < 3; d++) {
-        atomicMin(&s_box[d], b.lo[d]);
-        atomicMax(&s_box[3 + d], b.hi[d]);
+        atomicMin(&s_box[d], dep.lo[d]);
+        atomicMax(&s_box[3 + d], dep.hi[d]);
       }
     __syncthreads();
     int R0[3], e[3];
@@ -664,63 +838,15 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void k_
       const int R1 = min(s_box[3 + d], ghi[d]);
       e[d] = R1 >= R0[d] ? R1 - R0[d] + 1 : 0;
     }
-    const i64 nb64 = (i64)e[0] * e[1] * e[2];
-    const bool lds_f = nb64 > 0 && 6 * nb64 <= a.lds_doubles;
-    const int nb = lds_f ? (int)nb64 : 0;
-    if (lds_f) {
-      for (int f = 0; f < 6; f++) {
-        const double *src = a.F[f];
-        for (int idx = threadIdx.x; idx < nb; idx += BS) {
-          const int i = R0[0] + idx % e[0];
-          const int j = R0[1] + (idx / e[0]) % e[1];
-          const int k = R0[2] + idx / (e[0] * e[1]);
-          smem[f * nb + idx] = src[g.idx(i, j, k)];
-        }
-      }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0)
-      for (int d = 0; d < 3; d++) {
-        s_box[d] = INT_MAX;
-        s_box[3 + d] = INT_MIN;
-      }
-    if (!lds_f) {
-      // block too large for LDS: left to the overflow kernel
-      if (threadIdx.x == 0) {
-        const int slot = atomicAdd(a.ovf_count, 1);
-        if (slot < a.ovf_cap) a.ovf[slot] = make_int2(c, 0);
-      }
-      __syncthreads();
-      continue;
-    }
-    // ---- 2. gather, push, position push ---------------------------------
-    Box6 dep;
-    box_reset(dep);
-    {
-      LdsView<double> v[6];
-      for (int f = 0; f < 6; f++)
-        v[f] = LdsView<double>{smem + f * nb, R0[0], R0[1], R0[2], e[0], e[0] * e[1]};
-      for (int k = threadIdx.x; k < ch.n; k += BS)
-        gpd_push_particle<N, LO, PUSHER, DEPO>(a, p0 + k, v, dep);
-    }
-    wave_minmax(dep);
-    __syncthreads();  // field block no longer read; s_box reset visible
-    if ((threadIdx.x & 63) == 0)
-      for (int d = 0; d < 3; d++) {
-        atomicMin(&s_box[d], dep.lo[d]);
-        atomicMax(&s_box[3 + d], dep.hi[d]);
-      }
-    __syncthreads();
-    for (int d = 0; d < 3; d++) {
-      R0[d] = max(s_box[d], glo[d]);
-      const int R1 = min(s_box[3 + d], ghi[d]);
-      e[d] = R1 >= R0[d] ? R1 - R0[d] + 1 : 0;
-    }
-    const i64 nj64 = (i64)e[0] * e[1] * e[2];
-    const bool lds_j = nj64 > 0 && 3 * nj64 <= a.j_doubles;
-    if (!lds_j) {
-      // pushed, but the current block does not fit: deposit in the overflow kernel
-      if (threadIdx.x == 0) {
+    // LDS layout of the current block: odd row and plane pitches, so that
+    // lanes on cells an even number apart fall on different banks
+    const int py = a.jpad ? (e[0] | 1) : e[0];
+    const int pzr = py * e[1];
+    const int pz = a.jpad ? (pzr | 1) : pzr;
+    const i64 nj64 = (i64)pz * e[2];
+    if ((i64)e[0] * e[1] * e[2] <= 0 || 3 * nj64 > a.j_doubles) {
+      // pushed; the deposition is left to the overflow kernel
+      if (threadIdx.x == 0 && (i64)e[0] * e[1] * e[2] > 0) {
         const int slot = atomicAdd(a.ovf_count, 1);
         if (slot < a.ovf_cap) a.ovf[slot] = make_int2(c, 1);
       }
@@ -730,26 +856,26 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void k_
     const int nj = (int)nj64;
     for (int idx = threadIdx.x; idx < 3 * nj; idx += BS) smem[idx] = 0.0;
     __syncthreads();
-    // ---- 3. deposit the pushed particles ---------------------------------
-    // lanes of a wave take particles S apart (one or more cells apart in a
-    // cell-sorted chunk) so that their LDS atomics rarely hit one address
-    const int S = (ch.n + 63) >> 6;
+    // ---- 3. deposit -------------------------------------------------------
     {
-      const LdsAcc lacc{{smem, smem + nj, smem + 2 * nj}, R0[0], R0[1], R0[2], e[0], e[0] * e[1]};
+      const LdsAcc la{{smem, smem + nj, smem + 2 * nj}, R0[0], R0[1], R0[2], py, pz};
+      const int S = (ch.n + 63) >> 6;
       for (int kk = threadIdx.x; kk < S * 64; kk += BS) {
         const int k = a.spread ? (kk & 63) * S + (kk >> 6) : kk;
-        if (k >= ch.n) continue;
-        if (!gpd_deposit_general<N, DEPO>(a, p0 + k, lacc)) count_bad(a.bad);
+        if (k < ch.n && !gpd_deposit<N, DEPO>(a, p0 + k, la)) count_bad(a.bad);
       }
     }
     __syncthreads();
-    for (int idx = threadIdx.x; idx < nj; idx += BS) {
-      const int i = R0[0] + idx % e[0];
-      const int j = R0[1] + (idx / e[0]) % e[1];
-      const int k = R0[2] + idx / (e[0] * e[1]);
-      const i64 gi_ = g.idx(i, j, k);
+    // ---- 4. flush ----------------------------------------------------------
+    const int ne = e[0] * e[1] * e[2];
+    for (int idx = threadIdx.x; idx < ne; idx += BS) {
+      const int i = idx % e[0];
+      const int j = (idx / e[0]) % e[1];
+      const int k = idx / (e[0] * e[1]);
+      const i64 gi_ = g.idx(R0[0] + i, R0[1] + j, R0[2] + k);
+      const int li = i + py * j + pz * k;
       for (int c3 = 0; c3 < 3; c3++) {
-        const double v = smem[c3 * nj + idx];
+        const double v = smem[c3 * nj + li];
         if (v != 0.0) unsafeAtomicAdd(a.J[c3] + gi_, v);
       }
     }
@@ -757,30 +883,31 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void k_
   }
 }
 
-// Chunks whose field or current block did not fit in LDS: gather from
-// global memory and deposit with global atomics, one workgroup per chunk.
-// Its particles were already pushed when the stage is 1.
+// Work items whose blocks did not fit in LDS: gather from global memory and
+// deposit with global atomics, one workgroup per work item.  Stage 1: the
+// particles were already pushed by k_fused.
 template <int BS, int N, bool LO, int PUSHER, int DEPO>
 __global__ __launch_bounds__(BS) void k_fused_overflow(FusedArgs a) {
   const int n = min(*a.ovf_count, a.ovf_cap);
   const GridGeom &g = a.g;
+  FieldView<double> fg[6];
+  for (int f = 0; f < 6; f++) fg[f] = FieldView<double>{a.F[f] + g.idx(0, 0, 0), g.NX, g.NX * g.NY};
+  const GAcc ga{{a.J[0], a.J[1], a.J[2]}, g};
+  const int none[3] = {0, 0, 0};  // empty block: every gather reads global memory
   for (int o = blockIdx.x; o < n; o += gridDim.x) {
-    const int2 e = a.ovf[o];
-    const Chunk ch = a.chunks[e.x];
-    if (e.y == 0) {
-      FieldView<double> v[6];
-      for (int f = 0; f < 6; f++) v[f] = FieldView<double>{a.F[f] + g.idx(0, 0, 0), g.NX, g.NX * g.NY};
+    const int2 ov = a.ovf[o];
+    const Chunk ch = a.chunks[ov.x];
+    if (ov.y == 0) {
       Box6 dep;
       box_reset(dep);
       for (int k = threadIdx.x; k < ch.n; k += BS)
-        gpd_push_particle<N, LO, PUSHER, DEPO>(a, ch.begin + k, v, dep);
+        gpd_push<N, LO, PUSHER, DEPO>(a, ch.begin + k, fg, fg, none, none, dep);
     }
     __syncthreads();
-    const GAcc acc{{a.J[0], a.J[1], a.J[2]}, a.g};
     const int S = (ch.n + 63) >> 6;
     for (int kk = threadIdx.x; kk < S * 64; kk += BS) {
       const int k = a.spread ? (kk & 63) * S + (kk >> 6) : kk;
-      if (k < ch.n && !gpd_deposit_general<N, DEPO>(a, ch.begin + k, acc)) count_bad(a.bad);
+      if (k < ch.n && !gpd_deposit<N, DEPO>(a, ch.begin + k, ga)) count_bad(a.bad);
     }
     __syncthreads();
   }
@@ -799,8 +926,10 @@ struct Tuning {
                           // of static LDS) fit in a CU's 160 KiB
   int wg_per_cu = 2;      // persistent workgroups per CU
   int spread = 1;         // lane-spread mapping in the deposit phase
-  int chunk = 2048;       // default particles per work item
+  int chunk = 4096;       // default particles per work item
   int j_lds_bytes = -1;   // cap on the LDS current block (negative: all of lds_bytes)
+  int margin_pct = 0;     // motion margin of the current block, in 1/100 cell
+  int jpad = 0;           // odd row/plane pitch of the LDS current block
 };
 static Tuning g_tune;
 }  // namespace pxr
@@ -829,6 +958,11 @@ int pxrhip_set_tuning(const char *key, int64_t value) {
     g_tune.spread = value != 0;
   } else if (!strcmp(key, "fused.j_lds_bytes")) {
     g_tune.j_lds_bytes = value < 0 ? -1 : (value > (1 << 30) ? (1 << 30) : (int)value);
+  } else if (!strcmp(key, "fused.jpad")) {
+    g_tune.jpad = value != 0;
+  } else if (!strcmp(key, "fused.margin_pct")) {
+    if (value < 0 || value > 200) return set_error(PXRHIP_ERR_ARG, "fused.margin_pct must be in [0, 200]");
+    g_tune.margin_pct = (int)value;
   } else if (!strcmp(key, "fused.chunk")) {
     if (value < 1 || value > (1 << 20)) return set_error(PXRHIP_ERR_ARG, "fused.chunk must be in [1, 2^20]");
     g_tune.chunk = (int)value;
@@ -845,6 +979,8 @@ int pxrhip_get_tuning(const char *key, int64_t *value) {
   else if (!strcmp(key, "fused.spread")) *value = g_tune.spread;
   else if (!strcmp(key, "fused.chunk")) *value = g_tune.chunk;
   else if (!strcmp(key, "fused.j_lds_bytes")) *value = g_tune.j_lds_bytes;
+  else if (!strcmp(key, "fused.margin_pct")) *value = g_tune.margin_pct;
+  else if (!strcmp(key, "fused.jpad")) *value = g_tune.jpad;
   else return set_error(PXRHIP_ERR_ARG, "unknown tuning key '%s'", key);
   return PXRHIP_OK;
 }
@@ -957,25 +1093,34 @@ int pxrhip_tiles_gather_push_deposit(const pxrhip_grid *grid, const pxrhip_speci
   // chunk table
   const i64 cap = sp->capacity;
   const int nzmax = tt->nzmax;
-  // rows of a chunk: cells spanned on each axis by particles of r rows of
-  // the sorting grid (one cell more for the half-cell offset of the sorting
-  // grid), plus the order-N stencil (field block) and one cell of motion
-  // (current block)
-  int maxrows = 1;
+  // block geometry of the work items
+  BoxGeom bg;
   {
-    const double sx = grid->sorting_dx / grid->dx, sy = grid->sorting_dy / grid->dy;
-    const double sz = grid->sorting_dz / grid->dz;
-    const double wx = ceil((tt->nxmax - 1) * sx) + 1, wz = ceil(sz) + 1;
-    const double fl = g_tune.lds_bytes / 8.0;
-    const double jl = g_tune.j_lds_bytes < 0 ? fl : g_tune.j_lds_bytes / 8.0;
-    for (int r = 2; r <= 4096; r++) {
-      const double wy = ceil(r * sy) + 1;
-      const double fb = 6.0 * (wx + order + 1) * (wy + order + 1) * (wz + order + 1);
-      const double jb = 3.0 * (wx + order + 2) * (wy + order + 2) * (wz + order + 2);
-      if (fb > fl || jb > jl) break;
-      maxrows = r;
+    const double x0[3] = {grid->xmin, grid->ymin, grid->zmin};
+    const double dd[3] = {grid->dx, grid->dy, grid->dz};
+    const double sd[3] = {grid->sorting_dx, grid->sorting_dy, grid->sorting_dz};
+    const int64_t nn[3] = {grid->nx, grid->ny, grid->nz}, ng[3] = {grid->ngx, grid->ngy, grid->ngz};
+    const double margin = g_tune.margin_pct / 100.0;
+    for (int d = 0; d < 3; d++) {
+      bg.x0[d] = x0[d];
+      bg.di[d] = 1.0 / dd[d];
+      bg.sd[d] = sd[d];
+      const double cfl = dt / dd[d];  // |v| < c = 1: at most dt/dx cells per step
+      bg.mv[d] = margin < cfl ? margin : cfl;
+      bg.glo[d] = (int)-ng[d];
+      bg.ghi[d] = (int)(nn[d] + ng[d]);
     }
+    bg.s = 0.5;
+    bg.N = (int)order;
+    bg.M = l_lower_order_in_v ? (int)order - 1 : (int)order;
+    bg.depo = depo_algo ? 3 : 0;
+    bg.sorg = tt->sorg;
+    bg.tbox = tt->tbox;
+    bg.n3 = tt->n3;
   }
+  const i64 lds_doubles = g_tune.lds_bytes / 8;
+  const i64 j_doubles = g_tune.j_lds_bytes < 0 || g_tune.j_lds_bytes / 8 > lds_doubles
+                            ? lds_doubles : g_tune.j_lds_bytes / 8;
   const i64 nunits = (i64)tt->ntiles * nzmax;
   const i64 maxchunks = tt->nrows + (cap / ch + 1) + nunits + 16;
   const i64 nblk = (nunits + SCAN_B - 1) / SCAN_B;
@@ -990,7 +1135,9 @@ int pxrhip_tiles_gather_push_deposit(const pxrhip_grid *grid, const pxrhip_speci
   ca.ntiles = tt->ntiles;
   ca.ch = ch;
   ca.nzmax = nzmax;
-  ca.maxrows = maxrows;
+  ca.bg = bg;
+  ca.lds_doubles = lds_doubles;
+  ca.j_doubles = j_doubles;
   ca.tile_off = sp->tile_off;
   ca.tile_np = sp->tile_np;
   ca.bin_start = sp->bin_start;
@@ -1034,10 +1181,11 @@ int pxrhip_tiles_gather_push_deposit(const pxrhip_grid *grid, const pxrhip_speci
                                   grid->dy, grid->dz);
   a.q = sp->q; a.dx = grid->dx; a.dy = grid->dy; a.dz = grid->dz;
   a.bad = rt().d_bad;
-  a.lds_doubles = g_tune.lds_bytes / 8;
-  a.j_doubles = g_tune.j_lds_bytes < 0 || g_tune.j_lds_bytes / 8 > a.lds_doubles ? a.lds_doubles
-                                                                         : g_tune.j_lds_bytes / 8;
+  a.bg = bg;
+  a.lds_doubles = (int)lds_doubles;
+  a.j_doubles = (int)j_doubles;
   a.spread = g_tune.spread;
+  a.jpad = g_tune.jpad;
   a.ovf_count = ctl + 2;
   a.ovf = ovf;
   a.ovf_cap = maxchunks < INT_MAX ? (int)maxchunks : INT_MAX;

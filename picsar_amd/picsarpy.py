@@ -252,6 +252,24 @@ class _Picsar:
                   int(current_depo_algo), int(bool(self.l_nodalgrid)))
         c.finish()
 
+    # charge_deposition_3d.F90:2081
+    def pxr_depose_rho_n(self, rho, np_, xp, yp, zp, w, q, xmin, ymin, zmin, dx, dy, dz, nx, ny,
+                         nz, nxguard, nyguard, nzguard, nox, noy, noz, l_particles_weight=True,
+                         l4symtry=False):
+        n = int(np_)
+        shp = _gshape(nx, ny, nz, nxguard, nyguard, nzguard)
+        c = _Call()
+        R = [c.arr(rho, shp[0] * shp[1] * shp[2], "rho", write=True, fortran_shape=shp)]
+        P = [c.arr(a, n, nm) for a, nm in ((xp, "xp"), (yp, "yp"), (zp, "zp"))]
+        P.append(c.arr(w, n if l_particles_weight else 1, "w"))
+        R, P = c.ptrs(R), c.ptrs(P)
+        c.before()
+        _lib.call("pxrhip_depose_rho_n", R[0], n, *P, float(q), float(xmin), float(ymin),
+                  float(zmin), float(dx), float(dy), float(dz), int(nx), int(ny), int(nz),
+                  int(nxguard), int(nyguard), int(nzguard), int(nox), int(noy), int(noz),
+                  int(bool(l_particles_weight)), int(bool(l4symtry)))
+        c.finish()
+
     # yee.F90:637
     def pxrpush_em3d_evec(self, xlo, xhi, ylo, yhi, zlo, zhi, ex, exlo, exhi, ey, eylo, eyhi, ez,
                           ezlo, ezhi, bx, bxlo, bxhi, by, bylo, byhi, bz, bzlo, bzhi, jx, jxlo,

@@ -3,7 +3,7 @@
 Run in a tree where ``oracle/_ref/libpxrref.so`` has been built from the PICSAR
 sources (``bash oracle/build_ref.sh``):
 
-    python tests/golden/gen_golden.py
+    python tests/golden/gen_golden.py [family ...]   # default: every family
 
 Each ``.npz`` holds the seeded inputs of one kernel family and the outputs the
 reference's compiled FP64 routines produced for them.  Inputs include the
@@ -115,6 +115,23 @@ def gen_deposit(ref, rng):
     np.savez_compressed(os.path.join(HERE, "deposit.npz"), **d)
 
 
+def gen_rho(ref, rng):
+    """pxr_depose_rho_n and depose_rho_scalar_n_n_n, orders 1..3; the
+    positions include nodes, half nodes and guard cells."""
+    n = 256
+    xp, yp, zp = positions(rng, n, lo=-1.0, hi=None)
+    w = rng.uniform(0.5, 1.5, n)
+    q = 0.7
+    d = dict(xp=xp, yp=yp, zp=zp, w=w, q=q)
+    for order in (1, 2, 3):
+        for scalar in (False, True):
+            rho = np.zeros(SHAPE, order="F")
+            ref.depose_rho(rho, xp, yp, zp, w, q, XMIN, YMIN, ZMIN, DX, DY, DZ, NX, NY, NZ, NG,
+                           NG, NG, order, scalar=scalar)
+            d[f"o{order}_{'scalar' if scalar else 'n'}"] = rho
+    np.savez_compressed(os.path.join(HERE, "rho.npz"), **d)
+
+
 def gen_fdtd(ref, rng):
     F = [grid(rng) for _ in range(9)]
     d = {f"in{i}": f for i, f in enumerate(F)}
@@ -171,12 +188,14 @@ def main():
     ref = load_reference()
     if ref is None:
         sys.exit("oracle/_ref/libpxrref.so is missing: run bash oracle/build_ref.sh first")
-    rng = np.random.default_rng(20241202)
-    gen_gather(ref, rng)
-    gen_push(ref, rng)
-    gen_deposit(ref, rng)
-    gen_fdtd(ref, rng)
-    gen_sort(ref, rng)
+    which = set(sys.argv[1:]) or {"gather", "push", "deposit", "fdtd", "sort", "rho"}
+    rng = np.random.default_rng(20241202)  # one stream for the first five families
+    for name, fn in (("gather", gen_gather), ("push", gen_push), ("deposit", gen_deposit),
+                     ("fdtd", gen_fdtd), ("sort", gen_sort)):
+        if name in which:
+            fn(ref, rng)
+    if "rho" in which:
+        gen_rho(ref, np.random.default_rng(20241203))
     meta = dict(nx=NX, ny=NY, nz=NZ, ng=NG, dx=DX, dy=DY, dz=DZ, xmin=XMIN, ymin=YMIN, zmin=ZMIN)
     np.savez(os.path.join(HERE, "grid.npz"), **meta)
     for f in sorted(os.listdir(HERE)):

@@ -38,6 +38,13 @@ class OracleState:
         return (self.xmin, self.ymin, self.zmin)
 
     def step(self):
+        self.gather_push_deposit(zero_j=True)
+        self.fold_currents_and_advance_fields()
+        self.wrap()
+
+    def gather_push_deposit(self, zero_j=True):
+        """Gather + push of every species, then deposition of the pushed
+        particles into J (zeroed first when zero_j)."""
         o, f = self.orc, self.f
         nx, ny, nz, ng = self.nx, self.ny, self.nz, self.ng
         d = (self.dx, self.dy, self.dz)
@@ -56,8 +63,9 @@ class OracleState:
             else:
                 o.boris_push_u_3d(*u, s["gaminv"], *e, s["q"], s["m"], dt)
             o.pushxyz(s["x"], s["y"], s["z"], *u, s["gaminv"], dt)
-        for k in ("jx", "jy", "jz"):
-            f[k][...] = 0.0
+        if zero_j:
+            for k in ("jx", "jy", "jz"):
+                f[k][...] = 0.0
         for s in self.species:
             args = (f["jx"], f["jy"], f["jz"], s["x"], s["y"], s["z"], s["ux"], s["uy"], s["uz"],
                     s["gaminv"], s["w"], s["q"], *self.geom(), dt, *d, nx, ny, nz, ng, ng, ng,
@@ -65,6 +73,10 @@ class OracleState:
             bad = o.depose_esirkepov(*args) if self.depo == "esirkepov" else \
                 o.depose_direct(*args, False)
             assert bad == 0
+
+    def fold_currents_and_advance_fields(self):
+        o, f = self.orc, self.f
+        nx, ny, nz, ng = self.nx, self.ny, self.nz, self.ng
         for k in ("jx", "jy", "jz"):
             o.current_bc_periodic(f[k], nx, ny, nz, ng, ng, ng)
         self._bhalf()
@@ -73,6 +85,10 @@ class OracleState:
         self._bcs(("ex", "ey", "ez"))
         self._bhalf()
         self._bcs(("bx", "by", "bz"))
+
+    def wrap(self):
+        """Periodic wrap of the positions (particle_bcs)."""
+        nx, ny, nz = self.nx, self.ny, self.nz
         L = (nx * self.dx, ny * self.dy, nz * self.dz)
         x0 = self.geom()
         for s in self.species:

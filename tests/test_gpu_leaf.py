@@ -206,6 +206,69 @@ def test_deposit_golden(dev, pxr, golden, ggrid):
             assert rel(got, ref) < REL, (order, algo, rel(got, ref))
 
 
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_rho(dev, pxr, orc, golden, ggrid, order):
+    """pxr_depose_rho_n against the oracle (random particles, accumulating
+    into a non-zero rho) and the reference's golden vectors."""
+    rng = np.random.default_rng(300 + order)
+    n = 40000
+    dx, dy, dz = 0.9, 1.0, 1.2
+    xmin, ymin, zmin = 0.3, -1.0, 2.0
+    x, y, z = particles(rng, n, margin=1.0)
+    xp, yp, zp = xmin + x * dx, ymin + y * dy, zmin + z * dz
+    w = rng.uniform(0.5, 2.0, n)
+    r0 = np.asfortranarray(rng.standard_normal(SHAPE) * 0.1)
+    rr = r0.copy(order="F")
+    assert orc.depose_rho(rr, xp, yp, zp, w, -1.1, xmin, ymin, zmin, dx, dy, dz, NX, NY, NZ, NG,
+                          NG, NG, order) == 0
+    R = TG(dev, r0)
+    pxr.pxr_depose_rho_n(R, n, *[T(dev, a) for a in (xp, yp, zp, w)], -1.1, xmin, ymin, zmin,
+                         dx, dy, dz, NX, NY, NZ, NG, NG, NG, order, order, order)
+    assert rel(HG(R) - r0, rr - r0) < REL
+    d, g = golden("rho"), ggrid
+    ref = d[f"o{order}_n"]
+    R = TG(dev, np.zeros(ref.shape, order="F"))
+    pxr.pxr_depose_rho_n(R, len(d["xp"]), *[T(dev, d[k]) for k in ("xp", "yp", "zp", "w")],
+                         float(d["q"]), g["xmin"], g["ymin"], g["zmin"], g["dx"], g["dy"],
+                         g["dz"], g["nx"], g["ny"], g["nz"], g["ng"], g["ng"], g["ng"], order,
+                         order, order)
+    assert rel(HG(R), ref) < REL
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_esirkepov_continuity_on_gpu(dev, pxr, order):
+    """Discrete continuity of the GPU Esirkepov current: with rho of the same
+    order deposited on the GPU at the old and new positions,
+    (rho_new - rho_old)/dt + div J vanishes to round-off at every node.
+    Particles move up to ~0.8 cell, so they cross cell faces."""
+    rng = np.random.default_rng(400 + order)
+    n = 50000
+    dt, dx, dy, dz = 0.5, 1.0, 1.0, 1.0
+    x, y, z = particles(rng, n, margin=2.0)
+    u = [rng.uniform(-1.5, 1.5, n) for _ in range(3)]
+    g = 1 / np.sqrt(1 + u[0] ** 2 + u[1] ** 2 + u[2] ** 2)
+    w = rng.uniform(0.5, 1.5, n)
+    q = -0.8
+    J = [TG(dev, np.zeros(SHAPE, order="F")) for _ in range(3)]
+    pxr.depose_jxjyjz(*J, n, *[T(dev, a) for a in (x, y, z, *u, g, w)], q, 0.0, 0.0, 0.0, dt, dx,
+                      dy, dz, NX, NY, NZ, NG, NG, NG, order, order, order, 0)
+    xo, yo, zo = x - dt * (u[0] * g), y - dt * (u[1] * g), z - dt * (u[2] * g)
+    rho = []
+    for (a, b, c) in ((x, y, z), (xo, yo, zo)):
+        R = TG(dev, np.zeros(SHAPE, order="F"))
+        pxr.pxr_depose_rho_n(R, n, *[T(dev, v) for v in (a, b, c, w)], q, 0.0, 0.0, 0.0, dx, dy,
+                             dz, NX, NY, NZ, NG, NG, NG, order, order, order)
+        rho.append(HG(R))
+    Jx, Jy, Jz = (HG(j) for j in J)
+    div = np.zeros(SHAPE)
+    div[1:, :, :] += Jx[1:, :, :] - Jx[:-1, :, :]
+    div[:, 1:, :] += Jy[:, 1:, :] - Jy[:, :-1, :]
+    div[:, :, 1:] += Jz[:, :, 1:] - Jz[:, :, :-1]
+    res = (rho[0] - rho[1]) / dt + div
+    scale = np.abs(rho[0]).max() / dt
+    assert np.abs(res[1:, 1:, 1:]).max() < 1e-12 * scale, np.abs(res[1:, 1:, 1:]).max() / scale
+
+
 def test_fdtd_golden_bitexact(dev, pxr, golden, ggrid):
     d = golden("fdtd")
     g = ggrid

@@ -106,3 +106,18 @@ def test_sort(orc, golden, case):
     bad = orc.particle_bin_sorting(*P, *d["box" + case])
     assert bad == 0
     np.testing.assert_array_equal(np.stack(P), d["out" + case])
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_rho(orc, golden, ggrid, order):
+    """The oracle's charge density is pxr_depose_rho_n bit for bit; the
+    scalar routines group the order-2 products differently (last ulp)."""
+    d = golden("rho")
+    g = ggrid
+    rho = np.zeros(d[f"o{order}_n"].shape, order="F")
+    bad = orc.depose_rho(rho, d["xp"], d["yp"], d["zp"], d["w"], float(d["q"]), g["xmin"],
+                         g["ymin"], g["zmin"], g["dx"], g["dy"], g["dz"], g["nx"], g["ny"],
+                         g["nz"], g["ng"], g["ng"], g["ng"], order)
+    assert bad == 0
+    np.testing.assert_array_equal(rho, d[f"o{order}_n"])
+    assert _rel(rho, d[f"o{order}_scalar"]) < 1e-15
