@@ -202,3 +202,42 @@ def test_ten_steps(torch, orc):
     check_tiles(sim)
     compare_fields(sim, ref, tng, 1e-9)
     compare_particles(sim, ref, 1e-9, 1e-9)
+
+
+def test_c1_config_hundred_steps(torch, orc):
+    """The CPU-runnable configuration: 32^3 periodic thermal plasma, dx = 1,
+    dt = 0.95 CFL, electrons and ions at 8 ppc (u_th 0.01 c and 0.001 c),
+    Boris, order-1 direct deposition, Yee, one tile, 100 steps, FP64,
+    against the oracle's step.  FP64 atomics reorder sums on the GPU, so the
+    comparison after 100 coupled steps is at 1e-9."""
+    from picsar_amd.sim import Simulation
+    n, ppc = 32, 8
+    dt = default_dt(1.0, 1.0, 1.0)
+    sim = Simulation(n, n, n, 1.0, 1.0, 1.0, 0.0, 0.0, 0.0, 1, 1, 1, order=1,
+                     l_lower_order_in_v=True, pusher="boris", depo="direct", norder=2, dt=dt)
+    ref = OracleState(orc, n, n, n, sim.ng, 1.0, 1.0, 1.0, 0.0, 0.0, 0.0, dt, 1, True, "boris",
+                      "direct")
+    rng = np.random.default_rng(2024)
+    off = 0
+    for (q, m, u) in ((-1.0, 1.0, 0.01), (1.0, 1836.0, 0.001)):
+        p = list(thermal_plasma(rng, n, n, n, ppc, 1.0, 1.0, 1.0, 0.0, 0.0, 0.0, u,
+                                id_offset=off))
+        off += len(p[0])
+        p[6] = p[6] * (0.02 ** 2 / ppc)  # electron plasma frequency 0.02
+        ref.add_species(q, m, *p)
+        sim.add_species("e" if q < 0 else "i", q, m, *p, gaminv=ref.species[-1]["gaminv"])
+    for _ in range(100):
+        sim.step(1, check=False)
+        ref.step()
+    sim.check()
+    check_tiles(sim)
+    for k in ("ex", "ey", "ez", "bx", "by", "bz", "jx", "jy", "jz"):
+        assert rel(sim.fields[k].numpy(), ref.f[k]) <= 1e-9, k
+    for sp, rs in zip(sim.species, ref.species):
+        a = sp.arrays()
+        ia, ib = np.argsort(a["w"]), np.argsort(rs["w"])
+        np.testing.assert_array_equal(a["w"][ia], rs["w"][ib])
+        for k in ("x", "y", "z"):
+            assert np.abs(a[k][ia] - rs[k][ib]).max() <= 1e-9 * n, k
+        for k in ("ux", "uy", "uz"):
+            assert rel(a[k][ia], rs[k][ib]) <= 1e-9, k
