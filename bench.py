@@ -39,6 +39,7 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 BYTES_PER_PARTICLE_FP64 = 120  # read x,y,z,ux,uy,uz,gaminv,w; write x..gaminv
+BYTES_PER_PARTICLE_MIXED = 84  # positions FP64 (24 + 24), u, gaminv, w FP32 (20 + 16)
 BYTES_PER_CELL = 72            # read E, B (48 B); write J (24 B)
 OMEGA_PE = 0.02                # electron plasma frequency (c = dx = 1)
 
@@ -47,7 +48,8 @@ def profiled_traffic(args):
     """HBM bytes per fused launch from the committed counter summary
     (FETCH_SIZE and WRITE_SIZE passes scaled by a copy-kernel calibration),
     valid for the default workload and launch parameters only."""
-    if (args.n, args.ppc, args.tiles, args.chunk) != (128, 16, 8, 0) or args.tune:
+    if (args.n, args.ppc, args.tiles, args.chunk) != (128, 16, 8, 0) or args.tune or \
+            args.precision != "fp64" or args.workload != "thermal" or args.autotune:
         return None
     path = os.path.join(ROOT, "profiles", "fused_c2_hbm_traffic.md")
     try:
@@ -59,13 +61,42 @@ def profiled_traffic(args):
     return None
 
 
+def workload_name(args):
+    prec = "FP64" if args.precision == "fp64" else "mixed precision"
+    tiles = f"{(args.n + 1) // args.tiles}^3-node tiles"
+    if args.workload == "bunch":
+        return (f"Gaussian electron bunch (sigma 20 cells, peak 64 ppc, gamma 10) in a 1 ppc "
+                f"background, {args.n}^3 periodic, order-3 gather, Vay, order-3 Esirkepov, Yee, "
+                f"{tiles}, {prec}")
+    return (f"thermal plasma {args.n}^3 periodic, {args.ppc} ppc electrons + ions, order-3 "
+            f"gather, Vay, order-3 Esirkepov, Yee, {tiles}, {prec}")
+
+
 def build_workload(args):
     from picsar_amd.sim import Simulation
     n = args.n
     sim = Simulation(n, n, n, 1.0, 1.0, 1.0, 0.0, 0.0, 0.0, args.tiles, args.tiles, args.tiles,
                      order=3, l_lower_order_in_v=True, pusher="vay", depo="esirkepov", norder=2,
-                     cfl=0.95, chunk=args.chunk)
+                     cfl=0.95, chunk=args.chunk, precision=args.precision)
     rng = np.random.default_rng(12345)
+    if args.workload == "bunch":
+        # a Gaussian electron bunch (sigma 20 cells, peak 64 ppc, gamma = 10
+        # along z) in a uniform 1 ppc electron background: tiles hold from a
+        # few hundred to tens of thousands of particles
+        sig, peak, gamma = 20.0, 64.0, 10.0
+        nb = int(round(peak * (2 * np.pi) ** 1.5 * sig ** 3))
+        c = n / 2.0
+        pos = [np.clip(rng.normal(c, sig, nb), 0.0, np.nextafter(n, 0)) for _ in range(3)]
+        uz0 = np.sqrt(gamma ** 2 - 1.0)
+        mom = [rng.standard_normal(nb) * 0.01, rng.standard_normal(nb) * 0.01,
+               uz0 + rng.standard_normal(nb) * 0.01 * uz0]
+        sim.add_species("bunch", -1.0, 1.0, *pos, *mom, np.full(nb, OMEGA_PE ** 2 / peak))
+        del pos, mom
+        npart = n ** 3
+        pos = [rng.random(npart) * n for _ in range(3)]
+        mom = [rng.standard_normal(npart) * 0.01 for _ in range(3)]
+        sim.add_species("background", -1.0, 1.0, *pos, *mom, np.full(npart, OMEGA_PE ** 2))
+        return sim
     npart = n ** 3 * args.ppc
     for name, q, m, uth in (("electrons", -1.0, 1.0, 0.01), ("ions", 1.0, 1836.0, 0.001)):
         pos = [rng.random(npart) * n for _ in range(3)]
@@ -89,6 +120,11 @@ def run_gpu(args):
     sim = build_workload(args)
     N = sim.total_particles()
     log(f"workload ready: {N} particles, {sim.nx}x{sim.ny}x{sim.nz} cells")
+    tuned = None
+    if args.autotune:
+        from picsar_amd.autotune import autotune
+        tuned = autotune(sim)
+        log(f"autotune: {tuned['best']} ({tuned['ms_per_step']:.2f} ms/step gather+push+deposit)")
     ke_init, fe_init = sim.kinetic_energy(), sim.field_energy()
     for _ in range(args.warmup):
         sim.step(1, check=False)
@@ -135,9 +171,13 @@ def run_gpu(args):
         for i in range(nsp):
             launches.append(elapsed(b + 1 + 2 * i, b + 2 + 2 * i))
     per_launch_ms = float(np.mean(launches))
+    ni, no = ctypes.c_int64(0), ctypes.c_int64(0)
+    _lib.call("pxrhip_tiles_stats", ctypes.byref(ni), ctypes.byref(no))
+    log(f"last launch: {ni.value} work items, {no.value} finished from global memory")
     npart_launch = N / nsp
     ncells = sim.nx * sim.ny * sim.nz
-    bytes_launch = npart_launch * BYTES_PER_PARTICLE_FP64 + ncells * BYTES_PER_CELL
+    bpp = BYTES_PER_PARTICLE_MIXED if args.precision == "mixed" else BYTES_PER_PARTICLE_FP64
+    bytes_launch = npart_launch * bpp + ncells * BYTES_PER_CELL
     achieved = bytes_launch / (per_launch_ms * 1e-3) / 1e9
     ke, fe = sim.kinetic_energy(), sim.field_energy()
     # untimed steps up to the energy-drift horizon (counted from the start)
@@ -153,7 +193,7 @@ def run_gpu(args):
     return dict(sim=sim, N=N, wall=wall, gpu_ms=total_ms, stage_ms={k: v / K for k, v in stage.items()},
                 per_launch_ms=per_launch_ms, achieved=achieved, bytes_launch=bytes_launch,
                 ke0=ke0, fe0=fe0, ke=ke, fe=fe, ke_init=ke_init, fe_init=fe_init,
-                drift_steps=done + extra, ke_d=ke_d, fe_d=fe_d)
+                drift_steps=done + extra, ke_d=ke_d, fe_d=fe_d, tuned=tuned)
 
 
 # ---------------------------------------------------------------------------
@@ -236,6 +276,14 @@ def main():
     ap.add_argument("--ppc", type=int, default=16)
     ap.add_argument("--tiles", type=int, default=8, help="tiles per axis")
     ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--workload", choices=("thermal", "bunch"), default="thermal",
+                    help="thermal: periodic thermal plasma (electrons + ions); bunch: Gaussian "
+                         "gamma=10 electron bunch in a 1 ppc background (uneven tiles)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick the launch parameters with picsar_amd.autotune before timing")
+    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
+                    help="fp64: every particle array FP64; mixed: FP64 positions, FP32 momenta, "
+                         "gamma and weights, FP32 gather/push/LDS current, FP64 J and FDTD")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--drift-steps", type=int, default=1000,
                     help="total steps (warmup + timed + untimed) over which the energy drift "
@@ -258,13 +306,14 @@ def main():
     for kv in args.tune:
         k, v = kv.split("=")
         _lib.call("pxrhip_set_tuning", k.encode(), int(v))
+    r = run_gpu(args)
     for k in ("fused.lds_bytes", "fused.wg_per_cu", "fused.spread", "fused.chunk",
-              "fused.j_lds_bytes", "fused.jpad", "fused.margin_pct"):
+              "fused.j_lds_bytes", "fused.jpad", "fused.margin_pct", "mixed.jacc",
+              "mixed.lds_bytes", "mixed.wg_per_cu"):
         import ctypes
         v = ctypes.c_int64(0)
         _lib.call("pxrhip_get_tuning", k.encode(), ctypes.byref(v))
         tuning[k] = v.value
-    r = run_gpu(args)
     K = args.steps
     value = r["N"] * K / (r["gpu_ms"] * 1e-3)
     gpd = r["N"] / (r["stage_ms"]["gather_push_deposit"] * 1e-3)
@@ -279,12 +328,11 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (seeded on device)",
-        "config": {"workload": f"thermal plasma {args.n}^3 periodic, {args.ppc} ppc electrons + "
-                               f"ions, order-3 gather, Vay, order-3 Esirkepov, Yee, "
-                               f"{(args.n + 1) // args.tiles}^3-node tiles, FP64",
-                   "particles": r["N"], "cells": args.n ** 3, "tuning": tuning},
+        "dtype": "f64" if args.precision == "fp64" else "f32 (gather, push, LDS current) + f64 "
+                                                          "(positions, J, FDTD)",
+        "data": "synthetic (numpy seed 12345, resident on the GPU before timing)",
+        "config": {"workload": workload_name(args), "particles": r["N"], "cells": args.n ** 3,
+                   "tuning": tuning, "autotuned": r["tuned"] is not None},
         "stages": {"ms_per_step": r["stage_ms"],
                    "gather_push_deposit_particle_steps_per_s": gpd,
                    "wall_s_timed_region": r["wall"]},

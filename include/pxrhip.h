@@ -53,6 +53,18 @@ int pxrhip_version(void);
 /* number of particles skipped as out of range since the last reset */
 int pxrhip_bad_count(long long *count, int reset);
 
+/* HIP graphs of the library's work (no counterpart in the reference).
+ * pxrhip_init creates the library's own stream; pxrhip_graph_begin starts
+ * capturing it, and every entry point called until pxrhip_graph_end is
+ * recorded instead of run (only calls that do not synchronise may be
+ * recorded: the tiled step, the field solvers and guard exchanges; not
+ * copies to the host or checked leaf calls).  The recorded work keeps the
+ * pointers it was given.  pxrhip_graph_launch replays it on the stream. */
+int pxrhip_graph_begin(void);
+int pxrhip_graph_end(void **graph);
+int pxrhip_graph_launch(void *graph);
+int pxrhip_graph_destroy(void *graph);
+
 /* ---- device memory, timers (no counterpart in the reference) ------------- */
 /* The library runs on its own HIP runtime and stream.  Hosts that hold no
  * other GPU runtime allocate and move data here; copies to the host and
@@ -75,6 +87,9 @@ int pxrhip_timer_elapsed(int start, int stop, double *ms);
  * compute_kin_energy (kin_energy.F90:33-143) for one species */
 int pxrhip_kinetic_energy(int64_t np, const double *gaminv, const double *w,
                           double *out);
+/* the same for a mixed-precision species (float32 gaminv and w, FP64 sum) */
+int pxrhip_kinetic_energy_f32(int64_t np, const float *gaminv, const float *w,
+                              double *out);
 /* sum of f^2 over the nodes 0..n-1 of each axis; times dV/2 this is one
  * component of the field energy of diags.F90:1072-1114 */
 int pxrhip_field_energy(const double *f, int64_t nx, int64_t ny, int64_t nz,
@@ -238,6 +253,11 @@ typedef struct {
                          pxrhip_tiles_rebin_sort), or NULL */
   int64_t capacity;   /* allocated length of each particle array */
   double q, m;
+  /* 0: every particle array is float64.  1: mixed precision -- x, y, z are
+   * float64, while ux, uy, uz, gaminv and w point to float32 arrays (the
+   * pointers are cast); gather, push and the LDS current are FP32 and the
+   * global current is FP64. */
+  int64_t precision;
 } pxrhip_species;
 
 /* Number of sorting-grid cells over all tiles (the length of bin_start
@@ -280,10 +300,18 @@ int pxrhip_tiles_rebin_sort(const pxrhip_grid *grid, pxrhip_species *src,
  * "fused.j_lds_bytes" (largest current block accumulated in LDS; negative:
  * all of fused.lds_bytes), "fused.jpad" (0/1, odd row and plane pitches of
  * the LDS current block), "fused.margin_pct" (motion margin, in 1/100 cell,
- * of the current block estimated when work items are cut).  Work items whose field or current block exceeds
+ * of the current block estimated when work items are cut), "mixed.jacc"
+ * (LDS current of a mixed-precision species: 0 FP32 atomics, 1 FP32
+ * atomics compensated by TwoSum into a second array, 2 FP64 atomics),
+ * "mixed.lds_bytes" and "mixed.wg_per_cu" (the launch of a mixed-precision
+ * species; fused.lds_bytes and fused.wg_per_cu are those of an FP64 one).  Work items whose field or current block exceeds
  * its LDS budget are completed from global memory by a second kernel. */
 int pxrhip_set_tuning(const char *key, int64_t value);
 int pxrhip_get_tuning(const char *key, int64_t *value);
+
+/* Work items of the last pxrhip_tiles_gather_push_deposit and how many of
+ * them did not fit in LDS and ran from global memory (synchronises). */
+int pxrhip_tiles_stats(int64_t *nitems, int64_t *noverflow);
 
 /* Reads the device-side error counter of the tiled path (particles found
  * outside the arrays and skipped) and resets it; a non-zero count is

@@ -43,6 +43,11 @@ SIGNATURES = {
     "pxrhip_timer_record": [_i32],
     "pxrhip_timer_elapsed": [_i32, _i32, C.POINTER(C.c_double)],
     "pxrhip_kinetic_energy": [_I, _P, _P, C.POINTER(C.c_double)],
+    "pxrhip_graph_begin": [],
+    "pxrhip_graph_end": [C.POINTER(C.c_void_p)],
+    "pxrhip_graph_launch": [_P],
+    "pxrhip_graph_destroy": [_P],
+    "pxrhip_kinetic_energy_f32": [_I, _P, _P, C.POINTER(C.c_double)],
     "pxrhip_field_energy": [_P] + [_I] * 6 + [C.POINTER(C.c_double)],
     "pxrhip_geteb3d_energy_conserving": [_I] + [_P] * 9 + [_F] * 6 + [_I] * 9 + [_P] * 6 + [_I] * 5,
     "pxrhip_boris_push_u_3d": [_I] + [_P] * 10 + [_F] * 3,
@@ -60,6 +65,7 @@ SIGNATURES = {
     "pxrhip_particle_bin_sorting": [_I] + [_P] * 8 + [_I] + [_F] * 9,
     "pxrhip_tiles_nbins": [_P, _P],
     "pxrhip_tiles_check": [],
+    "pxrhip_tiles_stats": [C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
     "pxrhip_set_tuning": [C.c_char_p, _I],
     "pxrhip_get_tuning": [C.c_char_p, C.POINTER(C.c_int64)],
     "pxrhip_tiles_gather_push_deposit": [_P, _P] + [_P] * 6 + [_P] * 3 + [_F] + [_I] * 5,

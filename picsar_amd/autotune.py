@@ -1,0 +1,106 @@
+"""Launch-parameter autotuner of the tiled step.
+
+The reference's tile split (ntilex, ntiley, ntilez) is the user's and is
+kept; on the GPU the unit of work is a work item: a run of cell-sorted
+particles of one tile whose field and current blocks sit in LDS.  Its shape
+and the launch of the persistent kernel are set by the keys of
+``pxrhip_set_tuning``:
+
+* ``fused.chunk``     -- particles per work item (rows of the sorting grid
+  are added while the LDS blocks fit);
+* ``fused.wg_per_cu`` with ``fused.lds_bytes`` -- workgroups per CU and the
+  LDS each may use (2 x 79616, 1 x 155648, 3 x 52224 bytes);
+* ``fused.spread``    -- lanes on particles S apart in the deposition.
+
+``autotune`` times the gather + push + deposit stage of a Simulation for
+each candidate with the library's stream timers and keeps the fastest.  The
+trial steps are real steps of the simulation: launch parameters change the
+order of floating-point atomic additions only, not the algorithm.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import itertools
+from typing import Dict, Iterable, List, Optional
+
+from . import _lib
+
+KEYS = ("fused.chunk", "fused.wg_per_cu", "fused.lds_bytes", "fused.spread",
+        "mixed.wg_per_cu", "mixed.lds_bytes")
+TIMER_BASE = 65000  # timer slots used here (the library has 65536)
+
+DEFAULT_CANDIDATES: List[Dict[str, int]] = [
+    dict(zip(("fused.chunk", "fused.wg_per_cu", "fused.lds_bytes", "fused.spread"),
+             (ch, wg, lds, sp)))
+    for ch, (wg, lds), sp in itertools.product((1024, 2048, 4096, 8192),
+                                               ((2, 79616), (1, 155648), (3, 52224)), (1,))
+] + [{"fused.chunk": 4096, "fused.wg_per_cu": 2, "fused.lds_bytes": 79616, "fused.spread": 0}]
+
+
+def get_tuning(keys: Iterable[str] = KEYS) -> Dict[str, int]:
+    out = {}
+    for k in keys:
+        v = C.c_int64(0)
+        _lib.call("pxrhip_get_tuning", k.encode(), C.byref(v))
+        out[k] = v.value
+    return out
+
+
+def set_tuning(values: Dict[str, int]) -> None:
+    for k, v in values.items():
+        _lib.call("pxrhip_set_tuning", k.encode(), int(v))
+
+
+def _time_stage(sim, steps: int) -> float:
+    """Milliseconds per step of gather + push + deposit (all species), over
+    ``steps`` whole steps."""
+    total = 0.0
+    for s in range(steps):
+        a, b = TIMER_BASE + 2 * s, TIMER_BASE + 2 * s + 1
+        _lib.call("pxrhip_timer_record", a)
+        sim.gather_push_deposit()
+        _lib.call("pxrhip_timer_record", b)
+        sim.current_bcs()
+        sim.field_step()
+        sim.particle_bcs_and_sorting()
+        sim.it += 1
+    for s in range(steps):
+        ms = C.c_double(0.0)
+        _lib.call("pxrhip_timer_elapsed", TIMER_BASE + 2 * s, TIMER_BASE + 2 * s + 1, C.byref(ms))
+        total += ms.value
+    return total / steps
+
+
+def _for_precision(c: Dict[str, int], precision: str) -> Dict[str, int]:
+    """Mixed-precision species are launched with the mixed.* keys."""
+    if precision != "mixed":
+        return c
+    ren = {"fused.wg_per_cu": "mixed.wg_per_cu", "fused.lds_bytes": "mixed.lds_bytes"}
+    return {ren.get(k, k): v for k, v in c.items()}
+
+
+def autotune(sim, candidates: Optional[List[Dict[str, int]]] = None, steps: int = 2,
+             verbose: bool = False) -> Dict[str, object]:
+    """Try every candidate on ``sim`` for ``steps`` steps (after one step to
+    warm the candidate up), apply the fastest and return it with the
+    timings.  Raises if any trial step skipped a particle."""
+    cands = DEFAULT_CANDIDATES if candidates is None else candidates
+    cands = [_for_precision(c, getattr(sim, "precision", "fp64")) for c in cands]
+    before = get_tuning()
+    results = []
+    try:
+        for c in cands:
+            set_tuning(c)
+            _time_stage(sim, 1)
+            ms = _time_stage(sim, steps)
+            results.append((ms, c))
+            if verbose:
+                print(f"autotune: {c} -> {ms:.3f} ms/step")
+        sim.check()
+    except Exception:
+        set_tuning(before)
+        raise
+    best_ms, best = min(results, key=lambda r: r[0])
+    set_tuning(best)
+    return {"best": best, "ms_per_step": best_ms,
+            "trials": [{"tuning": c, "ms_per_step": ms} for ms, c in results]}

@@ -59,6 +59,19 @@ __global__ __launch_bounds__(256) void k_kin_partial(i64 np, const double *gi, c
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
+__global__ __launch_bounds__(256) void k_kin_partial_f32(i64 np, const float *gi, const float *w,
+                                                         double *part) {
+  __shared__ double s[256];
+  const i64 b0 = (i64)blockIdx.x * RED_B;
+  double acc = 0.0;
+  for (int k = threadIdx.x; k < RED_B; k += 256) {
+    const i64 i = b0 + k;
+    if (i < np) acc += (1.0 / (double)gi[i] - 1.0) * (double)w[i];
+  }
+  const double t = block_sum(acc, s);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
 // sum of f^2 over the nodes 0..n-1 of each axis (diags.F90:1072-1114)
 __global__ __launch_bounds__(256) void k_sq_partial(const double *f, GridGeom g, double *part) {
   __shared__ double s[256];
@@ -202,6 +215,20 @@ int pxrhip_kinetic_energy(int64_t np, const double *gaminv, const double *w, dou
   if (!part) return set_error(PXRHIP_ERR_HIP, "scratch allocation failed");
   hipLaunchKernelGGL(k_kin_partial, dim3((unsigned)nblk), dim3(256), 0, rt().stream, np, gaminv, w,
                      part);
+  PXR_LAUNCHED();
+  return reduce_finish(part, nblk, out);
+}
+
+int pxrhip_kinetic_energy_f32(int64_t np, const float *gaminv, const float *w, double *out) {
+  PXR_INIT();
+  if (np < 0 || !out || (np && (!gaminv || !w))) return set_error(PXRHIP_ERR_ARG, "bad arguments");
+  *out = 0.0;
+  if (!np) return PXRHIP_OK;
+  const i64 nblk = (np + RED_B - 1) / RED_B;
+  double *part = (double *)scratch(11, sizeof(double) * (size_t)(nblk + 1));
+  if (!part) return set_error(PXRHIP_ERR_HIP, "scratch allocation failed");
+  hipLaunchKernelGGL(k_kin_partial_f32, dim3((unsigned)nblk), dim3(256), 0, rt().stream, np, gaminv,
+                     w, part);
   PXR_LAUNCHED();
   return reduce_finish(part, nblk, out);
 }

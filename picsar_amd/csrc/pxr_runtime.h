@@ -12,6 +12,8 @@ namespace pxr {
 struct Runtime {
   int device = -1;
   hipStream_t stream = nullptr;
+  bool own_stream = false;  // created by pxrhip_init
+  bool capturing = false;   // a graph capture of the stream is open
   bool checks = true;
   unsigned long long *d_bad = nullptr;  // device counter of skipped particles
   int num_cus = 256;

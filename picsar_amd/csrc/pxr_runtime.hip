@@ -78,6 +78,12 @@ int pxrhip_init(int device) {
     return set_error(PXRHIP_ERR_UNSUPPORTED, "device %d is %s; this library is built for gfx950",
                      dev, prop.gcnArchName);
   g_rt.num_cus = prop.multiProcessorCount;
+  if (!g_rt.stream) {
+    // the library's own stream (default priority, ordered with the null
+    // stream); it can be captured into a HIP graph, the null stream cannot
+    PXR_HIP(hipStreamCreate(&g_rt.stream));
+    g_rt.own_stream = true;
+  }
   if (!g_rt.d_bad) {
     PXR_HIP(hipMalloc(&g_rt.d_bad, 2 * sizeof(unsigned long long)));
     PXR_HIP(hipMemset(g_rt.d_bad, 0, 2 * sizeof(unsigned long long)));
@@ -96,6 +102,9 @@ int pxrhip_finalize(void) {
   }
   if (g_rt.d_bad) hipFree(g_rt.d_bad);
   g_rt.d_bad = nullptr;
+  if (g_rt.own_stream && g_rt.stream) hipStreamDestroy(g_rt.stream);
+  g_rt.stream = nullptr;
+  g_rt.own_stream = false;
   g_rt.device = -1;
   return PXRHIP_OK;
 }
@@ -103,7 +112,59 @@ int pxrhip_finalize(void) {
 const char *pxrhip_last_error(void) { return g_err; }
 
 int pxrhip_set_stream(void *s) {
+  if (g_rt.capturing) return set_error(PXRHIP_ERR_ARG, "the stream is being captured");
   g_rt.stream = (hipStream_t)s;
+  g_rt.own_stream = false;
+  return PXRHIP_OK;
+}
+
+// ---- HIP graphs of the library's work ---------------------------------------
+struct GraphHandle {
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+};
+
+int pxrhip_graph_begin(void) {
+  PXR_INIT();
+  if (!g_rt.stream) return set_error(PXRHIP_ERR_ARG, "the null stream cannot be captured");
+  if (g_rt.capturing) return set_error(PXRHIP_ERR_ARG, "a capture is already open");
+  PXR_HIP(hipStreamBeginCapture(g_rt.stream, hipStreamCaptureModeRelaxed));
+  g_rt.capturing = true;
+  return PXRHIP_OK;
+}
+
+int pxrhip_graph_end(void **handle) {
+  PXR_INIT();
+  if (!handle) return set_error(PXRHIP_ERR_ARG, "null handle");
+  *handle = nullptr;
+  if (!g_rt.capturing) return set_error(PXRHIP_ERR_ARG, "no capture is open");
+  g_rt.capturing = false;
+  hipGraph_t g = nullptr;
+  PXR_HIP(hipStreamEndCapture(g_rt.stream, &g));
+  hipGraphExec_t e = nullptr;
+  hipError_t err = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+  if (err != hipSuccess) {
+    hipGraphDestroy(g);
+    return set_error(PXRHIP_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(err));
+  }
+  *handle = new GraphHandle{g, e};
+  return PXRHIP_OK;
+}
+
+int pxrhip_graph_launch(void *handle) {
+  PXR_INIT();
+  if (!handle) return set_error(PXRHIP_ERR_ARG, "null graph");
+  PXR_HIP(hipGraphLaunch(((GraphHandle *)handle)->exec, g_rt.stream));
+  return PXRHIP_OK;
+}
+
+int pxrhip_graph_destroy(void *handle) {
+  if (!handle) return PXRHIP_OK;
+  GraphHandle *h = (GraphHandle *)handle;
+  if (g_rt.stream) (void)hipStreamSynchronize(g_rt.stream);
+  (void)hipGraphExecDestroy(h->exec);
+  (void)hipGraphDestroy(h->graph);
+  delete h;
   return PXRHIP_OK;
 }
 

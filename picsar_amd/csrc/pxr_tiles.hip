@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "pxr_device.h"
+#include "pxr_fused.h"
 #include "pxr_kernels.h"
 #include "pxr_runtime.h"
 
@@ -341,8 +342,11 @@ __global__ __launch_bounds__(256) void k_fill_i32_t(int *p, i64 n) {
 }
 
 struct ScatterArgs {
-  const double *src[8];
-  double *dst[8];
+  const double *src[3];  // positions
+  double *dst[3];
+  const void *srcq[5];   // ux, uy, uz, gaminv, w: FP64, or FP32 when mixed
+  void *dstq[5];
+  int mixed;
   const int64_t *src_off;
   int ntiles;
   const int *key;
@@ -362,7 +366,14 @@ __global__ __launch_bounds__(256) void k_rb_scatter(ScatterArgs a, i64 cap) {
   if (!act) return;
   const i64 d = (i64)slot + rank;
 #pragma unroll
-  for (int c = 0; c < 8; c++) a.dst[c][d] = a.src[c][i];
+  for (int c = 0; c < 3; c++) a.dst[c][d] = a.src[c][i];
+  if (a.mixed) {
+#pragma unroll
+    for (int c = 0; c < 5; c++) ((float *)a.dstq[c])[d] = ((const float *)a.srcq[c])[i];
+  } else {
+#pragma unroll
+    for (int c = 0; c < 5; c++) ((double *)a.dstq[c])[d] = ((const double *)a.srcq[c])[i];
+  }
 }
 
 __global__ __launch_bounds__(256) void k_rb_tiles(int ntiles, const int64_t *bin_base,
@@ -378,90 +389,13 @@ __global__ __launch_bounds__(256) void k_rb_tiles(int ntiles, const int64_t *bin
 // ---------------------------------------------------------------------------
 // chunk table of the fused kernel
 // ---------------------------------------------------------------------------
-// Node blocks of a work item.  The particles of a work item sit in known
-// cells of the sorting grid (tile t, plane z, rows y0..y1), so the index
-// ranges of their gather stencils and of their currents follow from the
-// positions bounding those cells; the index functions of the reference's
-// shapes are non-decreasing in the position.
-struct BoxGeom {
-  double x0[3], di[3];   // grid origin and inverse cell size
-  double sd[3];          // sorting cell size
-  double mv[3];          // motion margin of the current block, in cells
-  double s;              // staggering shift (1/2)
-  int N, M, depo;        // shape orders (nodal, staggered) and deposition (0 Esirkepov, 3 direct)
-  int glo[3], ghi[3];    // node range of the grid arrays
-  const double *sorg;    // sorting-grid origin of every tile
-  const double *tbox;    // membership [tmin, tmax) of every tile
-  const int *n3;         // sorting cells of every tile
-};
-
-PXR_DEV void gather_range(const BoxGeom &b, double ulo, double uhi, int &lo, int &hi) {
-  const int N = b.N, M = b.M;
-  auto jn = [&](double v) { return (N == 2 ? f_nint(v) : f_floor(v)) - (N == 1 ? 0 : 1); };
-  auto js = [&](double v) {
-    return (M == 0 || M == 2) ? f_floor((v + 0.5) - b.s) - (M == 2 ? 1 : 0)
-                              : f_floor(v - b.s) - (M == 3 ? 1 : 0);
-  };
-  lo = min(jn(ulo), js(ulo));
-  hi = max(jn(uhi) + N, js(uhi) + M);
-}
-
-PXR_DEV void deposit_range(const BoxGeom &b, double ulo, double uhi, double m, int &lo, int &hi) {
-  const int N = b.N;
-  if (b.depo == 0) {  // Esirkepov: old and new node index, support ESup<N>
-    auto ei = [&](double v) { return N == 2 ? f_nint(v) : f_floor(v); };
-    lo = ei(ulo - m) + (N == 1 ? 0 : -1);
-    hi = ei(uhi + m) + (N == 3 ? 2 : 1);
-  } else {  // direct: nodal and staggered shapes at the mid position
-    auto dj = [&](double v) { return N == 2 ? f_nint(v) - 1 : f_floor(v) - (N == 1 ? 0 : 1); };
-    lo = min(dj(ulo - m), dj((ulo - m) - b.s));
-    hi = max(dj(uhi + m), dj((uhi + m) - b.s)) + N;
-  }
-}
-
-// normalized coordinate range of the particles of tile t in sorting rows
-// y0..y1 of plane z (intersected with the tile's membership)
-PXR_DEV void rows_range(const BoxGeom &b, int t, int z, int y0, int y1, double *ulo, double *uhi) {
-  const double *so = b.sorg + 3 * t, *tb = b.tbox + 6 * t;
-  const int c0[3] = {0, y0, z};
-  const int c1[3] = {b.n3[3 * t] - 1, y1, z};
-  for (int d = 0; d < 3; d++) {
-    const double plo = fmax(so[d] + b.sd[d] * c0[d], tb[d]);
-    const double phi = fmin(so[d] + b.sd[d] * (c1[d] + 1), tb[3 + d]);
-    ulo[d] = (plo - b.x0[d]) * b.di[d] - 1e-7;
-    uhi[d] = (phi - b.x0[d]) * b.di[d] + 1e-7;
-  }
-}
-
-// field block (FR0, fe) and current block (JR0, je) for that range, clamped
-// to the arrays; returns the doubles they take in LDS (6 + 3 arrays)
-PXR_DEV i64 blocks_for(const BoxGeom &b, const double *ulo, const double *uhi, int *FR0, int *fe,
-                       int *JR0, int *je) {
-  for (int d = 0; d < 3; d++) {
-    int lo, hi;
-    gather_range(b, ulo[d], uhi[d], lo, hi);
-    FR0[d] = max(lo, b.glo[d]);
-    fe[d] = max(min(hi, b.ghi[d]) - FR0[d] + 1, 0);
-    deposit_range(b, ulo[d], uhi[d], b.mv[d], lo, hi);
-    JR0[d] = max(lo, b.glo[d]);
-    je[d] = max(min(hi, b.ghi[d]) - JR0[d] + 1, 0);
-  }
-  return 6 * (i64)fe[0] * fe[1] * fe[2] + 3 * (i64)je[0] * je[1] * je[2];
-}
-
-struct Chunk {
-  long long begin;
-  int n;
-  int tile;
-  int z, y0, y1;  // sorting-grid plane and rows of the particles; z < 0: unknown
-  int pad;
-};
-
 struct ChunkArgs {
   int ntiles, ch, nzmax;  // nzmax: largest number of sorting planes of a tile
   BoxGeom bg;
-  i64 lds_doubles;        // LDS budget of a work item's field + current blocks
-  i64 j_doubles;          // budget of its current block
+  i64 lds_bytes;          // LDS budget of a work item's field block
+  i64 j_bytes;            // budget of its current block
+  int fnode_bytes;        // LDS bytes per node of the six field arrays
+  int jnode_bytes;        // LDS bytes per node of the current arrays
   const int64_t *tile_off, *tile_np, *bin_start;
   const int *n3;
   const int64_t *bin_base;
@@ -517,9 +451,13 @@ PXR_DEV int walk_unit(const ChunkArgs &a, int t, int z, i64 pos) {
       int FR0[3], fe[3], JR0[3], je[3];
       rows_range(a.bg, t, z, y0, y, ulo, uhi);
       blocks_for(a.bg, ulo, uhi, FR0, fe, JR0, je);
-      // current block, with room for the odd pitches of k_fused
-      const i64 jpz = ((i64)(je[0] | 1) * je[1]) | 1;
-      fits = 6 * (i64)fe[0] * fe[1] * fe[2] <= a.lds_doubles && 3 * jpz * je[2] <= a.j_doubles;
+      // current block, with room for one node of motion on each side
+      // (the block of k_fused is the box of the pushed particles) and for
+      // its odd pitches
+      const int j0 = je[0] + 2, j1 = je[1] + 2, j2 = je[2] + 2;
+      const i64 jpz = ((i64)(j0 | 1) * j1) | 1;
+      fits = a.fnode_bytes * (i64)fe[0] * fe[1] * fe[2] <= a.lds_bytes &&
+             a.jnode_bytes * jpz * j2 <= a.j_bytes;
     }
     if (cnt > 0 && (cnt + rn > a.ch || !fits)) {
       emit(cb, cnt, z, y0, ylast);
@@ -559,367 +497,11 @@ __global__ __launch_bounds__(256) void k_chunk_write(ChunkArgs a) {
   }
 }
 
-// ---------------------------------------------------------------------------
-// fused gather + push + deposit
-// ---------------------------------------------------------------------------
-struct FusedArgs {
-  const Chunk *chunks;
-  const int *nchunks;
-  int *queue;
-  double *x, *y, *z, *ux, *uy, *uz, *gi;
-  const double *w;
-  const double *F[6];
-  double *J[3];
-  GridGeom g;
-  double xmin, ymin, zmin, dxi, dyi, dzi, s;
-  double cst;        // Boris: q*dt/(2m); Vay: q*dt/m
-  double dt;
-  EsirkConst<double> ec;
-  double q, dx, dy, dz;  // direct deposition
-  unsigned long long *bad;
-  BoxGeom bg;            // work-item block geometry
-  int lds_doubles;       // dynamic LDS size in doubles (field and current blocks)
-  int j_doubles;         // largest current block kept in LDS, in doubles
-  int spread;            // lanes take particles S apart
-  int jpad;              // odd pitches for the LDS current block
-  int *ovf_count;        // work items whose blocks do not fit in LDS ...
-  int2 *ovf;             // ... (work item, stage): 0 untouched, 1 pushed
-  int ovf_cap;
-};
-
-struct Box6 {
-  int lo[3], hi[3];
-};
-
-PXR_DEV void box_reset(Box6 &b) {
-  for (int d = 0; d < 3; d++) {
-    b.lo[d] = INT_MAX;
-    b.hi[d] = INT_MIN;
-  }
-}
-
-PXR_DEV void wave_minmax(Box6 &b) {
-#pragma unroll
-  for (int d = 0; d < 3; d++) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      b.lo[d] = min(b.lo[d], __shfl_xor(b.lo[d], o));
-      b.hi[d] = max(b.hi[d], __shfl_xor(b.hi[d], o));
-    }
-  }
-}
-
-struct LdsAcc {
-  double *p[3];
-  int i0, j0, k0, sy, sz;
-  PXR_DEV void add(int c, int i, int j, int k, double v) const {
-    atomicAdd(p[c] + ((i - i0) + sy * (j - j0) + sz * (k - k0)), v);
-  }
-};
-
-struct GAcc {
-  double *p[3];
-  GridGeom g;
-  PXR_DEV void add(int c, int i, int j, int k, double v) const {
-    unsafeAtomicAdd(p[c] + g.idx(i, j, k), v);
-  }
-};
-
-PXR_DEV bool in_block(const int *lo, const int *hi, const int *R0, const int *e) {
-  return lo[0] >= R0[0] && lo[1] >= R0[1] && lo[2] >= R0[2] && hi[0] < R0[0] + e[0] &&
-         hi[1] < R0[1] + e[1] && hi[2] < R0[2] + e[2];
-}
-
-// Gather, push and position push of particle p.  Fields are read from the
-// LDS block when the stencil lies inside it (views fl), else from global
-// memory (fg).  Extends the box of the current the particle will deposit.
-template <int N, bool LO, int PUSHER, int DEPO, typename VL, typename VG>
-PXR_DEV void gpd_push(const FusedArgs &a, i64 p, const VL *fl, const VG *fg, const int *FR0,
-                      const int *fe, Box6 &dep) {
-  constexpr int M = LO ? N - 1 : N;
-  const GridGeom &g = a.g;
-  const double x = a.x[p], y = a.y[p], z = a.z[p];
-  GatherStencil<N, M, double> st;
-  gather_stencil<N, M, double>(st, (x - a.xmin) * a.dxi, (y - a.ymin) * a.dyi,
-                               (z - a.zmin) * a.dzi, a.s);
-  int l2[3], h2[3];
-  gather_bounds<N, M, double>(st, l2, h2);
-  if (!g.inside(l2[0], l2[1], l2[2]) || !g.inside(h2[0], h2[1], h2[2])) {
-    count_bad(a.bad);  // outside the arrays: skipped, as the reference cannot hold it
-    return;
-  }
-  double E[6];
-  if (in_block(l2, h2, FR0, fe)) gather_apply<N, M, double>(st, fl, E);
-  else gather_apply<N, M, double>(st, fg, E);
-  double u[3] = {a.ux[p], a.uy[p], a.uz[p]};
-  double gi = a.gi[p];
-  if (PUSHER == 1) vay_push<double>(u, gi, E[0], E[1], E[2], E[3], E[4], E[5], a.cst);
-  else boris_push<double>(u, gi, E[0], E[1], E[2], E[3], E[4], E[5], a.cst);
-  // pxr_pushxyz (boris_3d.F90:880-882)
-  const double xn = x + u[0] * gi * a.dt;
-  const double yn = y + u[1] * gi * a.dt;
-  const double zn = z + u[2] * gi * a.dt;
-  a.ux[p] = u[0];
-  a.uy[p] = u[1];
-  a.uz[p] = u[2];
-  a.gi[p] = gi;
-  a.x[p] = xn;
-  a.y[p] = yn;
-  a.z[p] = zn;
-  int b0[3], b1[3];
-  if (DEPO == 0) {
-    EsirkepovIn<double> in;
-    esirkepov_prepare<N, double>(in, xn, yn, zn, u[0], u[1], u[2], gi, a.w[p], a.ec);
-    if (!esirkepov_box<N, double>(in, b0, b1)) return;
-  } else {
-    DirectIn<double> in;
-    direct_prepare<N, double>(in, xn, yn, zn, u[0], u[1], u[2], gi, a.w[p], a.q, a.xmin, a.ymin,
-                              a.zmin, a.dt, a.dx, a.dy, a.dz, a.s);
-    direct_box<N>(in, b0, b1);
-  }
-  for (int d = 0; d < 3; d++) {
-    dep.lo[d] = min(dep.lo[d], b0[d]);
-    dep.hi[d] = max(dep.hi[d], b1[d]);
-  }
-}
-
-// Deposit pushed particle p (its state is read back from memory).  Returns
-// false when the particle left the arrays (counted, skipped).
-template <int N, int DEPO, typename Acc>
-PXR_DEV bool gpd_deposit(const FusedArgs &a, i64 p, const Acc &acc) {
-  const GridGeom &g = a.g;
-  int b0[3], b1[3];
-  if (DEPO == 0) {
-    EsirkepovIn<double> in;
-    esirkepov_prepare<N, double>(in, a.x[p], a.y[p], a.z[p], a.ux[p], a.uy[p], a.uz[p], a.gi[p],
-                                 a.w[p], a.ec);
-    if (!esirkepov_box<N, double>(in, b0, b1) || !g.inside(b0[0], b0[1], b0[2]) ||
-        !g.inside(b1[0], b1[1], b1[2]))
-      return false;
-    esirkepov_deposit<N, double>(in, acc);
-  } else {
-    DirectIn<double> in;
-    direct_prepare<N, double>(in, a.x[p], a.y[p], a.z[p], a.ux[p], a.uy[p], a.uz[p], a.gi[p],
-                              a.w[p], a.q, a.xmin, a.ymin, a.zmin, a.dt, a.dx, a.dy, a.dz, a.s);
-    direct_box<N>(in, b0, b1);
-    if (!g.inside(b0[0], b0[1], b0[2]) || !g.inside(b1[0], b1[1], b1[2])) return false;
-    direct_deposit<N, double>(in, acc);
-  }
-  return true;
-}
-
-// Persistent kernel over the work items.  Per work item:
-//   1. the node box of its field block, from the sorting-grid cells that
-//      hold its particles (or, without sorting information, from one pass
-//      over the positions); the six field blocks are staged into LDS;
-//   2. gather (from LDS; a particle whose stencil leaves the block, e.g.
-//      after a push without re-sorting, reads global memory), push and
-//      position push, lanes on consecutive particles; the box of the
-//      currents is reduced across the workgroup;
-//   3. the current block is zeroed in LDS and the pushed particles are
-//      deposited with LDS atomics; lanes of a wave take particles S apart
-//      (one or more cells apart in a cell-sorted run) so that their atomics
-//      rarely meet;
-//   4. the current block is added to global J.
-// A work item whose field block does not fit in LDS is left whole to
-// k_fused_overflow; one whose current block does not fit is left to it
-// after the push.
-template <int BS, int N, bool LO, int PUSHER, int DEPO>
-__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void k_fused(FusedArgs a) {
-  constexpr int M = LO ? N - 1 : N;
-  extern __shared__ double smem[];
-  __shared__ int s_chunk;
-  __shared__ int s_box[6];
-  const GridGeom &g = a.g;
-  const int glo[3] = {-g.gx, -g.gy, -g.gz};
-  const int ghi[3] = {g.nx + g.gx, g.ny + g.gy, g.nz + g.gz};
-  const double x0[3] = {a.xmin, a.ymin, a.zmin}, di[3] = {a.dxi, a.dyi, a.dzi};
-  const int nch = *a.nchunks;
-  for (;;) {
-    if (threadIdx.x == 0) {
-      s_chunk = atomicAdd(a.queue, 1);
-      for (int d = 0; d < 3; d++) {
-        s_box[d] = INT_MAX;
-        s_box[3 + d] = INT_MIN;
-      }
-    }
-    __syncthreads();
-    const int c = s_chunk;
-    if (c >= nch) break;
-    const Chunk ch = a.chunks[c];
-    const i64 p0 = ch.begin;
-    // ---- 1. field block -------------------------------------------------
-    double ulo[3], uhi[3];
-    if (ch.z >= 0) {
-      rows_range(a.bg, ch.tile, ch.z, ch.y0, ch.y1, ulo, uhi);
-    } else {
-      // no sorting information: one pass over the positions, in whole cells
-      Box6 b;
-      box_reset(b);
-      for (int k = threadIdx.x; k < ch.n; k += BS) {
-        const i64 p = p0 + k;
-        const double pp[3] = {a.x[p], a.y[p], a.z[p]};
-        for (int d = 0; d < 3; d++) {
-          const int cc = f_floor((pp[d] - x0[d]) * di[d]);
-          b.lo[d] = min(b.lo[d], cc);
-          b.hi[d] = max(b.hi[d], cc);
-        }
-      }
-      wave_minmax(b);
-      if ((threadIdx.x & 63) == 0)
-        for (int d = 0; d < 3; d++) {
-          atomicMin(&s_box[d], b.lo[d]);
-          atomicMax(&s_box[3 + d], b.hi[d]);
-        }
-      __syncthreads();
-      for (int d = 0; d < 3; d++) {
-        const int lo = max(s_box[d], glo[d] - 4), hi = min(s_box[3 + d], ghi[d] + 4);
-        ulo[d] = lo - 1e-7;
-        uhi[d] = hi + 1.0 + 1e-7;
-      }
-      __syncthreads();
-      if (threadIdx.x == 0)
-        for (int d = 0; d < 3; d++) {
-          s_box[d] = INT_MAX;
-          s_box[3 + d] = INT_MIN;
-        }
-    }
-    int FR0[3], fe[3];
-    for (int d = 0; d < 3; d++) {
-      int lo, hi;
-      gather_range(a.bg, ulo[d], uhi[d], lo, hi);
-      FR0[d] = max(lo, glo[d]);
-      fe[d] = max(min(hi, ghi[d]) - FR0[d] + 1, 0);
-    }
-    const i64 nb64 = (i64)fe[0] * fe[1] * fe[2];
-    if (nb64 <= 0 || 6 * nb64 > a.lds_doubles) {
-      if (threadIdx.x == 0) {
-        const int slot = atomicAdd(a.ovf_count, 1);
-        if (slot < a.ovf_cap) a.ovf[slot] = make_int2(c, 0);
-      }
-      __syncthreads();
-      continue;
-    }
-    const int nb = (int)nb64;
-    for (int f = 0; f < 6; f++) {
-      const double *src = a.F[f];
-      for (int idx = threadIdx.x; idx < nb; idx += BS) {
-        const int i = FR0[0] + idx % fe[0];
-        const int j = FR0[1] + (idx / fe[0]) % fe[1];
-        const int k = FR0[2] + idx / (fe[0] * fe[1]);
-        smem[f * nb + idx] = src[g.idx(i, j, k)];
-      }
-    }
-    __syncthreads();
-    // ---- 2. gather, push, position push ---------------------------------
-    Box6 dep;
-    box_reset(dep);
-    {
-      LdsView<double> fl[6];
-      FieldView<double> fg[6];
-      for (int f = 0; f < 6; f++) {
-        fl[f] = LdsView<double>{smem + f * nb, FR0[0], FR0[1], FR0[2], fe[0], fe[0] * fe[1]};
-        fg[f] = FieldView<double>{a.F[f] + g.idx(0, 0, 0), g.NX, g.NX * g.NY};
-      }
-      for (int k = threadIdx.x; k < ch.n; k += BS)
-        gpd_push<N, LO, PUSHER, DEPO>(a, p0 + k, fl, fg, FR0, fe, dep);
-    }
-    wave_minmax(dep);
-    __syncthreads();  // field block no longer read
-    if ((threadIdx.x & 63) == 0)
-      for (int d = 0; d < 3; d++) {
-        atomicMin(&s_box[d], dep.lo[d]);
-        atomicMax(&s_box[3 + d], dep.hi[d]);
-      }
-    __syncthreads();
-    int R0[3], e[3];
-    for (int d = 0; d < 3; d++) {
-      R0[d] = max(s_box[d], glo[d]);
-      const int R1 = min(s_box[3 + d], ghi[d]);
-      e[d] = R1 >= R0[d] ? R1 - R0[d] + 1 : 0;
-    }
-    // LDS layout of the current block: odd row and plane pitches, so that
-    // lanes on cells an even number apart fall on different banks
-    const int py = a.jpad ? (e[0] | 1) : e[0];
-    const int pzr = py * e[1];
-    const int pz = a.jpad ? (pzr | 1) : pzr;
-    const i64 nj64 = (i64)pz * e[2];
-    if ((i64)e[0] * e[1] * e[2] <= 0 || 3 * nj64 > a.j_doubles) {
-      // pushed; the deposition is left to the overflow kernel
-      if (threadIdx.x == 0 && (i64)e[0] * e[1] * e[2] > 0) {
-        const int slot = atomicAdd(a.ovf_count, 1);
-        if (slot < a.ovf_cap) a.ovf[slot] = make_int2(c, 1);
-      }
-      __syncthreads();
-      continue;
-    }
-    const int nj = (int)nj64;
-    for (int idx = threadIdx.x; idx < 3 * nj; idx += BS) smem[idx] = 0.0;
-    __syncthreads();
-    // ---- 3. deposit -------------------------------------------------------
-    {
-      const LdsAcc la{{smem, smem + nj, smem + 2 * nj}, R0[0], R0[1], R0[2], py, pz};
-      const int S = (ch.n + 63) >> 6;
-      for (int kk = threadIdx.x; kk < S * 64; kk += BS) {
-        const int k = a.spread ? (kk & 63) * S + (kk >> 6) : kk;
-        if (k < ch.n && !gpd_deposit<N, DEPO>(a, p0 + k, la)) count_bad(a.bad);
-      }
-    }
-    __syncthreads();
-    // ---- 4. flush ----------------------------------------------------------
-    const int ne = e[0] * e[1] * e[2];
-    for (int idx = threadIdx.x; idx < ne; idx += BS) {
-      const int i = idx % e[0];
-      const int j = (idx / e[0]) % e[1];
-      const int k = idx / (e[0] * e[1]);
-      const i64 gi_ = g.idx(R0[0] + i, R0[1] + j, R0[2] + k);
-      const int li = i + py * j + pz * k;
-      for (int c3 = 0; c3 < 3; c3++) {
-        const double v = smem[c3 * nj + li];
-        if (v != 0.0) unsafeAtomicAdd(a.J[c3] + gi_, v);
-      }
-    }
-    __syncthreads();
-  }
-}
-
-// Work items whose blocks did not fit in LDS: gather from global memory and
-// deposit with global atomics, one workgroup per work item.  Stage 1: the
-// particles were already pushed by k_fused.
-template <int BS, int N, bool LO, int PUSHER, int DEPO>
-__global__ __launch_bounds__(BS) void k_fused_overflow(FusedArgs a) {
-  const int n = min(*a.ovf_count, a.ovf_cap);
-  const GridGeom &g = a.g;
-  FieldView<double> fg[6];
-  for (int f = 0; f < 6; f++) fg[f] = FieldView<double>{a.F[f] + g.idx(0, 0, 0), g.NX, g.NX * g.NY};
-  const GAcc ga{{a.J[0], a.J[1], a.J[2]}, g};
-  const int none[3] = {0, 0, 0};  // empty block: every gather reads global memory
-  for (int o = blockIdx.x; o < n; o += gridDim.x) {
-    const int2 ov = a.ovf[o];
-    const Chunk ch = a.chunks[ov.x];
-    if (ov.y == 0) {
-      Box6 dep;
-      box_reset(dep);
-      for (int k = threadIdx.x; k < ch.n; k += BS)
-        gpd_push<N, LO, PUSHER, DEPO>(a, ch.begin + k, fg, fg, none, none, dep);
-    }
-    __syncthreads();
-    const int S = (ch.n + 63) >> 6;
-    for (int kk = threadIdx.x; kk < S * 64; kk += BS) {
-      const int k = a.spread ? (kk & 63) * S + (kk >> 6) : kk;
-      if (k < ch.n && !gpd_deposit<N, DEPO>(a, ch.begin + k, ga)) count_bad(a.bad);
-    }
-    __syncthreads();
-  }
-}
-
 }  // namespace pxr
 
 // ===========================================================================
 // C ABI
 // ===========================================================================
-static const int FUSED_BS = 256;
-
 namespace pxr {
 struct Tuning {
   int lds_bytes = 79616;  // dynamic LDS per workgroup: two workgroups (with their 2 KiB
@@ -930,6 +512,11 @@ struct Tuning {
   int j_lds_bytes = -1;   // cap on the LDS current block (negative: all of lds_bytes)
   int margin_pct = 0;     // motion margin of the current block, in 1/100 cell
   int jpad = 0;           // odd row/plane pitch of the LDS current block
+  int jacc = 2;           // mixed precision LDS current: 0 FP32, 1 compensated FP32, 2 FP64
+  // launch of a mixed-precision species: its kernels take fewer registers,
+  // so three workgroups with smaller LDS blocks fit on a CU
+  int mx_lds_bytes = 52224;
+  int mx_wg_per_cu = 3;
 };
 static Tuning g_tune;
 }  // namespace pxr
@@ -958,6 +545,16 @@ int pxrhip_set_tuning(const char *key, int64_t value) {
     g_tune.spread = value != 0;
   } else if (!strcmp(key, "fused.j_lds_bytes")) {
     g_tune.j_lds_bytes = value < 0 ? -1 : (value > (1 << 30) ? (1 << 30) : (int)value);
+  } else if (!strcmp(key, "mixed.lds_bytes")) {
+    if (value < 4096 || value > 160 * 1024 - 1024)
+      return set_error(PXRHIP_ERR_ARG, "mixed.lds_bytes must be in [4096, 159744]");
+    g_tune.mx_lds_bytes = (int)(value & ~7);
+  } else if (!strcmp(key, "mixed.wg_per_cu")) {
+    if (value < 1 || value > 16) return set_error(PXRHIP_ERR_ARG, "mixed.wg_per_cu must be in [1, 16]");
+    g_tune.mx_wg_per_cu = (int)value;
+  } else if (!strcmp(key, "mixed.jacc")) {
+    if (value < 0 || value > 2) return set_error(PXRHIP_ERR_ARG, "mixed.jacc must be 0, 1 or 2");
+    g_tune.jacc = (int)value;
   } else if (!strcmp(key, "fused.jpad")) {
     g_tune.jpad = value != 0;
   } else if (!strcmp(key, "fused.margin_pct")) {
@@ -981,7 +578,22 @@ int pxrhip_get_tuning(const char *key, int64_t *value) {
   else if (!strcmp(key, "fused.j_lds_bytes")) *value = g_tune.j_lds_bytes;
   else if (!strcmp(key, "fused.margin_pct")) *value = g_tune.margin_pct;
   else if (!strcmp(key, "fused.jpad")) *value = g_tune.jpad;
+  else if (!strcmp(key, "mixed.jacc")) *value = g_tune.jacc;
+  else if (!strcmp(key, "mixed.lds_bytes")) *value = g_tune.mx_lds_bytes;
+  else if (!strcmp(key, "mixed.wg_per_cu")) *value = g_tune.mx_wg_per_cu;
   else return set_error(PXRHIP_ERR_ARG, "unknown tuning key '%s'", key);
+  return PXRHIP_OK;
+}
+
+int pxrhip_tiles_stats(int64_t *nitems, int64_t *noverflow) {
+  PXR_INIT();
+  int *ctl = (int *)scratch(10, 4 * sizeof(int));
+  if (!ctl) return set_error(PXRHIP_ERR_HIP, "scratch allocation failed");
+  int h[4] = {0, 0, 0, 0};
+  PXR_HIP(hipMemcpyAsync(h, ctl, sizeof(h), hipMemcpyDeviceToHost, rt().stream));
+  PXR_HIP(hipStreamSynchronize(rt().stream));
+  if (nitems) *nitems = h[0];
+  if (noverflow) *noverflow = h[2];
   return PXRHIP_OK;
 }
 
@@ -1004,6 +616,8 @@ static int check_species(const pxrhip_species *s, bool need_bins) {
   if (need_bins && !s->bin_start) return set_error(PXRHIP_ERR_ARG, "null bin_start");
   if (s->capacity < 0 || s->capacity > ((int64_t)1 << 31) - 1)
     return set_error(PXRHIP_ERR_ARG, "capacity must be in [0, 2^31)");
+  if (s->precision != 0 && s->precision != 1)
+    return set_error(PXRHIP_ERR_ARG, "precision must be 0 (FP64) or 1 (mixed)");
   return PXRHIP_OK;
 }
 
@@ -1013,6 +627,8 @@ int pxrhip_tiles_rebin_sort(const pxrhip_grid *grid, pxrhip_species *src, pxrhip
   int r = get_table(grid, &tt);
   if (r) return r;
   if ((r = check_species(src, false)) || (r = check_species(dst, true))) return r;
+  if (dst->precision != src->precision)
+    return set_error(PXRHIP_ERR_ARG, "source and destination species differ in precision");
   if (dst->capacity < src->capacity)
     return set_error(PXRHIP_ERR_CAPACITY, "dst capacity smaller than src capacity");
   const i64 cap = src->capacity;
@@ -1055,12 +671,18 @@ int pxrhip_tiles_rebin_sort(const pxrhip_grid *grid, pxrhip_species *src, pxrhip
   hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nblk), dim3(256), 0, st, hist, nbins, bsum,
                      (i64 *)dst->bin_start, (i64 *)cursor);
   ScatterArgs sa;
-  double *S[8] = {src->x, src->y, src->z, src->ux, src->uy, src->uz, src->gaminv, src->w};
-  double *D[8] = {dst->x, dst->y, dst->z, dst->ux, dst->uy, dst->uz, dst->gaminv, dst->w};
-  for (int k = 0; k < 8; k++) {
+  double *S[3] = {src->x, src->y, src->z}, *D[3] = {dst->x, dst->y, dst->z};
+  void *SQ[5] = {src->ux, src->uy, src->uz, src->gaminv, src->w};
+  void *DQ[5] = {dst->ux, dst->uy, dst->uz, dst->gaminv, dst->w};
+  for (int k = 0; k < 3; k++) {
     sa.src[k] = S[k];
     sa.dst[k] = D[k];
   }
+  for (int k = 0; k < 5; k++) {
+    sa.srcq[k] = SQ[k];
+    sa.dstq[k] = DQ[k];
+  }
+  sa.mixed = src->precision == 1;
   sa.src_off = src->tile_off;
   sa.ntiles = tt->ntiles;
   sa.key = key;
@@ -1118,9 +740,13 @@ int pxrhip_tiles_gather_push_deposit(const pxrhip_grid *grid, const pxrhip_speci
     bg.tbox = tt->tbox;
     bg.n3 = tt->n3;
   }
-  const i64 lds_doubles = g_tune.lds_bytes / 8;
-  const i64 j_doubles = g_tune.j_lds_bytes < 0 || g_tune.j_lds_bytes / 8 > lds_doubles
-                            ? lds_doubles : g_tune.j_lds_bytes / 8;
+  const bool mx = sp->precision == 1;
+  const i64 lds_bytes = mx ? g_tune.mx_lds_bytes : g_tune.lds_bytes;
+  const i64 j_bytes = g_tune.j_lds_bytes < 0 || g_tune.j_lds_bytes > lds_bytes ? lds_bytes
+                                                                              : g_tune.j_lds_bytes;
+  const int esize = mx ? 4 : 8;
+  const int jacc = mx ? g_tune.jacc : 2;  // FP64 species: FP64 LDS current
+  const int jnode = jacc == 0 ? 12 : 24;  // 3 FP32 | 3 FP32 + 3 compensations | 3 FP64
   const i64 nunits = (i64)tt->ntiles * nzmax;
   const i64 maxchunks = tt->nrows + (cap / ch + 1) + nunits + 16;
   const i64 nblk = (nunits + SCAN_B - 1) / SCAN_B;
@@ -1136,8 +762,10 @@ int pxrhip_tiles_gather_push_deposit(const pxrhip_grid *grid, const pxrhip_speci
   ca.ch = ch;
   ca.nzmax = nzmax;
   ca.bg = bg;
-  ca.lds_doubles = lds_doubles;
-  ca.j_doubles = j_doubles;
+  ca.lds_bytes = lds_bytes;
+  ca.j_bytes = j_bytes;
+  ca.fnode_bytes = 6 * esize;
+  ca.jnode_bytes = jnode;
   ca.tile_off = sp->tile_off;
   ca.tile_np = sp->tile_np;
   ca.bin_start = sp->bin_start;
@@ -1162,7 +790,16 @@ int pxrhip_tiles_gather_push_deposit(const pxrhip_grid *grid, const pxrhip_speci
   a.nchunks = ctl;
   a.queue = ctl + 1;
   a.x = sp->x; a.y = sp->y; a.z = sp->z;
-  a.ux = sp->ux; a.uy = sp->uy; a.uz = sp->uz; a.gi = sp->gaminv; a.w = sp->w;
+  a.ux = a.uy = a.uz = a.gi = nullptr;
+  a.w = nullptr;
+  a.fux = a.fuy = a.fuz = a.fgi = nullptr;
+  a.fw = nullptr;
+  if (mx) {
+    a.fux = (float *)sp->ux; a.fuy = (float *)sp->uy; a.fuz = (float *)sp->uz;
+    a.fgi = (float *)sp->gaminv; a.fw = (const float *)sp->w;
+  } else {
+    a.ux = sp->ux; a.uy = sp->uy; a.uz = sp->uz; a.gi = sp->gaminv; a.w = sp->w;
+  }
   const double *F[6] = {exg, eyg, ezg, bxg, byg, bzg};
   for (int f = 0; f < 6; f++) a.F[f] = F[f];
   a.J[0] = jx; a.J[1] = jy; a.J[2] = jz;
@@ -1182,42 +819,20 @@ int pxrhip_tiles_gather_push_deposit(const pxrhip_grid *grid, const pxrhip_speci
   a.q = sp->q; a.dx = grid->dx; a.dy = grid->dy; a.dz = grid->dz;
   a.bad = rt().d_bad;
   a.bg = bg;
-  a.lds_doubles = (int)lds_doubles;
-  a.j_doubles = (int)j_doubles;
+  a.lds_bytes = (int)lds_bytes;
+  a.j_bytes = (int)j_bytes;
   a.spread = g_tune.spread;
   a.jpad = g_tune.jpad;
+  a.jacc = jacc;
   a.ovf_count = ctl + 2;
   a.ovf = ovf;
   a.ovf_cap = maxchunks < INT_MAX ? (int)maxchunks : INT_MAX;
-  const int nwg = rt().num_cus * g_tune.wg_per_cu;
-  dim3 grd(nwg), blk(FUSED_BS);
-  const size_t lds = (size_t)g_tune.lds_bytes;
+  const int nwg = rt().num_cus * (mx ? g_tune.mx_wg_per_cu : g_tune.wg_per_cu);
+  const size_t lds = (size_t)lds_bytes;
   const int key = (int)(order * 100 + (l_lower_order_in_v ? 10 : 0) + pusher * 2 + (depo_algo ? 1 : 0));
-#define PXR_FUSED_CASE(N_, LO_, P_, D_)                                                          \
-  case N_ * 100 + (LO_ ? 10 : 0) + P_ * 2 + (D_ ? 1 : 0):                                        \
-    if (lds > 65536)                                                                             \
-      PXR_HIP(hipFuncSetAttribute((const void *)&k_fused<FUSED_BS, N_, LO_, P_, D_>,             \
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));       \
-    hipLaunchKernelGGL((k_fused<FUSED_BS, N_, LO_, P_, D_>), grd, blk, lds, st, a);              \
-    hipLaunchKernelGGL((k_fused_overflow<FUSED_BS, N_, LO_, P_, D_>), dim3(2 * rt().num_cus), blk, 0, \
-                       st, a);                                                                   \
-    break;
-  switch (key) {
-    PXR_FUSED_CASE(1, true, 0, 0) PXR_FUSED_CASE(1, true, 0, 3) PXR_FUSED_CASE(1, true, 1, 0)
-    PXR_FUSED_CASE(1, true, 1, 3) PXR_FUSED_CASE(1, false, 0, 0) PXR_FUSED_CASE(1, false, 0, 3)
-    PXR_FUSED_CASE(1, false, 1, 0) PXR_FUSED_CASE(1, false, 1, 3)
-    PXR_FUSED_CASE(2, true, 0, 0) PXR_FUSED_CASE(2, true, 0, 3) PXR_FUSED_CASE(2, true, 1, 0)
-    PXR_FUSED_CASE(2, true, 1, 3) PXR_FUSED_CASE(2, false, 0, 0) PXR_FUSED_CASE(2, false, 0, 3)
-    PXR_FUSED_CASE(2, false, 1, 0) PXR_FUSED_CASE(2, false, 1, 3)
-    PXR_FUSED_CASE(3, true, 0, 0) PXR_FUSED_CASE(3, true, 0, 3) PXR_FUSED_CASE(3, true, 1, 0)
-    PXR_FUSED_CASE(3, true, 1, 3) PXR_FUSED_CASE(3, false, 0, 0) PXR_FUSED_CASE(3, false, 0, 3)
-    PXR_FUSED_CASE(3, false, 1, 0) PXR_FUSED_CASE(3, false, 1, 3)
-    default:
-      return set_error(PXRHIP_ERR_ARG, "unsupported kernel combination");
-  }
-#undef PXR_FUSED_CASE
-  PXR_LAUNCHED();
-  return PXRHIP_OK;
+  if (!mx) return launch_fused_fp64(key, a, nwg, lds, st);
+  return jacc == 2 ? launch_fused_mixed_f64acc(key, a, nwg, lds, st)
+                   : launch_fused_mixed_f32(key, a, nwg, lds, st);
 }
 
 }  // extern "C"

@@ -13,18 +13,19 @@ import numpy as np
 
 from . import _lib
 
-_DT = {np.dtype(np.float64): 8, np.dtype(np.int64): 8}
+_DT = {np.dtype(np.float64): 8, np.dtype(np.int64): 8, np.dtype(np.float32): 4}
 
 
 class DeviceArray:
-    """A device buffer of ``n`` float64 or int64 values."""
+    """A device buffer of ``n`` float64, int64 or float32 values."""
 
     def __init__(self, n: int, dtype=np.float64):
         self.n = int(n)
         self.dtype = np.dtype(dtype)
         if self.dtype not in _DT:
-            raise TypeError("DeviceArray holds float64 or int64")
-        self.nbytes = self.n * 8
+            raise TypeError("DeviceArray holds float64, int64 or float32")
+        self.itemsize = _DT[self.dtype]
+        self.nbytes = self.n * self.itemsize
         p = C.c_void_p(0)
         _lib.call("pxrhip_malloc", C.byref(p), max(self.nbytes, 8))
         self._ptr = p.value
@@ -56,25 +57,30 @@ class DeviceArray:
         a = np.ascontiguousarray(np.asarray(a, dtype=self.dtype).ravel())
         if offset < 0 or offset + a.size > self.n:
             raise ValueError("upload past the end of the device array")
-        _lib.call("pxrhip_memcpy_htod", C.c_void_p(self._ptr + 8 * offset),
+        _lib.call("pxrhip_memcpy_htod", C.c_void_p(self._ptr + self.itemsize * offset),
                   a.ctypes.data_as(C.c_void_p), a.nbytes)
 
     def numpy(self, count: int = None) -> np.ndarray:
         n = self.n if count is None else int(count)
         out = np.empty(n, dtype=self.dtype)
         _lib.call("pxrhip_memcpy_dtoh", out.ctypes.data_as(C.c_void_p), C.c_void_p(self._ptr),
-                  n * 8)
+                  n * self.itemsize)
         return out
 
     def fill(self, value):
         if self.dtype == np.float64:
             _lib.call("pxrhip_fill_f64", C.c_void_p(self._ptr), self.n, float(value))
-        else:
+        elif self.dtype == np.int64:
             _lib.call("pxrhip_fill_i64", C.c_void_p(self._ptr), self.n, int(value))
+        else:
+            self.upload(np.full(self.n, value, dtype=self.dtype))
 
     def copy_from(self, other: "DeviceArray", count: int = None):
+        if other.dtype != self.dtype:
+            raise TypeError("copy between device arrays of different dtypes")
         n = min(self.n, other.n) if count is None else int(count)
-        _lib.call("pxrhip_memcpy_dtod", C.c_void_p(self._ptr), C.c_void_p(other.ptr), n * 8)
+        _lib.call("pxrhip_memcpy_dtod", C.c_void_p(self._ptr), C.c_void_p(other.ptr),
+                  n * self.itemsize)
 
 
 class DeviceGrid(DeviceArray):

@@ -51,12 +51,22 @@ class PxrGrid(C.Structure):
 class PxrSpecies(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in ("x", "y", "z", "ux", "uy", "uz", "gaminv", "w",
                                           "tile_off", "tile_np", "bin_start")] + \
-               [("capacity", C.c_int64), ("q", C.c_double), ("m", C.c_double)]
+               [("capacity", C.c_int64), ("q", C.c_double), ("m", C.c_double),
+                ("precision", C.c_int64)]
 
 
 PUSHERS = {"boris": 0, "vay": 1}
 DEPOS = {"esirkepov": 0, "direct": 3}
 COMPONENTS = ("x", "y", "z", "ux", "uy", "uz", "gaminv", "w")
+PRECISIONS = {"fp64": 0, "mixed": 1}
+
+
+def component_dtype(name: str, precision: str):
+    """Storage type of a particle component: positions are always float64;
+    in mixed precision momenta, gaminv and weights are float32."""
+    if precision == "mixed" and name not in ("x", "y", "z"):
+        return np.float32
+    return np.float64
 
 
 @dataclass
@@ -80,6 +90,7 @@ class Species:
     cur: int = 0
     lfreeze: bool = False   # particle_species%lfreeze: not pushed
     ldodepos: bool = True   # particle_species%ldodepos: deposits current
+    precision: str = "fp64"
 
     @property
     def active(self) -> _Buffers:
@@ -100,6 +111,7 @@ class Species:
         s.capacity = self.capacity
         s.q = float(self.q)
         s.m = float(self.m)
+        s.precision = PRECISIONS[self.precision]
         return s
 
 
@@ -114,8 +126,11 @@ class Simulation:
     def __init__(self, nx, ny, nz, dx=1.0, dy=1.0, dz=1.0, xmin=0.0, ymin=0.0, zmin=0.0,
                  ntx=1, nty=1, ntz=1, order=1, l_lower_order_in_v=True, pusher="boris",
                  depo="esirkepov", norder=2, dt=None, cfl=0.95, nguard=None, sorting_dx=1.0,
-                 sorting_dy=1.0, sorting_dz=1.0, chunk=0):
+                 sorting_dy=1.0, sorting_dz=1.0, chunk=0, precision="fp64"):
         _lib.call("pxrhip_init", -1)
+        if precision not in PRECISIONS:
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        self.precision = precision
         self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
         self.dx, self.dy, self.dz = float(dx), float(dy), float(dz)
         self.xmin, self.ymin, self.zmin = float(xmin), float(ymin), float(zmin)
@@ -179,9 +194,9 @@ class Simulation:
             if v.size != n:
                 raise ValueError(f"{k}: {v.size} values, expected {n}")
         sp = Species(name=name, q=float(q), m=float(m), capacity=cap, npart=n,
-                     lfreeze=lfreeze, ldodepos=ldodepos)
+                     lfreeze=lfreeze, ldodepos=ldodepos, precision=self.precision)
         for _ in range(2):
-            arrays = {k: DeviceArray(cap) for k in COMPONENTS}
+            arrays = {k: DeviceArray(cap, component_dtype(k, self.precision)) for k in COMPONENTS}
             b = _Buffers(arrays=arrays, tile_off=DeviceArray(self.ntiles + 1, np.int64),
                          tile_np=DeviceArray(self.ntiles, np.int64),
                          bin_start=DeviceArray(self.nbins + 1, np.int64))
@@ -314,6 +329,50 @@ class Simulation:
             self.check()
 
     # ------------------------------------------------------------------
+    # HIP graph of whole steps
+    # ------------------------------------------------------------------
+    def capture_steps(self, nst=2) -> "StepGraph":
+        """Record ``nst`` steps (an even number: each step swaps the two
+        buffers of every species, so after an even number the buffers are
+        where they were) into a HIP graph without running them.  Run one
+        step first so that the library's scratch buffers have their sizes;
+        particle counts must not change while the graph is in use."""
+        nst = int(nst)
+        if nst < 2 or nst % 2:
+            raise ValueError("capture an even number of steps")
+        if self.it == 0:
+            raise RuntimeError("run one step before capturing (scratch buffers are sized then)")
+        curs = [sp.cur for sp in self.species]
+        _lib.call("pxrhip_graph_begin")
+        try:
+            for _ in range(nst):
+                self.gather_push_deposit()
+                self.current_bcs()
+                self.field_step()
+                self.particle_bcs_and_sorting()
+        except Exception:
+            h = C.c_void_p(0)
+            try:
+                _lib.call("pxrhip_graph_end", C.byref(h))
+            finally:
+                if h.value:
+                    _lib.call("pxrhip_graph_destroy", h)
+            raise
+        h = C.c_void_p(0)
+        _lib.call("pxrhip_graph_end", C.byref(h))
+        for sp, c in zip(self.species, curs):
+            if sp.cur != c:  # recording flipped the buffers; nothing ran
+                raise RuntimeError("species buffers out of step after capture")
+        return StepGraph(self, h.value, nst)
+
+    def step_graph(self, graph: "StepGraph", nreplay=1, check=True):
+        """Replay a captured graph ``nreplay`` times."""
+        for _ in range(int(nreplay)):
+            graph.launch()
+        if check:
+            self.check()
+
+    # ------------------------------------------------------------------
     # diagnostics (diags.F90:847-1114, kin_energy.F90:33-143)
     # ------------------------------------------------------------------
     def kinetic_energy(self) -> float:
@@ -322,8 +381,8 @@ class Simulation:
         for sp in self.species:
             out = C.c_double(0.0)
             b = sp.active
-            _lib.call("pxrhip_kinetic_energy", sp.npart, b.arrays["gaminv"].ptr, b.arrays["w"].ptr,
-                      C.byref(out))
+            fn = "pxrhip_kinetic_energy_f32" if sp.precision == "mixed" else "pxrhip_kinetic_energy"
+            _lib.call(fn, sp.npart, b.arrays["gaminv"].ptr, b.arrays["w"].ptr, C.byref(out))
             tot += out.value * sp.m
         return tot
 
@@ -340,6 +399,26 @@ class Simulation:
 
     def total_particles(self) -> int:
         return sum(sp.npart for sp in self.species)
+
+
+class StepGraph:
+    """A HIP graph of ``nst`` whole steps of one Simulation."""
+
+    def __init__(self, sim: "Simulation", handle: int, nst: int):
+        self.sim, self.handle, self.nst = sim, handle, nst
+
+    def launch(self):
+        _lib.call("pxrhip_graph_launch", C.c_void_p(self.handle))
+        self.sim.it += self.nst
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h:
+            try:
+                _lib.call("pxrhip_graph_destroy", C.c_void_p(h))
+            except Exception:
+                pass
+            self.handle = None
 
 
 def _fd_weights(norder: int) -> np.ndarray:

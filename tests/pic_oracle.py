@@ -135,16 +135,18 @@ class OracleState:
                                            self.ng, self.ng, self.ng, 0, 0, 0)
 
 
-def thermal_plasma(rng, nx, ny, nz, ppc, dx, dy, dz, xmin, ymin, zmin, uth, id_offset=0):
+def thermal_plasma(rng, nx, ny, nz, ppc, dx, dy, dz, xmin, ymin, zmin, uth, id_offset=0,
+                   id_scale=1e-9):
     """Uniform random positions, Maxwellian momenta (normal per component);
-    weights carry a unique id in their low digits so that particles can be
-    matched after re-ordering."""
+    weights 1 + id * id_scale carry a unique id so that particles can be
+    matched after re-ordering (id_scale = 2**-15 keeps them exact and
+    distinct in float32 for up to 32768 particles)."""
     n = nx * ny * nz * ppc
     x = xmin + rng.uniform(0, nx, n) * dx
     y = ymin + rng.uniform(0, ny, n) * dy
     z = zmin + rng.uniform(0, nz, n) * dz
     ux, uy, uz = (rng.standard_normal(n) * uth for _ in range(3))
-    w = 1.0 + (np.arange(n) + id_offset) * 1e-9
+    w = 1.0 + (np.arange(n) + id_offset) * id_scale
     return x, y, z, ux, uy, uz, w
 
 

@@ -1,0 +1,147 @@
+"""Mixed precision on the GPU against the FP64 oracle.
+
+A mixed-precision species keeps FP64 positions and FP32 momenta, gamma and
+weights; the gather, the push and the per-particle current are FP32; the
+LDS current block accumulates in FP64 (default), compensated FP32 or plain
+FP32; the global current is FP64.  The bar
+of the method is 1e-5 relative error against the FP64 oracle on J and on
+the momenta after one step; several steps are compared at a looser bar.
+The discrete continuity of the mixed Esirkepov current is checked with
+FP64 charge densities at the old and new positions.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from tests.test_gpu_step import build, rel
+
+pytestmark = pytest.mark.gpu
+
+MIXED_TOL = 1e-5
+
+
+@pytest.fixture(scope="module")
+def torch():
+    from picsar_amd import _lib
+    _lib.call("pxrhip_init", 0)  # raises unless a gfx950 device is present
+    return None
+
+
+def set_tuning(key, value):
+    from picsar_amd import _lib
+    old = C.c_int64(0)
+    _lib.call("pxrhip_get_tuning", key.encode(), C.byref(old))
+    _lib.call("pxrhip_set_tuning", key.encode(), value)
+    return old.value
+
+
+def mixed_build(torch, orc, order, lo, pusher, depo, **kw):
+    return build(torch, orc, order, lo, pusher, depo, precision="mixed", id_scale=2.0 ** -15,
+                 **kw)
+
+
+def errors(sim, ref):
+    out = {}
+    for k in ("jx", "jy", "jz", "ex", "ey", "ez", "bx", "by", "bz"):
+        out[k] = rel(sim.fields[k].numpy(), ref.f[k])
+    for sp, rs in zip(sim.species, ref.species):
+        a = sp.arrays()
+        ia, ib = np.argsort(a["w"]), np.argsort(rs["w"].astype(np.float32))
+        for k in ("ux", "uy", "uz"):
+            out[f"{sp.name}.{k}"] = rel(a[k][ia].astype(np.float64), rs[k][ib])
+        for k in ("x", "y", "z"):
+            out[f"{sp.name}.{k}.abs"] = np.abs(a[k][ia] - rs[k][ib]).max()
+    return out
+
+
+def unique_float32_weights(ref):
+    # weights carry particle ids; they must stay distinct in float32
+    for rs in ref.species:
+        assert len(np.unique(rs["w"].astype(np.float32))) == len(rs["w"])
+
+
+@pytest.mark.parametrize("order,lo,pusher,depo", [
+    (1, True, "boris", "direct"),
+    (2, True, "vay", "esirkepov"),
+    (3, True, "vay", "esirkepov"),
+    (3, False, "boris", "esirkepov"),
+    (3, True, "vay", "direct"),
+])
+@pytest.mark.parametrize("jacc", [2, 1, 0])
+def test_one_step_against_fp64_oracle(torch, orc, order, lo, pusher, depo, jacc):
+    """jacc: the LDS current in FP64 (2), compensated FP32 (1), FP32 (0)."""
+    old = set_tuning("mixed.jacc", jacc)
+    try:
+        sim, ref, _ = mixed_build(torch, orc, order, lo, pusher, depo, ppc=2)
+        unique_float32_weights(ref)
+        # the oracle runs on the same (FP32-rounded) momenta and weights
+        for sp, rs in zip(sim.species, ref.species):
+            for k in ("ux", "uy", "uz", "w"):
+                rs[k][...] = rs[k].astype(np.float32).astype(np.float64)
+            rs["gaminv"][...] = 1.0 / np.sqrt(1 + rs["ux"] ** 2 + rs["uy"] ** 2 + rs["uz"] ** 2)
+        sim.step(1)
+        ref.step()
+    finally:
+        set_tuning("mixed.jacc", old)
+    err = errors(sim, ref)
+    for k in ("jx", "jy", "jz", "ex", "ey", "ez", "bx", "by", "bz"):
+        assert err[k] <= MIXED_TOL, (k, err[k])
+    for k, v in err.items():
+        if k.endswith(("ux", "uy", "uz")):
+            assert v <= MIXED_TOL, (k, v)
+        if k.endswith(".abs"):
+            assert v <= 1e-7, (k, v)
+
+
+def test_ten_steps_against_fp64_oracle(torch, orc):
+    sim, ref, _ = mixed_build(torch, orc, 3, True, "vay", "esirkepov", ppc=2)
+    for sp, rs in zip(sim.species, ref.species):
+        for k in ("ux", "uy", "uz", "w"):
+            rs[k][...] = rs[k].astype(np.float32).astype(np.float64)
+        rs["gaminv"][...] = 1.0 / np.sqrt(1 + rs["ux"] ** 2 + rs["uy"] ** 2 + rs["uz"] ** 2)
+    for _ in range(10):
+        sim.step(1)
+        ref.step()
+    err = errors(sim, ref)
+    for k in ("jx", "jy", "jz", "ex", "ey", "ez", "bx", "by", "bz"):
+        assert err[k] <= 1e-4, (k, err[k])
+    for k, v in err.items():
+        if k.endswith(("ux", "uy", "uz")):
+            assert v <= 1e-4, (k, v)
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_mixed_esirkepov_continuity(torch, orc, order):
+    """(rho_new - rho_old)/dt + div J with J from the mixed-precision tiled
+    kernel and FP64 rho at the positions before and after the push: zero
+    to FP32 round-off (relative to rho/dt), although particles move only a
+    small fraction of a cell (the factored shape differences)."""
+    from picsar_amd.devmem import DeviceArray, DeviceGrid
+    from picsar_amd.picsarpy import picsar
+    sim, ref, _ = mixed_build(torch, orc, order, True, "vay", "esirkepov", ppc=2,
+                              uth=(0.02, 0.002), field_amp=0.0)
+    n, g = (sim.nx, sim.ny, sim.nz), sim.ng
+
+    def rho_now():
+        r = DeviceGrid(*n, g)
+        for sp in sim.species:
+            a = sp.arrays()
+            picsar.pxr_depose_rho_n(r, sp.npart, *[DeviceArray.from_numpy(a[k]) for k in
+                                                   ("x", "y", "z")],
+                                    DeviceArray.from_numpy(a["w"].astype(np.float64)), sp.q,
+                                    sim.xmin, sim.ymin, sim.zmin, sim.dx, sim.dy, sim.dz, *n,
+                                    g, g, g, order, order, order)
+        return r.numpy()
+    r0 = rho_now()
+    sim.gather_push_deposit()
+    sim.check()
+    r1 = rho_now()
+    J = [sim.fields[k].numpy() for k in ("jx", "jy", "jz")]
+    div = np.zeros_like(r0)
+    div[1:, :, :] += (J[0][1:, :, :] - J[0][:-1, :, :]) / sim.dx
+    div[:, 1:, :] += (J[1][:, 1:, :] - J[1][:, :-1, :]) / sim.dy
+    div[:, :, 1:] += (J[2][:, :, 1:] - J[2][:, :, :-1]) / sim.dz
+    res = (r1 - r0) / sim.dt + div
+    scale = np.abs(r1).max() / sim.dt
+    assert np.abs(res[1:, 1:, 1:]).max() <= 1e-6 * scale, np.abs(res[1:, 1:, 1:]).max() / scale

@@ -27,7 +27,8 @@ def rel(a, b):
 
 
 def build(torch, orc, order, lo, pusher, depo, norder=2, ntiles=(2, 2, 2), n=(16, 14, 12),
-          ppc=4, uth=(0.2, 0.02), chunk=0, field_amp=0.05, seed=1, d=(1.0, 0.9, 1.1)):
+          ppc=4, uth=(0.2, 0.02), chunk=0, field_amp=0.05, seed=1, d=(1.0, 0.9, 1.1),
+          precision="fp64", id_scale=1e-9):
     from picsar_amd.sim import Simulation, _fd_weights
     nx, ny, nz = n
     dx, dy, dz = d
@@ -35,7 +36,7 @@ def build(torch, orc, order, lo, pusher, depo, norder=2, ntiles=(2, 2, 2), n=(16
     dt = default_dt(dx, dy, dz)
     sim = Simulation(nx, ny, nz, dx, dy, dz, xmin, ymin, zmin, *ntiles, order=order,
                      l_lower_order_in_v=lo, pusher=pusher, depo=depo, norder=norder, dt=dt,
-                     chunk=chunk)
+                     chunk=chunk, precision=precision)
     ref = OracleState(orc, nx, ny, nz, sim.ng, dx, dy, dz, xmin, ymin, zmin, dt, order, lo,
                       pusher, depo, norder, _fd_weights(norder) if norder > 2 else None)
     rng = np.random.default_rng(seed)
@@ -47,7 +48,8 @@ def build(torch, orc, order, lo, pusher, depo, norder=2, ntiles=(2, 2, 2), n=(16
         sim.fields[k].upload(a)
     off = 0
     for (q, m, u) in ((-1.0, 1.0, uth[0]), (1.0, 1836.0, uth[1])):
-        p = thermal_plasma(rng, nx, ny, nz, ppc, dx, dy, dz, xmin, ymin, zmin, u, id_offset=off)
+        p = thermal_plasma(rng, nx, ny, nz, ppc, dx, dy, dz, xmin, ymin, zmin, u, id_offset=off,
+                           id_scale=id_scale)
         off += len(p[0])
         ref.add_species(q, m, *p)
         sim.add_species("e" if q < 0 else "i", q, m, *p, gaminv=ref.species[-1]["gaminv"])
@@ -241,3 +243,97 @@ def test_c1_config_hundred_steps(torch, orc):
             assert np.abs(a[k][ia] - rs[k][ib]).max() <= 1e-9 * n, k
         for k in ("ux", "uy", "uz"):
             assert rel(a[k][ia], rs[k][ib]) <= 1e-9, k
+
+
+def test_graph_replays_match_eager_steps(torch, orc):
+    """Two whole steps captured in a HIP graph and replayed twice, with eager
+    work and synchronisation between the replays, give the eager steps'
+    state after each replay, and the oracle's after five steps."""
+    sim_e, ref, tng = build(torch, orc, 3, True, "vay", "esirkepov", ppc=2)
+    sim_g, _, _ = build(torch, orc, 3, True, "vay", "esirkepov", ppc=2)
+    sim_e.step(1)
+    sim_g.step(1)
+    ref.step()
+    graph = sim_g.capture_steps(2)
+    for replay in range(2):
+        sim_e.step(2)
+        sim_g.step_graph(graph)
+        for _ in range(2):
+            ref.step()
+        sim_g.kinetic_energy()  # eager work and a synchronisation between replays
+        for k in ("ex", "ey", "ez", "bx", "by", "bz", "jx", "jy", "jz"):
+            a, b = sim_g.fields[k].numpy(), sim_e.fields[k].numpy()
+            assert rel(a, b) <= 1e-12, (replay, k, rel(a, b))
+        check_tiles(sim_g)
+    assert sim_g.it == sim_e.it == 5
+    compare_fields(sim_g, ref, tng, 1e-11)
+    compare_particles(sim_g, ref, 1e-11, 1e-11)
+    del graph
+
+
+def test_autotune_keeps_the_physics(torch, orc):
+    """The autotuner's trial steps are real steps: after trying two launch
+    configurations (3 steps each) the simulation is the oracle's after 6
+    steps, and the faster configuration is left applied."""
+    from picsar_amd.autotune import autotune, get_tuning, set_tuning
+    before = get_tuning()
+    sim, ref, tng = build(torch, orc, 3, True, "vay", "esirkepov", ppc=2)
+    cands = [{"fused.chunk": 512, "fused.spread": 1}, {"fused.chunk": 4096, "fused.spread": 0}]
+    try:
+        res = autotune(sim, cands, steps=2)
+        assert res["best"] in cands and len(res["trials"]) == 2
+        assert get_tuning(("fused.chunk",))["fused.chunk"] == res["best"]["fused.chunk"]
+    finally:
+        set_tuning(before)
+    for _ in range(6):
+        ref.step()
+    assert sim.it == 6
+    check_tiles(sim)
+    compare_fields(sim, ref, tng, 1e-9)
+    compare_particles(sim, ref, 1e-9, 1e-9)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_relativistic_bunch_on_uneven_tiles(torch, orc, precision):
+    """A dense gamma = 10 electron bunch (moving half a cell per step along
+    z, so that most particles cross a cell face) in a sparse background:
+    tiles hold from a handful to thousands of particles.  Three steps
+    against the oracle (FP64 at 1e-12 per step compounded to 1e-11; mixed
+    precision at 1e-5)."""
+    from picsar_amd.sim import Simulation
+    n = (24, 24, 32)
+    dt = default_dt(1.0, 1.0, 1.0)
+    sim = Simulation(*n, 1.0, 1.0, 1.0, 0.0, 0.0, 0.0, 3, 3, 4, order=3, l_lower_order_in_v=True,
+                     pusher="vay", depo="esirkepov", norder=2, dt=dt, precision=precision)
+    ref = OracleState(orc, *n, sim.ng, 1.0, 1.0, 1.0, 0.0, 0.0, 0.0, dt, 3, True, "vay",
+                      "esirkepov")
+    rng = np.random.default_rng(77)
+    nb, nbg = 20000, 24 * 24 * 32
+    scale = 2.0 ** -15 if precision == "mixed" else 1e-9
+    pos = [np.clip(rng.normal(c, 2.5, nb), 0.0, np.nextafter(m, 0)) for c, m in
+           ((12.0, 24.0), (12.0, 24.0), (16.0, 32.0))]
+    mom = [rng.standard_normal(nb) * 0.05, rng.standard_normal(nb) * 0.05,
+           np.sqrt(99.0) + rng.standard_normal(nb) * 0.1]
+    bg = [rng.random(nbg) * m for m in n]
+    bmom = [rng.standard_normal(nbg) * 0.01 for _ in range(3)]
+    for name, p, u, off in (("bunch", pos, mom, 0), ("background", bg, bmom, nb)):
+        w = 1e-3 * (1.0 + (np.arange(len(p[0])) + off) * scale)
+        if precision == "mixed":
+            u = [a.astype(np.float32).astype(np.float64) for a in u]
+            w = w.astype(np.float32).astype(np.float64)
+        ref.add_species(-1.0, 1.0, *p, *u, w)
+        sim.add_species(name, -1.0, 1.0, *p, *u, w, gaminv=ref.species[-1]["gaminv"])
+    counts = sim.species[0].active.tile_np.numpy()
+    assert counts.max() > 50 * max(counts.min(), 1)
+    for _ in range(3):
+        sim.step(1)
+        ref.step()
+    check_tiles(sim)
+    tol = 1e-11 if precision == "fp64" else 1e-5
+    for k in ("ex", "ey", "ez", "bx", "by", "bz", "jx", "jy", "jz"):
+        assert rel(sim.fields[k].numpy(), ref.f[k]) <= tol, (k, rel(sim.fields[k].numpy(), ref.f[k]))
+    for sp, rs in zip(sim.species, ref.species):
+        a = sp.arrays()
+        ia, ib = np.argsort(a["w"]), np.argsort(rs["w"])
+        for k in ("ux", "uy", "uz"):
+            assert rel(a[k][ia].astype(np.float64), rs[k][ib]) <= tol, k
