@@ -295,6 +295,7 @@ int pxrhip_tiles_rebin_sort(const pxrhip_grid *grid, pxrhip_species *src,
  * (its CPU blocking factors lvec_* play this role).  Keys:
  * "fused.lds_bytes" (dynamic LDS per workgroup), "fused.wg_per_cu",
  * "fused.spread" (0/1, lane-spread particle order in the deposition phase),
+ * "fused.sstride" (largest lane stride of that order, 0: the whole work item),
  * "fused.chunk" (default particles per work item; a work item also holds no
  * more sorting-grid rows than its LDS blocks allow),
  * "fused.j_lds_bytes" (largest current block accumulated in LDS; negative:

@@ -117,6 +117,7 @@ struct FusedArgs {
   int lds_bytes;         // dynamic LDS of a workgroup
   int j_bytes;           // largest current block kept in LDS
   int spread;            // lanes take particles S apart
+  int sstride;           // largest S (0: spread over the whole work item)
   int jpad;              // odd pitches for the LDS current block
   int jacc;              // mixed precision: LDS current accumulation (see LdsAccMx)
   int *ovf_count;        // work items whose blocks do not fit in LDS ...
@@ -516,20 +517,28 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void k_
     __syncthreads();
     // ---- 3. deposit -------------------------------------------------------
     {
-      const int S = (ch.n + 63) >> 6;
+      // lanes take particles S apart inside super-blocks of 64*S particles
+      int S = (ch.n + 63) >> 6;
+      if (a.sstride > 0 && a.sstride < S) S = a.sstride;
+      const int SB = 64 * S;
+      const int nkk = ((ch.n + SB - 1) / SB) * SB;
+      auto pick = [&](int kk) {
+        const int r = kk % SB;
+        return a.spread ? (kk - r) + (r & 63) * S + (r >> 6) : kk;
+      };
       if constexpr (MX) {
         float *se = reinterpret_cast<float *>(smem) + 3 * nj;  // compensation (JACC 1)
         const LdsAccMx<JACC> la{{sj, sj + nj, sj + 2 * nj}, {se, se + nj, se + 2 * nj},
                                 R0[0], R0[1], R0[2], py, pz};
-        for (int kk = threadIdx.x; kk < S * 64; kk += BS) {
-          const int k = a.spread ? (kk & 63) * S + (kk >> 6) : kk;
+        for (int kk = threadIdx.x; kk < nkk; kk += BS) {
+          const int k = pick(kk);
           if (k < ch.n && !gpd_deposit_mx<N, DEPO>(a, p0 + k, la)) count_bad(a.bad);
         }
       } else {
         double *sjd = reinterpret_cast<double *>(smem);
         const LdsAcc la{{sjd, sjd + nj, sjd + 2 * nj}, R0[0], R0[1], R0[2], py, pz};
-        for (int kk = threadIdx.x; kk < S * 64; kk += BS) {
-          const int k = a.spread ? (kk & 63) * S + (kk >> 6) : kk;
+        for (int kk = threadIdx.x; kk < nkk; kk += BS) {
+          const int k = pick(kk);
           if (k < ch.n && !gpd_deposit<N, DEPO>(a, p0 + k, la)) count_bad(a.bad);
         }
       }

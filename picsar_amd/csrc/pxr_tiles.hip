@@ -512,6 +512,7 @@ struct Tuning {
   int j_lds_bytes = -1;   // cap on the LDS current block (negative: all of lds_bytes)
   int margin_pct = 0;     // motion margin of the current block, in 1/100 cell
   int jpad = 0;           // odd row/plane pitch of the LDS current block
+  int sstride = 0;        // largest lane stride of the deposition (0: whole work item)
   int jacc = 2;           // mixed precision LDS current: 0 FP32, 1 compensated FP32, 2 FP64
   // launch of a mixed-precision species: its kernels take fewer registers,
   // so three workgroups with smaller LDS blocks fit on a CU
@@ -555,6 +556,9 @@ int pxrhip_set_tuning(const char *key, int64_t value) {
   } else if (!strcmp(key, "mixed.jacc")) {
     if (value < 0 || value > 2) return set_error(PXRHIP_ERR_ARG, "mixed.jacc must be 0, 1 or 2");
     g_tune.jacc = (int)value;
+  } else if (!strcmp(key, "fused.sstride")) {
+    if (value < 0 || value > (1 << 20)) return set_error(PXRHIP_ERR_ARG, "fused.sstride must be in [0, 2^20]");
+    g_tune.sstride = (int)value;
   } else if (!strcmp(key, "fused.jpad")) {
     g_tune.jpad = value != 0;
   } else if (!strcmp(key, "fused.margin_pct")) {
@@ -578,6 +582,7 @@ int pxrhip_get_tuning(const char *key, int64_t *value) {
   else if (!strcmp(key, "fused.j_lds_bytes")) *value = g_tune.j_lds_bytes;
   else if (!strcmp(key, "fused.margin_pct")) *value = g_tune.margin_pct;
   else if (!strcmp(key, "fused.jpad")) *value = g_tune.jpad;
+  else if (!strcmp(key, "fused.sstride")) *value = g_tune.sstride;
   else if (!strcmp(key, "mixed.jacc")) *value = g_tune.jacc;
   else if (!strcmp(key, "mixed.lds_bytes")) *value = g_tune.mx_lds_bytes;
   else if (!strcmp(key, "mixed.wg_per_cu")) *value = g_tune.mx_wg_per_cu;
@@ -823,6 +828,7 @@ int pxrhip_tiles_gather_push_deposit(const pxrhip_grid *grid, const pxrhip_speci
   a.j_bytes = (int)j_bytes;
   a.spread = g_tune.spread;
   a.jpad = g_tune.jpad;
+  a.sstride = g_tune.sstride;
   a.jacc = jacc;
   a.ovf_count = ctl + 2;
   a.ovf = ovf;
