@@ -284,9 +284,9 @@ int pxrhip_tiles_gather_push_deposit(
  * particle_sorting_sub (sorting.F90:106-215): every particle is wrapped into
  * the periodic box, assigned to the tile that now holds it (tile index of
  * particle_boundaries.F90:397) and the tiles are ordered by sorting-grid
- * cell (pxr_particle_bin_sorting's key).  src is read (its positions are
- * wrapped in place) and the result is written to dst, which must have the
- * same capacity; dst's tile_off, tile_np and bin_start are filled in.
+ * cell (pxr_particle_bin_sorting's key).  src is read and left as it is;
+ * the result, with wrapped positions, is written to dst, which must have
+ * the same capacity; dst's tile_off, tile_np and bin_start are filled in.
  * Particles of one cell keep no particular order among themselves. */
 int pxrhip_tiles_rebin_sort(const pxrhip_grid *grid, pxrhip_species *src,
                             pxrhip_species *dst);

@@ -409,4 +409,32 @@ PXR_DEV void wave_peers(bool active, int key, int &rank, int &cnt, int &leader) 
   leader = mine ? __ffsll((unsigned long long)mine) - 1 : lane;
 }
 
+// Runs of equal keys on neighbouring active lanes (keys that are mostly
+// sorted along the lanes): head is the lowest lane of this lane's run, cnt
+// the run's length on its head lane and 0 elsewhere.  A run ends at a key
+// change or an inactive lane.  Two ballots, whatever the number of keys.
+PXR_DEV void wave_runs(int key, int &head, int &cnt) {
+  const int lane = __lane_id();
+  const unsigned long long act = __ballot(true);
+  const int prev = __shfl(key, (lane + 63) & 63);
+  const bool h = lane == 0 || !((act >> (lane - 1)) & 1ull) || prev != key;
+  const unsigned long long heads = __ballot(h);
+  const unsigned long long upto = (2ull << lane) - 1ull;  // lanes 0..lane (all for lane 63)
+  head = 63 - __clzll((long long)(heads & upto));
+  // boundaries strictly above this lane
+  const unsigned long long stop = (heads | ~act) & ~upto;
+  const int next = stop ? __ffsll((unsigned long long)stop) - 1 : 64;
+  cnt = h ? next - lane : 0;
+}
+
+// Block index for 1-D grids whose neighbouring blocks touch neighbouring
+// memory: workgroups are dispatched to the 8 XCDs round-robin, so block b
+// runs on XCD b % 8; this maps the blocks of each XCD to one contiguous
+// range of logical blocks (a bijection on [0, nblocks)), so that lines two
+// neighbouring blocks share stay in one XCD's L2.
+PXR_DEV i64 xcd_block(i64 b, i64 nblocks) {
+  const i64 q = nblocks / 8, r = nblocks % 8, x = b % 8, j = b / 8;
+  return x * q + (x < r ? x : r) + j;
+}
+
 }  // namespace pxr

@@ -199,59 +199,77 @@ static int get_table(const pxrhip_grid *g, TileTable **out) {
 // ---------------------------------------------------------------------------
 // rebin + sort kernels
 // ---------------------------------------------------------------------------
-struct RebinArgs {
-  double *x, *y, *z;
-  const int64_t *tile_off;  // src, ntiles+1
-  int ntiles;
+// Grid geometry of the re-binning
+struct RebinGeom {
   double xmin[3], xmax[3], len[3], dx[3], sdi[3];
   int nt[3], n0[3];
-  const double *sorg;
-  const int *n3;
-  const int64_t *bin_base;
+  const double *sorg;       // sorting-grid origin of every tile
+  const int *n3;            // sorting cells of every tile
+  const int64_t *bin_base;  // first bin of every tile
+};
+
+// periodic wrap of coordinate d into [xmin, xmax) (particle_bcs_mpi_blocking,
+// periodic case)
+PXR_DEV double wrap_coord(const RebinGeom &r, int d, double v) {
+  double w = v;
+  if (v < r.xmin[d]) w = v + r.len[d];
+  if (v >= r.xmax[d]) w = v - r.len[d];
+  return w;
+}
+
+// Sorting bin of the particle at p: wrapped position, tile index
+// (particle_boundaries.F90:397) and its cell in that tile's sorting grid
+// (sorting.F90:294-304).  ok is false (cell 0 of the tile) if the cell lies
+// outside that grid.
+PXR_DEV int rebin_bin(const RebinGeom &r, const double *p, bool &ok) {
+  double pw[3];
+  int it[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    pw[d] = wrap_coord(r, d, p[d]);
+    const int t = (int)floor(((pw[d] - r.xmin[d]) + r.dx[d] / 2.0) / ((double)r.n0[d] * r.dx[d]));
+    it[d] = min(max(t, 0), r.nt[d] - 1);
+  }
+  const int t = it[0] + r.nt[0] * (it[1] + r.nt[1] * it[2]);
+  int c[3];
+  ok = true;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const double f = floor((pw[d] - r.sorg[3 * t + d]) * r.sdi[d]);
+    const int n3 = r.n3[3 * t + d];
+    if (!(f >= 0.0) || f >= (double)n3) ok = false;  // also catches NaN
+    c[d] = ok ? (int)f : 0;
+  }
+  const int cell = c[2] * r.n3[3 * t] * r.n3[3 * t + 1] + c[1] * r.n3[3 * t] + c[0];
+  return (int)(r.bin_base[t] + cell);
+}
+
+struct RebinArgs {
+  const double *x, *y, *z;
+  const int64_t *tile_off;  // src, ntiles+1
+  int ntiles;
+  RebinGeom rg;
   int *key;       // global bin of every particle
   int *hist;      // nbins counters
   unsigned long long *bad;
 };
 
-// periodic wrap (particle_bcs_mpi_blocking, periodic case), tile index
-// (particle_boundaries.F90:397) and sorting key (sorting.F90:294-304)
+// Sorting bin of every particle, counted into the bin histogram.  The
+// particles come cell-sorted from the previous re-binning and few change
+// cell in a step, so equal bins sit on neighbouring lanes: one atomic per
+// run of them.
 __global__ __launch_bounds__(256) void k_rb_keys(RebinArgs a, i64 cap) {
-  const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  const i64 i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   const i64 n = a.tile_off[a.ntiles];
-  const bool act = i < n && i < cap;
-  int gbin = -1;
-  if (act) {
-    double p[3] = {a.x[i], a.y[i], a.z[i]};
-    int it[3];
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-      const double v = p[d];
-      if (v < a.xmin[d]) p[d] = v + a.len[d];
-      if (v >= a.xmax[d]) p[d] = v - a.len[d];
-      int t = (int)floor(((p[d] - a.xmin[d]) + a.dx[d] / 2.0) / ((double)a.n0[d] * a.dx[d]));
-      it[d] = min(max(t, 0), a.nt[d] - 1);
-    }
-    a.x[i] = p[0];
-    a.y[i] = p[1];
-    a.z[i] = p[2];
-    const int t = it[0] + a.nt[0] * (it[1] + a.nt[1] * it[2]);
-    int c[3];
-    bool ok = true;
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-      const double f = floor((p[d] - a.sorg[3 * t + d]) * a.sdi[d]);
-      const int n3 = a.n3[3 * t + d];
-      if (!(f >= 0.0) || f >= (double)n3) ok = false;  // also catches NaN
-      c[d] = ok ? (int)f : 0;
-    }
-    if (!ok) count_bad(a.bad);
-    const int cell = c[2] * a.n3[3 * t] * a.n3[3 * t + 1] + c[1] * a.n3[3 * t] + c[0];
-    gbin = (int)(a.bin_base[t] + cell);
-    a.key[i] = gbin;
-  }
-  int rank, cnt, leader;
-  wave_peers(act, gbin, rank, cnt, leader);
-  if (act && rank == 0) atomicAdd(a.hist + gbin, cnt);
+  if (i >= n || i >= cap) return;
+  const double p[3] = {a.x[i], a.y[i], a.z[i]};
+  bool ok;
+  const int gbin = rebin_bin(a.rg, p, ok);
+  if (!ok) count_bad(a.bad);
+  a.key[i] = gbin;
+  int head, cnt;
+  wave_runs(gbin, head, cnt);
+  if (cnt) atomicAdd(a.hist + gbin, cnt);
 }
 
 // exclusive scan of n counts into 64-bit starts: three passes over blocks
@@ -347,26 +365,28 @@ struct ScatterArgs {
   const void *srcq[5];   // ux, uy, uz, gaminv, w: FP64, or FP32 when mixed
   void *dstq[5];
   int mixed;
+  RebinGeom rg;          // the wrap of the positions
   const int64_t *src_off;
   int ntiles;
   const int *key;
   unsigned long long *cursor;  // running start of every bin
 };
 
+// Counting-sort scatter: each run of equal bins on neighbouring lanes takes
+// its slots with one atomic; positions are wrapped as they are copied.
 __global__ __launch_bounds__(256) void k_rb_scatter(ScatterArgs a, i64 cap) {
-  const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  const i64 i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   const i64 n = a.src_off[a.ntiles];
-  const bool act = i < n && i < cap;
-  const int k = act ? a.key[i] : -1;
-  int rank, cnt, leader;
-  wave_peers(act, k, rank, cnt, leader);
+  if (i >= n || i >= cap) return;
+  const int k = a.key[i];
+  int head, cnt;
+  wave_runs(k, head, cnt);
   unsigned long long slot = 0;
-  if (act && rank == 0) slot = atomicAdd(a.cursor + k, (unsigned long long)cnt);
-  slot = __shfl(slot, leader);
-  if (!act) return;
-  const i64 d = (i64)slot + rank;
+  if (cnt) slot = atomicAdd(a.cursor + k, (unsigned long long)cnt);
+  slot = __shfl(slot, head);
+  const i64 d = (i64)slot + (__lane_id() - head);
 #pragma unroll
-  for (int c = 0; c < 3; c++) a.dst[c][d] = a.src[c][i];
+  for (int c = 0; c < 3; c++) a.dst[c][d] = wrap_coord(a.rg, c, a.src[c][i]);
   if (a.mixed) {
 #pragma unroll
     for (int c = 0; c < 5; c++) ((float *)a.dstq[c])[d] = ((const float *)a.srcq[c])[i];
@@ -626,6 +646,26 @@ static int check_species(const pxrhip_species *s, bool need_bins) {
   return PXRHIP_OK;
 }
 
+static RebinGeom make_rebin_geom(const pxrhip_grid *grid, const TileTable *tt) {
+  RebinGeom rg;
+  const double xm[3] = {grid->xmin, grid->ymin, grid->zmin}, d[3] = {grid->dx, grid->dy, grid->dz};
+  const double sd[3] = {grid->sorting_dx, grid->sorting_dy, grid->sorting_dz};
+  const int64_t nc[3] = {grid->nx, grid->ny, grid->nz};
+  for (int k = 0; k < 3; k++) {
+    rg.xmin[k] = xm[k];
+    rg.len[k] = (double)nc[k] * d[k];
+    rg.xmax[k] = xm[k] + rg.len[k];
+    rg.dx[k] = d[k];
+    rg.sdi[k] = 1.0 / sd[k];
+    rg.nt[k] = tt->nt[k];
+    rg.n0[k] = tt->n0[k];
+  }
+  rg.sorg = tt->sorg;
+  rg.n3 = tt->n3;
+  rg.bin_base = tt->bin_base;
+  return rg;
+}
+
 int pxrhip_tiles_rebin_sort(const pxrhip_grid *grid, pxrhip_species *src, pxrhip_species *dst) {
   PXR_INIT();
   TileTable *tt;
@@ -645,27 +685,14 @@ int pxrhip_tiles_rebin_sort(const pxrhip_grid *grid, pxrhip_species *src, pxrhip
   i64 *bsum = (i64 *)scratch(6, sizeof(i64) * (size_t)(nblk + 1));
   unsigned long long *cursor = (unsigned long long *)scratch(7, sizeof(i64) * (size_t)nbins);
   if (!key || !hist || !bsum || !cursor) return set_error(PXRHIP_ERR_HIP, "scratch allocation failed");
+  const RebinGeom rg = make_rebin_geom(grid, tt);
   RebinArgs a;
   a.x = src->x;
   a.y = src->y;
   a.z = src->z;
   a.tile_off = src->tile_off;
   a.ntiles = tt->ntiles;
-  const double xm[3] = {grid->xmin, grid->ymin, grid->zmin}, d[3] = {grid->dx, grid->dy, grid->dz};
-  const double sd[3] = {grid->sorting_dx, grid->sorting_dy, grid->sorting_dz};
-  const int64_t nc[3] = {grid->nx, grid->ny, grid->nz};
-  for (int k = 0; k < 3; k++) {
-    a.xmin[k] = xm[k];
-    a.len[k] = (double)nc[k] * d[k];
-    a.xmax[k] = xm[k] + a.len[k];
-    a.dx[k] = d[k];
-    a.sdi[k] = 1.0 / sd[k];
-    a.nt[k] = tt->nt[k];
-    a.n0[k] = tt->n0[k];
-  }
-  a.sorg = tt->sorg;
-  a.n3 = tt->n3;
-  a.bin_base = tt->bin_base;
+  a.rg = rg;
   a.key = key;
   a.hist = hist;
   a.bad = rt().d_bad;
@@ -688,6 +715,7 @@ int pxrhip_tiles_rebin_sort(const pxrhip_grid *grid, pxrhip_species *src, pxrhip
     sa.dstq[k] = DQ[k];
   }
   sa.mixed = src->precision == 1;
+  sa.rg = rg;
   sa.src_off = src->tile_off;
   sa.ntiles = tt->ntiles;
   sa.key = key;
