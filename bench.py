@@ -308,7 +308,8 @@ def main():
         _lib.call("pxrhip_set_tuning", k.encode(), int(v))
     r = run_gpu(args)
     for k in ("fused.lds_bytes", "fused.wg_per_cu", "fused.spread", "fused.chunk",
-              "fused.j_lds_bytes", "fused.jpad", "fused.margin_pct", "mixed.jacc",
+              "fused.j_lds_bytes", "fused.jpad", "fused.margin_pct", "fused.sstride",
+              "fused.xcd_queue", "mixed.jacc",
               "mixed.lds_bytes", "mixed.wg_per_cu"):
         import ctypes
         v = ctypes.c_int64(0)

@@ -95,10 +95,18 @@ struct Chunk {
   int pad;
 };
 
+constexpr int XQ_END = 16;
+
 struct FusedArgs {
   const Chunk *chunks;
   const int *nchunks;
-  int *queue;
+  // nq work queues (1, or 8: one per XCD): queue q hands out work items
+  // [q*n/nq, (q+1)*n/nq), first to the workgroups b with b % nq == q
+  // (workgroups are dispatched to the 8 XCDs round-robin, so block b runs on
+  // XCD b % 8), which then move on to the other queues in turn.  xq[q] is
+  // the next item of queue q, xq[XQ_END + q] its end.
+  int *xq;
+  int nq;
   double *x, *y, *z;
   double *ux, *uy, *uz, *gi;  // FP64 species
   const double *w;
@@ -384,9 +392,24 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void k_
   const int ghi[3] = {g.nx + g.gx, g.ny + g.gy, g.nz + g.gz};
   const double x0[3] = {a.xmin, a.ymin, a.zmin}, di[3] = {a.dxi, a.dyi, a.dzi};
   const int nch = *a.nchunks;
+  int hop = 0;  // thread 0: queues drained so far
   for (;;) {
     if (threadIdx.x == 0) {
-      s_chunk = atomicAdd(a.queue, 1);
+      // one attempt per pass: -1 sends the workgroup round again, to the
+      // next queue.  (Queue bounds are read from memory: computing them here
+      // makes hipcc fail on some variants.)
+      int c = nch;
+      if (hop < a.nq) {
+        const int q = (int)((blockIdx.x + hop) % a.nq);
+        const int t = atomicAdd(a.xq + q, 1);
+        if (t < a.xq[XQ_END + q]) {
+          c = t;
+        } else {
+          c = -1;
+          hop++;
+        }
+      }
+      s_chunk = c;
       for (int d = 0; d < 3; d++) {
         s_box[d] = INT_MAX;
         s_box[3 + d] = INT_MIN;
@@ -395,6 +418,10 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void k_
     __syncthreads();
     const int c = s_chunk;
     if (c >= nch) break;
+    if (c < 0) {  // uniform
+      __syncthreads();  // s_chunk read by all before thread 0 writes it again
+      continue;
+    }
     const Chunk ch = a.chunks[c];
     const i64 p0 = ch.begin;
     // ---- 1. field block -------------------------------------------------

@@ -423,6 +423,8 @@ struct ChunkArgs {
   i64 *start;       // exclusive scan of count, nunits+1 entries
   Chunk *out;
   int *nchunks, *queue;
+  int *xq;          // work queues of the fused kernel (FusedArgs::xq) ...
+  int nq;           // ... and their number
   i64 maxchunks;
 };
 
@@ -514,6 +516,11 @@ __global__ __launch_bounds__(256) void k_chunk_write(ChunkArgs a) {
     *a.nchunks = (int)min(tot, a.maxchunks);
     *a.queue = 0;
     a.queue[1] = 0;  // overflow count (ctl[2])
+    const int n = (int)min(tot, a.maxchunks);
+    for (int q = 0; q < a.nq; q++) {
+      a.xq[q] = (int)((i64)n * q / a.nq);
+      a.xq[XQ_END + q] = (int)((i64)n * (q + 1) / a.nq);
+    }
   }
 }
 
@@ -534,6 +541,7 @@ struct Tuning {
   int jpad = 0;           // odd row/plane pitch of the LDS current block
   int sstride = 0;        // largest lane stride of the deposition (0: whole work item)
   int jacc = 2;           // mixed precision LDS current: 0 FP32, 1 compensated FP32, 2 FP64
+  int xcd_queue = 0;      // one work queue per XCD (measured: less HBM traffic, no faster)
   // launch of a mixed-precision species: its kernels take fewer registers,
   // so three workgroups with smaller LDS blocks fit on a CU
   int mx_lds_bytes = 52224;
@@ -579,6 +587,8 @@ int pxrhip_set_tuning(const char *key, int64_t value) {
   } else if (!strcmp(key, "fused.sstride")) {
     if (value < 0 || value > (1 << 20)) return set_error(PXRHIP_ERR_ARG, "fused.sstride must be in [0, 2^20]");
     g_tune.sstride = (int)value;
+  } else if (!strcmp(key, "fused.xcd_queue")) {
+    g_tune.xcd_queue = value != 0;
   } else if (!strcmp(key, "fused.jpad")) {
     g_tune.jpad = value != 0;
   } else if (!strcmp(key, "fused.margin_pct")) {
@@ -602,6 +612,7 @@ int pxrhip_get_tuning(const char *key, int64_t *value) {
   else if (!strcmp(key, "fused.j_lds_bytes")) *value = g_tune.j_lds_bytes;
   else if (!strcmp(key, "fused.margin_pct")) *value = g_tune.margin_pct;
   else if (!strcmp(key, "fused.jpad")) *value = g_tune.jpad;
+  else if (!strcmp(key, "fused.xcd_queue")) *value = g_tune.xcd_queue;
   else if (!strcmp(key, "fused.sstride")) *value = g_tune.sstride;
   else if (!strcmp(key, "mixed.jacc")) *value = g_tune.jacc;
   else if (!strcmp(key, "mixed.lds_bytes")) *value = g_tune.mx_lds_bytes;
@@ -787,7 +798,8 @@ int pxrhip_tiles_gather_push_deposit(const pxrhip_grid *grid, const pxrhip_speci
   i64 *cstart = (i64 *)scratch(12, sizeof(i64) * (size_t)(nunits + 1));
   i64 *bsum = (i64 *)scratch(13, sizeof(i64) * (size_t)(nblk + 1));
   Chunk *chunks = (Chunk *)scratch(9, sizeof(Chunk) * (size_t)maxchunks);
-  int *ctl = (int *)scratch(10, 4 * sizeof(int));
+  // 0 work items, 2 overflow count, 4.. work queues (FusedArgs::xq)
+  int *ctl = (int *)scratch(10, (4 + XQ_END + 8) * sizeof(int));
   int2 *ovf = (int2 *)scratch(14, sizeof(int2) * (size_t)maxchunks);
   if (!cnt || !cstart || !bsum || !chunks || !ctl || !ovf) return set_error(PXRHIP_ERR_HIP, "scratch allocation failed");
   ChunkArgs ca;
@@ -809,6 +821,8 @@ int pxrhip_tiles_gather_push_deposit(const pxrhip_grid *grid, const pxrhip_speci
   ca.out = chunks;
   ca.nchunks = ctl;
   ca.queue = ctl + 1;
+  ca.xq = ctl + 4;
+  ca.nq = g_tune.xcd_queue ? 8 : 1;
   ca.maxchunks = maxchunks;
   hipLaunchKernelGGL(k_chunk_count, dim3(grid1d(nunits, 256)), dim3(256), 0, st, ca);
   hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nblk), dim3(256), 0, st, cnt, nunits, bsum);
@@ -821,7 +835,8 @@ int pxrhip_tiles_gather_push_deposit(const pxrhip_grid *grid, const pxrhip_speci
   FusedArgs a;
   a.chunks = chunks;
   a.nchunks = ctl;
-  a.queue = ctl + 1;
+  a.xq = ctl + 4;
+  a.nq = ca.nq;
   a.x = sp->x; a.y = sp->y; a.z = sp->z;
   a.ux = a.uy = a.uz = a.gi = nullptr;
   a.w = nullptr;
