@@ -298,8 +298,6 @@ int pxrhip_tiles_rebin_sort(const pxrhip_grid *grid, pxrhip_species *src,
  * "fused.sstride" (largest lane stride of that order, 0: the whole work item),
  * "fused.chunk" (default particles per work item; a work item also holds no
  * more sorting-grid rows than its LDS blocks allow),
- * "fused.xcd_queue" (0/1, one work queue per XCD over a contiguous range of
- * work items, drained by that XCD's workgroups first),
  * "fused.j_lds_bytes" (largest current block accumulated in LDS; negative:
  * all of fused.lds_bytes), "fused.jpad" (0/1, odd row and plane pitches of
  * the LDS current block), "fused.margin_pct" (motion margin, in 1/100 cell,

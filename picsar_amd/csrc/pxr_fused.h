@@ -58,11 +58,12 @@ PXR_DEV void deposit_range(const BoxGeom &b, double ulo, double uhi, double m, i
 }
 
 // normalized coordinate range of the particles of tile t in sorting rows
-// y0..y1 of plane z (intersected with the tile's membership)
-PXR_DEV void rows_range(const BoxGeom &b, int t, int z, int y0, int y1, double *ulo, double *uhi) {
+// y0..y1 of planes z0..z1 (intersected with the tile's membership)
+PXR_DEV void rows_range(const BoxGeom &b, int t, int z0, int z1, int y0, int y1, double *ulo,
+                        double *uhi) {
   const double *so = b.sorg + 3 * t, *tb = b.tbox + 6 * t;
-  const int c0[3] = {0, y0, z};
-  const int c1[3] = {b.n3[3 * t] - 1, y1, z};
+  const int c0[3] = {0, y0, z0};
+  const int c1[3] = {b.n3[3 * t] - 1, y1, z1};
   for (int d = 0; d < 3; d++) {
     const double plo = fmax(so[d] + b.sd[d] * c0[d], tb[d]);
     const double phi = fmin(so[d] + b.sd[d] * (c1[d] + 1), tb[3 + d]);
@@ -91,22 +92,14 @@ struct Chunk {
   long long begin;
   int n;
   int tile;
-  int z, y0, y1;  // sorting-grid plane and rows of the particles; z < 0: unknown
-  int pad;
+  int z, y0, y1;  // sorting-grid planes z..z1 and rows y0..y1 of the particles (several
+  int z1;         // planes: whole planes); z < 0: unknown
 };
-
-constexpr int XQ_END = 16;
 
 struct FusedArgs {
   const Chunk *chunks;
   const int *nchunks;
-  // nq work queues (1, or 8: one per XCD): queue q hands out work items
-  // [q*n/nq, (q+1)*n/nq), first to the workgroups b with b % nq == q
-  // (workgroups are dispatched to the 8 XCDs round-robin, so block b runs on
-  // XCD b % 8), which then move on to the other queues in turn.  xq[q] is
-  // the next item of queue q, xq[XQ_END + q] its end.
-  int *xq;
-  int nq;
+  int *queue;
   double *x, *y, *z;
   double *ux, *uy, *uz, *gi;  // FP64 species
   const double *w;
@@ -392,24 +385,9 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void k_
   const int ghi[3] = {g.nx + g.gx, g.ny + g.gy, g.nz + g.gz};
   const double x0[3] = {a.xmin, a.ymin, a.zmin}, di[3] = {a.dxi, a.dyi, a.dzi};
   const int nch = *a.nchunks;
-  int hop = 0;  // thread 0: queues drained so far
   for (;;) {
     if (threadIdx.x == 0) {
-      // one attempt per pass: -1 sends the workgroup round again, to the
-      // next queue.  (Queue bounds are read from memory: computing them here
-      // makes hipcc fail on some variants.)
-      int c = nch;
-      if (hop < a.nq) {
-        const int q = (int)((blockIdx.x + hop) % a.nq);
-        const int t = atomicAdd(a.xq + q, 1);
-        if (t < a.xq[XQ_END + q]) {
-          c = t;
-        } else {
-          c = -1;
-          hop++;
-        }
-      }
-      s_chunk = c;
+      s_chunk = atomicAdd(a.queue, 1);
       for (int d = 0; d < 3; d++) {
         s_box[d] = INT_MAX;
         s_box[3 + d] = INT_MIN;
@@ -418,16 +396,12 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void k_
     __syncthreads();
     const int c = s_chunk;
     if (c >= nch) break;
-    if (c < 0) {  // uniform
-      __syncthreads();  // s_chunk read by all before thread 0 writes it again
-      continue;
-    }
     const Chunk ch = a.chunks[c];
     const i64 p0 = ch.begin;
     // ---- 1. field block -------------------------------------------------
     double ulo[3], uhi[3];
     if (ch.z >= 0) {
-      rows_range(a.bg, ch.tile, ch.z, ch.y0, ch.y1, ulo, uhi);
+      rows_range(a.bg, ch.tile, ch.z, ch.z1, ch.y0, ch.y1, ulo, uhi);
     } else {
       // no sorting information: one pass over the positions, in whole cells
       Box6 b;

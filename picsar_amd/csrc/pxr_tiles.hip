@@ -420,33 +420,96 @@ struct ChunkArgs {
   const int *n3;
   const int64_t *bin_base;
   int *count;       // chunks per unit (tile, plane)
+  int *group;       // per unit: last plane of the plane group it heads, or GROUP_*
   i64 *start;       // exclusive scan of count, nunits+1 entries
   Chunk *out;
   int *nchunks, *queue;
-  int *xq;          // work queues of the fused kernel (FusedArgs::xq) ...
-  int nq;           // ... and their number
   i64 maxchunks;
 };
 
-// A unit is one z-plane of the sorting grid of one tile.  Its chunks are
+// plane roles in ChunkArgs::group (a value >= 0 is the last plane of the
+// group of whole planes that this plane heads)
+constexpr int GROUP_ROWS = -1;    // cut into runs of rows by walk_unit
+constexpr int GROUP_MEMBER = -2;  // inside a group headed by an earlier plane
+constexpr int GROUP_EMPTY = -3;   // no particles
+
+// whether the field and current blocks of the particles of tile t in rows
+// y0..y1 of planes z0..z1 fit the LDS budgets
+PXR_DEV bool blocks_fit(const ChunkArgs &a, int t, int z0, int z1, int y0, int y1) {
+  double ulo[3], uhi[3];
+  int FR0[3], fe[3], JR0[3], je[3];
+  rows_range(a.bg, t, z0, z1, y0, y1, ulo, uhi);
+  blocks_for(a.bg, ulo, uhi, FR0, fe, JR0, je);
+  // current block, with room for one node of motion on each side (the
+  // block of k_fused is the box of the pushed particles) and for its odd
+  // pitches
+  const int j0 = je[0] + 2, j1 = je[1] + 2, j2 = je[2] + 2;
+  const i64 jpz = ((i64)(j0 | 1) * j1) | 1;
+  return a.fnode_bytes * (i64)fe[0] * fe[1] * fe[2] <= a.lds_bytes &&
+         a.jnode_bytes * jpz * j2 <= a.j_bytes;
+}
+
+// Groups of consecutive whole planes of tile t that together hold at most
+// `ch` particles and whose blocks fit: sparse planes then share one work
+// item instead of paying a work item's fixed cost each.  A plane too large
+// to be taken whole is cut into runs of rows by walk_unit.
+__global__ __launch_bounds__(256) void k_chunk_groups(ChunkArgs a) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.ntiles) return;
+  int *grp = a.group + (i64)t * a.nzmax;
+  const int nx3 = a.n3[3 * t], ny3 = a.n3[3 * t + 1], nz3 = a.n3[3 * t + 2];
+  if (!a.bin_start) {
+    for (int z = 0; z < a.nzmax; z++) grp[z] = GROUP_ROWS;
+    return;
+  }
+  for (int z = nz3; z < a.nzmax; z++) grp[z] = GROUP_EMPTY;
+  const i64 plane = (i64)ny3 * nx3, base = a.bin_base[t];
+  auto count = [&](int z) { return a.bin_start[base + (z + 1) * plane] - a.bin_start[base + z * plane]; };
+  int z = 0;
+  while (z < nz3) {
+    const i64 n = count(z);
+    if (n <= 0) {
+      grp[z++] = GROUP_EMPTY;
+      continue;
+    }
+    if (n > a.ch || !blocks_fit(a, t, z, z, 0, ny3 - 1)) {
+      grp[z++] = GROUP_ROWS;
+      continue;
+    }
+    int z1 = z;
+    i64 cnt = n;
+    while (z1 + 1 < nz3) {
+      const i64 m = count(z1 + 1);
+      if (m <= 0 || cnt + m > a.ch || !blocks_fit(a, t, z, z1 + 1, 0, ny3 - 1)) break;
+      cnt += m;
+      z1++;
+    }
+    grp[z] = z1;
+    for (int zz = z + 1; zz <= z1; zz++) grp[zz] = GROUP_MEMBER;
+    z = z1 + 1;
+  }
+}
+
+// A unit is one z-plane of the sorting grid of one tile.  A plane that
+// heads a group of whole planes is one chunk.  Otherwise its chunks are
 // runs of whole rows (cells of one y-row) of at most `ch` particles whose
 // field and current blocks fit the LDS budget; rows denser than that are
-// cut.  A chunk therefore spans a compact block of
-// cells: the tile's width in x, a few rows in y, one plane in z.  Without
-// sorting information the tile is one unit cut into runs of ch particles.
+// cut.  Such a chunk spans a compact block of cells: the tile's width in x,
+// a few rows in y, one plane in z.  Without sorting information the tile is
+// one unit cut into runs of ch particles.
 template <bool WRITE>
 PXR_DEV int walk_unit(const ChunkArgs &a, int t, int z, i64 pos) {
   int nch = 0;
-  auto emit = [&](int64_t b, int64_t n, int zz, int y0, int y1) {
+  auto emit = [&](int64_t b, int64_t n, int zz, int zz1, int y0, int y1) {
     if (WRITE && pos + nch < a.maxchunks) {
       Chunk c;
       c.begin = b;
       c.n = (int)n;
       c.tile = t;
       c.z = zz;
+      c.z1 = zz1;
       c.y0 = y0;
       c.y1 = y1;
-      c.pad = 0;
       a.out[pos + nch] = c;
     }
     nch++;
@@ -455,34 +518,28 @@ PXR_DEV int walk_unit(const ChunkArgs &a, int t, int z, i64 pos) {
   if (np <= 0) return 0;
   if (!a.bin_start) {
     if (z != 0) return 0;
-    for (int64_t b = 0; b < np; b += a.ch) emit(off + b, min((int64_t)a.ch, np - b), -1, 0, 0);
+    for (int64_t b = 0; b < np; b += a.ch) emit(off + b, min((int64_t)a.ch, np - b), -1, -1, 0, 0);
     return nch;
   }
   const int nx3 = a.n3[3 * t], ny3 = a.n3[3 * t + 1], nz3 = a.n3[3 * t + 2];
   if (z >= nz3) return 0;
+  const int g = a.group[(i64)t * a.nzmax + z];
   const int64_t base = a.bin_base[t] + (int64_t)z * ny3 * nx3;
+  if (g >= 0) {  // whole planes z..g
+    const int64_t b0 = a.bin_start[base], b1 = a.bin_start[base + (int64_t)(g - z + 1) * ny3 * nx3];
+    emit(b0, b1 - b0, z, g, 0, ny3 - 1);
+    return nch;
+  }
+  if (g != GROUP_ROWS) return 0;
   int64_t cb = 0, cnt = 0;
   int y0 = 0, ylast = 0;  // first and last row of the open chunk
   for (int y = 0; y < ny3; y++) {
     const int64_t row = base + (int64_t)y * nx3;
     const int64_t rb = a.bin_start[row], rn = a.bin_start[row + nx3] - rb;
     if (rn <= 0) continue;
-    bool fits = true;
-    if (cnt > 0) {
-      double ulo[3], uhi[3];
-      int FR0[3], fe[3], JR0[3], je[3];
-      rows_range(a.bg, t, z, y0, y, ulo, uhi);
-      blocks_for(a.bg, ulo, uhi, FR0, fe, JR0, je);
-      // current block, with room for one node of motion on each side
-      // (the block of k_fused is the box of the pushed particles) and for
-      // its odd pitches
-      const int j0 = je[0] + 2, j1 = je[1] + 2, j2 = je[2] + 2;
-      const i64 jpz = ((i64)(j0 | 1) * j1) | 1;
-      fits = a.fnode_bytes * (i64)fe[0] * fe[1] * fe[2] <= a.lds_bytes &&
-             a.jnode_bytes * jpz * j2 <= a.j_bytes;
-    }
+    const bool fits = cnt == 0 || blocks_fit(a, t, z, z, y0, y);
     if (cnt > 0 && (cnt + rn > a.ch || !fits)) {
-      emit(cb, cnt, z, y0, ylast);
+      emit(cb, cnt, z, z, y0, ylast);
       cnt = 0;
     }
     if (cnt == 0) {
@@ -492,13 +549,13 @@ PXR_DEV int walk_unit(const ChunkArgs &a, int t, int z, i64 pos) {
     cnt += rn;
     ylast = y;
     while (cnt > a.ch) {  // a row denser than a chunk: cut it
-      emit(cb, a.ch, z, y0, y);
+      emit(cb, a.ch, z, z, y0, y);
       cb += a.ch;
       cnt -= a.ch;
       y0 = y;
     }
   }
-  if (cnt > 0) emit(cb, cnt, z, y0, ylast);
+  if (cnt > 0) emit(cb, cnt, z, z, y0, ylast);
   return nch;
 }
 
@@ -516,11 +573,6 @@ __global__ __launch_bounds__(256) void k_chunk_write(ChunkArgs a) {
     *a.nchunks = (int)min(tot, a.maxchunks);
     *a.queue = 0;
     a.queue[1] = 0;  // overflow count (ctl[2])
-    const int n = (int)min(tot, a.maxchunks);
-    for (int q = 0; q < a.nq; q++) {
-      a.xq[q] = (int)((i64)n * q / a.nq);
-      a.xq[XQ_END + q] = (int)((i64)n * (q + 1) / a.nq);
-    }
   }
 }
 
@@ -541,7 +593,6 @@ struct Tuning {
   int jpad = 0;           // odd row/plane pitch of the LDS current block
   int sstride = 0;        // largest lane stride of the deposition (0: whole work item)
   int jacc = 2;           // mixed precision LDS current: 0 FP32, 1 compensated FP32, 2 FP64
-  int xcd_queue = 0;      // one work queue per XCD (measured: less HBM traffic, no faster)
   // launch of a mixed-precision species: its kernels take fewer registers,
   // so three workgroups with smaller LDS blocks fit on a CU
   int mx_lds_bytes = 52224;
@@ -587,8 +638,6 @@ int pxrhip_set_tuning(const char *key, int64_t value) {
   } else if (!strcmp(key, "fused.sstride")) {
     if (value < 0 || value > (1 << 20)) return set_error(PXRHIP_ERR_ARG, "fused.sstride must be in [0, 2^20]");
     g_tune.sstride = (int)value;
-  } else if (!strcmp(key, "fused.xcd_queue")) {
-    g_tune.xcd_queue = value != 0;
   } else if (!strcmp(key, "fused.jpad")) {
     g_tune.jpad = value != 0;
   } else if (!strcmp(key, "fused.margin_pct")) {
@@ -612,7 +661,6 @@ int pxrhip_get_tuning(const char *key, int64_t *value) {
   else if (!strcmp(key, "fused.j_lds_bytes")) *value = g_tune.j_lds_bytes;
   else if (!strcmp(key, "fused.margin_pct")) *value = g_tune.margin_pct;
   else if (!strcmp(key, "fused.jpad")) *value = g_tune.jpad;
-  else if (!strcmp(key, "fused.xcd_queue")) *value = g_tune.xcd_queue;
   else if (!strcmp(key, "fused.sstride")) *value = g_tune.sstride;
   else if (!strcmp(key, "mixed.jacc")) *value = g_tune.jacc;
   else if (!strcmp(key, "mixed.lds_bytes")) *value = g_tune.mx_lds_bytes;
@@ -795,13 +843,13 @@ int pxrhip_tiles_gather_push_deposit(const pxrhip_grid *grid, const pxrhip_speci
   const i64 maxchunks = tt->nrows + (cap / ch + 1) + nunits + 16;
   const i64 nblk = (nunits + SCAN_B - 1) / SCAN_B;
   int *cnt = (int *)scratch(8, sizeof(int) * (size_t)nunits);
+  int *grp = (int *)scratch(15, sizeof(int) * (size_t)nunits);
   i64 *cstart = (i64 *)scratch(12, sizeof(i64) * (size_t)(nunits + 1));
   i64 *bsum = (i64 *)scratch(13, sizeof(i64) * (size_t)(nblk + 1));
   Chunk *chunks = (Chunk *)scratch(9, sizeof(Chunk) * (size_t)maxchunks);
-  // 0 work items, 2 overflow count, 4.. work queues (FusedArgs::xq)
-  int *ctl = (int *)scratch(10, (4 + XQ_END + 8) * sizeof(int));
+  int *ctl = (int *)scratch(10, 4 * sizeof(int));
   int2 *ovf = (int2 *)scratch(14, sizeof(int2) * (size_t)maxchunks);
-  if (!cnt || !cstart || !bsum || !chunks || !ctl || !ovf) return set_error(PXRHIP_ERR_HIP, "scratch allocation failed");
+  if (!cnt || !grp || !cstart || !bsum || !chunks || !ctl || !ovf) return set_error(PXRHIP_ERR_HIP, "scratch allocation failed");
   ChunkArgs ca;
   ca.ntiles = tt->ntiles;
   ca.ch = ch;
@@ -817,13 +865,13 @@ int pxrhip_tiles_gather_push_deposit(const pxrhip_grid *grid, const pxrhip_speci
   ca.n3 = tt->n3;
   ca.bin_base = tt->bin_base;
   ca.count = cnt;
+  ca.group = grp;
   ca.start = cstart;
   ca.out = chunks;
   ca.nchunks = ctl;
   ca.queue = ctl + 1;
-  ca.xq = ctl + 4;
-  ca.nq = g_tune.xcd_queue ? 8 : 1;
   ca.maxchunks = maxchunks;
+  hipLaunchKernelGGL(k_chunk_groups, dim3(grid1d(tt->ntiles, 64)), dim3(64), 0, st, ca);
   hipLaunchKernelGGL(k_chunk_count, dim3(grid1d(nunits, 256)), dim3(256), 0, st, ca);
   hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nblk), dim3(256), 0, st, cnt, nunits, bsum);
   hipLaunchKernelGGL(k_scan_i64_1wg, dim3(1), dim3(1024), 0, st, bsum, nblk, (i64 *)nullptr);
@@ -835,8 +883,7 @@ int pxrhip_tiles_gather_push_deposit(const pxrhip_grid *grid, const pxrhip_speci
   FusedArgs a;
   a.chunks = chunks;
   a.nchunks = ctl;
-  a.xq = ctl + 4;
-  a.nq = ca.nq;
+  a.queue = ctl + 1;
   a.x = sp->x; a.y = sp->y; a.z = sp->z;
   a.ux = a.uy = a.uz = a.gi = nullptr;
   a.w = nullptr;

@@ -180,7 +180,6 @@ def test_chunking_does_not_change_results(torch, orc, chunk):
     {"fused.spread": 0},
     {"fused.jpad": 0},
     {"fused.sstride": 5},             # lanes 5 particles apart in super-blocks of 320
-    {"fused.xcd_queue": 1},           # one work queue per XCD
 ])
 @pytest.mark.parametrize("depo", ["esirkepov", "direct"])
 def test_overflow_and_variants_match(torch, orc, tune, depo):
