@@ -61,9 +61,20 @@ def profiled_traffic(args):
     return None
 
 
+def laser_dims(args):
+    """Cells of the laser workload: 512 x 512 x 1024 at --n 512."""
+    return args.n, args.n, 2 * args.n
+
+
 def workload_name(args):
     prec = "FP64" if args.precision == "fp64" else "mixed precision"
     tiles = f"{(args.n + 1) // args.tiles}^3-node tiles"
+    if args.workload == "laser":
+        nx, ny, nz = laser_dims(args)
+        return (f"laser-driven underdense plasma {nx}x{ny}x{nz}, electrons ramped over 100 cells "
+                f"to 4 ppc (immobile ions neither move nor deposit and are not carried), "
+                f"Gaussian laser a0=2, lambda 32 cells, waist 64 cells, antenna at z=0, "
+                f"order-4 FDTD, order-3 gather, Vay, order-3 Esirkepov, 16^3-node tiles, {prec}")
     if args.workload == "bunch":
         return (f"Gaussian electron bunch (sigma 20 cells, peak 64 ppc, gamma 10) in a 1 ppc "
                 f"background, {args.n}^3 periodic, order-3 gather, Vay, order-3 Esirkepov, Yee, "
@@ -72,8 +83,54 @@ def workload_name(args):
             f"gather, Vay, order-3 Esirkepov, Yee, {tiles}, {prec}")
 
 
+def build_laser_workload(args):
+    """A laser-driven underdense plasma: order-4 FDTD; electrons at 4 ppc
+    with a weight that ramps linearly over 100 cells (from z = 16) to a
+    plateau at 0.1 critical density; a Gaussian laser (a0 = 2, lambda = 32
+    cells, waist 64 cells) launched by an antenna in the plane z = 0.  The
+    electrons are generated and uploaded slab by slab."""
+    from picsar_amd.laser import emax_from_a0
+    from picsar_amd.sim import Simulation
+    nx, ny, nz = laser_dims(args)
+    sim = Simulation(nx, ny, nz, 1.0, 1.0, 1.0, 0.0, 0.0, 0.0, (nx + 1) // 16, (ny + 1) // 16,
+                     (nz + 1) // 16, order=3, l_lower_order_in_v=True, pusher="vay",
+                     depo="esirkepov", norder=4, cfl=0.95, chunk=args.chunk,
+                     precision=args.precision)
+    lam, ppc, z0, ramp = 32.0, 4, 16, 100
+    k0 = 2 * np.pi / lam
+    n_plateau = 0.1 * k0 ** 2       # 0.1 n_c with omega^2 = n_c (m = q = eps0 = 1)
+    rng = np.random.default_rng(12345)
+    nper = nx * ny * ppc
+
+    def slabs(step=8):
+        for k0_ in range(z0, nz, step):
+            if (k0_ - z0) % (16 * step) == 0:
+                log(f"generating electrons: z planes {k0_} of {nz}")
+            k1 = min(k0_ + step, nz)
+            cells = np.arange(nx * ny * (k1 - k0_))
+            kk = k0_ + cells // (nx * ny)
+            jj = (cells // nx) % ny
+            ii = cells % nx
+            c = np.repeat(np.stack([ii, jj, kk]), ppc, axis=1).astype(np.float64)
+            m = c.shape[1]
+            x = c[0] + rng.random(m)
+            y = c[1] + rng.random(m)
+            z = c[2] + rng.random(m)
+            dens = n_plateau * np.clip((z - z0) / ramp, 0.0, 1.0)
+            u = [rng.standard_normal(m) * 0.001 for _ in range(3)]
+            yield dict(x=x, y=y, z=z, ux=u[0], uy=u[1], uz=u[2], w=dens / ppc)
+
+    sim.add_species_chunks("electrons", -1.0, 1.0, nper * (nz - z0), slabs())
+    sim.add_laser_antenna(spot=(nx / 2.0, ny / 2.0, 0.0), vector=(0, 0, 1), polvector1=(1, 0, 0),
+                          lambda_laser=lam, laser_w0=64.0, laser_ctau=2 * lam, t_peak=4 * lam,
+                          emax1=emax_from_a0(2.0, lam), charge=-1.0)
+    return sim
+
+
 def build_workload(args):
     from picsar_amd.sim import Simulation
+    if args.workload == "laser":
+        return build_laser_workload(args)
     n = args.n
     sim = Simulation(n, n, n, 1.0, 1.0, 1.0, 0.0, 0.0, 0.0, args.tiles, args.tiles, args.tiles,
                      order=3, l_lower_order_in_v=True, pusher="vay", depo="esirkepov", norder=2,
@@ -146,6 +203,7 @@ def run_gpu(args):
             _lib.call("pxrhip_timer_record", b + 1 + 2 * i)
             sim.gather_push_deposit_species(sp)
             _lib.call("pxrhip_timer_record", b + 2 + 2 * i)
+        sim.laser_push_deposit()
         _lib.call("pxrhip_timer_record", b + 2 * nsp + 1)
         sim.current_bcs()
         _lib.call("pxrhip_timer_record", b + 2 * nsp + 2)
@@ -296,9 +354,10 @@ def main():
     ap.add_argument("--ppc", type=int, default=16)
     ap.add_argument("--tiles", type=int, default=8, help="tiles per axis")
     ap.add_argument("--chunk", type=int, default=0)
-    ap.add_argument("--workload", choices=("thermal", "bunch"), default="thermal",
+    ap.add_argument("--workload", choices=("thermal", "bunch", "laser"), default="thermal",
                     help="thermal: periodic thermal plasma (electrons + ions); bunch: Gaussian "
-                         "gamma=10 electron bunch in a 1 ppc background (uneven tiles)")
+                         "gamma=10 electron bunch in a 1 ppc background (uneven tiles); laser: "
+                         "laser-driven underdense plasma on n x n x 2n cells (512: the large run)")
     ap.add_argument("--autotune", action="store_true",
                     help="pick the launch parameters with picsar_amd.autotune before timing")
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",

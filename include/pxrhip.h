@@ -260,6 +260,30 @@ typedef struct {
   int64_t precision;
 } pxrhip_species;
 
+/* Laser antenna particles: laserp_pusher_gaussian
+ * (laser_pusher_manager_3d.F90:195-271, profile gaussian_profile :368-415)
+ * and laserp_pusher_hanning (:281-356, hanning_profile :426-478), same
+ * argument order.  pid is the device (np x npidd) column-major pid array;
+ * columns 2 and 3 hold the antenna-frame coordinates.  polvector1,
+ * polvector2 (3 doubles each) and q_z (complex: re, im) are host arrays.
+ * u = charge * amp * 0.01 c / emax, gaminv = 1, x += dtt * u (the
+ * reference reads the module's dt, which its caller passes as dtt).
+ * lvect (a CPU blocking factor) is ignored.  Normalised units, c = 1. */
+int pxrhip_laserp_pusher_gaussian(
+    int64_t np, int64_t npidd, const double *pid, double *xp, double *yp,
+    double *zp, double *uxp, double *uyp, double *uzp, double *gaminv,
+    double dtt, int64_t lvect, double emax, double emax1, double emax2,
+    const double *polvector1, const double *polvector2, double k0_laser,
+    const double *q_z, double charge, double laser_tau, double real_time,
+    double t_peak, int64_t temporal_order, double polangle);
+int pxrhip_laserp_pusher_hanning(
+    int64_t np, int64_t npidd, const double *pid, double *xp, double *yp,
+    double *zp, double *uxp, double *uyp, double *uzp, double *gaminv,
+    double dtt, int64_t lvect, double emax, double emax1, double emax2,
+    const double *polvector1, const double *polvector2, double k0_laser,
+    const double *q_z, double charge, double real_time, double t_peak,
+    int64_t temporal_order, double polangle);
+
 /* Number of sorting-grid cells over all tiles (the length of bin_start
  * minus one) for this geometry. */
 int pxrhip_tiles_nbins(const pxrhip_grid *grid, int64_t *nbins);

@@ -12,6 +12,8 @@
 #   direct_current_deposition_3d.F90 -> depose_jxjyjz_scalar_{1_1_1,2_2_2,3_3_3}
 #   sorting.F90           -> pxr_particle_bin_sorting
 #   submain.F90           -> FD_weights
+#   laser_pusher_manager_3d.F90 -> gaussian_profile, hanning_profile (the
+#                            pushers around them read module state)
 # -DLIBRARY selects normalised units (clight = mu0 = eps0 = 1).
 set -euo pipefail
 REF=${PXR_REF:-/root/reference}
@@ -44,6 +46,8 @@ D="$SRC/particle_deposition/current_deposition/direct/direct_current_deposition_
 for o in 1_1_1 2_2_2 3_3_3; do extract_sub "$D" "depose_jxjyjz_scalar_$o" >> "$TMP/direct_scalar.F90"; done
 extract_sub "$SRC/housekeeping/sorting.F90" pxr_particle_bin_sorting > "$TMP/binsort.F90"
 extract_sub "$SRC/submain.F90" FD_weights > "$TMP/fdweights.F90"
+L="$SRC/particle_pushers/laser_pusher_manager_3d.F90"
+{ extract_sub "$L" gaussian_profile; extract_sub "$L" hanning_profile; } > "$TMP/laserprof.F90"
 
 cd "$TMP"
 $FC $FFLAGS -c pxr_consts.F90 -o pxr_consts.o
@@ -60,7 +64,7 @@ for f in field_gathering/energy_conserving/field_gathering_o1_3d.F90 \
   $FC $FFLAGS -c "$SRC/$f" -o "$o"
   OBJS="$OBJS $o"
 done
-for f in direct_scalar binsort fdweights; do
+for f in direct_scalar binsort fdweights laserprof; do
   $FC $FFLAGS -c "$f.F90" -o "$f.o"
   OBJS="$OBJS $f.o"
 done

@@ -825,3 +825,142 @@ void orc_current_bc_periodic(double *a, i64 nx, i64 ny, i64 nz, i64 gx, i64 gy,
       for (i64 k = 0; k <= gz; k++) a[gidx(&g, i, j, nz - gz + k)] += a[gidx(&g, i, j, -gz + k)];
     }
 }
+
+/* ---- laser antenna (laser_pusher_manager_3d.F90) ---------------------- */
+/* Complex numbers as (re, im) pairs, in the reference's evaluation order;
+ * the division is Smith's (1962) algorithm.                              */
+typedef struct { double re, im; } cplx;
+static inline cplx c_mul(cplx a, cplx b) {
+  cplx r = {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+  return r;
+}
+static inline cplx c_scale(cplx a, double s) { cplx r = {a.re * s, a.im * s}; return r; }
+static inline cplx c_sub(cplx a, cplx b) { cplx r = {a.re - b.re, a.im - b.im}; return r; }
+static inline cplx c_add(cplx a, cplx b) { cplx r = {a.re + b.re, a.im + b.im}; return r; }
+static inline cplx c_div(cplx a, cplx b) {
+  cplx r;
+  if (fabs(b.re) >= fabs(b.im)) {
+    const double t = b.im / b.re, den = b.re + b.im * t;
+    r.re = (a.re + a.im * t) / den;
+    r.im = (a.im - a.re * t) / den;
+  } else {
+    const double t = b.re / b.im, den = b.im + b.re * t;
+    r.re = (a.re * t + a.im) / den;
+    r.im = (a.im * t - a.re) / den;
+  }
+  return r;
+}
+static inline cplx c_exp(cplx a) {
+  const double e = exp(a.re);
+  cplx r = {e * cos(a.im), e * sin(a.im)};
+  return r;
+}
+/* x**n for an integer n >= 0 by binary powering (as compilers expand a
+ * real**integer power) */
+static inline double ipow(double x, i64 n) {
+  double r = 1.0;
+  while (n > 0) {
+    if (n & 1) r *= x;
+    x *= x;
+    n >>= 1;
+  }
+  return r;
+}
+
+static const double PXR_PI = 3.14159265358979323;
+
+/* gaussian_profile, laser_pusher_manager_3d.F90:368-415 (clight = 1) */
+static void gaussian_amp(double xx, double yy, double *amp, double emax1, double emax2,
+                         const double *pv1, const double *pv2, double k0, cplx qz,
+                         double tau, double t, double tpeak, i64 torder, double polangle) {
+  const cplx j = {0.0, 1.0};
+  const cplx jk = c_scale(j, k0);
+  const cplx a = c_scale(c_scale(jk, 1.0), t - tpeak);
+  const cplx b = c_div(c_scale(jk, xx * xx + yy * yy), c_scale(qz, 2.0));
+  cplx u1, u2;
+  if (torder == 0) {
+    u1 = c_sub(a, b);
+    u2 = c_sub(a, b);
+  } else {
+    const double env = ipow((t - tpeak) / tau, torder);
+    const cplx e = {env, 0.0};
+    u1 = c_sub(c_sub(a, b), e);
+    u2 = c_add(c_sub(c_sub(a, b), e), c_scale(j, polangle * 2.0 * PXR_PI));
+  }
+  u1 = c_scale(c_exp(u1), emax1);
+  u2 = c_scale(c_exp(u2), emax2);
+  for (int d = 0; d < 3; d++) amp[d] = c_add(c_scale(u1, pv1[d]), c_scale(u2, pv2[d])).im;
+}
+
+/* hanning_profile, laser_pusher_manager_3d.F90:426-478 (clight = 1) */
+static void hanning_amp(double xx, double yy, double *amp, double emax1, double emax2,
+                        const double *pv1, const double *pv2, double k0, cplx qz,
+                        double t, double tpeak, double polangle) {
+  if (!(t <= 2.0 * tpeak)) {
+    amp[0] = amp[1] = amp[2] = 0.0;
+    return;
+  }
+  const double idd = 1.0;
+  const cplx j = {0.0, 1.0};
+  const cplx jk = c_scale(j, k0);
+  const cplx x = c_div((cplx){xx * xx + yy * yy, 0.0}, c_scale(qz, 2.0));
+  const cplx arg = c_sub(x, (cplx){1.0 * (t - tpeak), 0.0});
+  cplx u1 = c_scale(c_mul(jk, arg), -1.0);
+  cplx u2 = c_add(c_scale(c_mul(jk, arg), -1.0),
+                  c_scale(c_scale(c_scale(j, polangle), 2.0), PXR_PI));
+  const double win = 0.5 - 0.5 * cos(t * 2.0 * PXR_PI / (2.0 * tpeak));
+  u1 = c_scale(c_scale(c_scale(c_exp(u1), idd), emax1), win);
+  u2 = c_scale(c_scale(c_scale(c_exp(u2), idd), emax2), win);
+  for (int d = 0; d < 3; d++) amp[d] = c_add(c_scale(u1, pv1[d]), c_scale(u2, pv2[d])).re;
+}
+
+/* The antenna's field amplitude at antenna-frame coordinates (xx, yy):
+ * window 0 = gaussian_profile, 1 = hanning_profile.  amp is (n, 3). */
+void orc_laser_profile(i64 window, i64 n, const double *xx, const double *yy, double *amp,
+                       double emax1, double emax2, const double *pv1, const double *pv2,
+                       double k0, double qz_re, double qz_im, double tau, double t,
+                       double tpeak, i64 torder, double polangle) {
+  const cplx qz = {qz_re, qz_im};
+  for (i64 i = 0; i < n; i++) {
+    if (window == 0)
+      gaussian_amp(xx[i], yy[i], amp + 3 * i, emax1, emax2, pv1, pv2, k0, qz, tau, t, tpeak,
+                   torder, polangle);
+    else
+      hanning_amp(xx[i], yy[i], amp + 3 * i, emax1, emax2, pv1, pv2, k0, qz, t, tpeak,
+                  polangle);
+  }
+}
+
+/* laserp_pusher_gaussian / laserp_pusher_hanning,
+ * laser_pusher_manager_3d.F90:195-271 / :281-356: the antenna-frame
+ * coordinates are pid columns 2 and 3 (pid is np x npid, column-major);
+ * u = charge * amp * (0.01 c / emax), gaminv = 1, x += dt * u.  (The
+ * reference pushes with the module's dt, which its one caller passes as
+ * dtt; dt here is that value.)                                           */
+void orc_laserp_pusher(i64 window, i64 np, i64 npid, const double *pid, double *xp,
+                       double *yp, double *zp, double *uxp, double *uyp, double *uzp,
+                       double *gaminv, double dt, double emax, double emax1, double emax2,
+                       const double *pv1, const double *pv2, double k0, double qz_re,
+                       double qz_im, double charge, double tau, double t, double tpeak,
+                       i64 torder, double polangle) {
+  const cplx qz = {qz_re, qz_im};
+  const double disp_max = 0.01 * 1.0;
+  const double coeff = disp_max / emax;
+  (void)npid;
+  for (i64 i = 0; i < np; i++) {
+    double amp[3];
+    const double xx = pid[np + i], yy = pid[2 * np + i];
+    if (window == 0)
+      gaussian_amp(xx, yy, amp, emax1, emax2, pv1, pv2, k0, qz, tau, t, tpeak, torder,
+                   polangle);
+    else
+      hanning_amp(xx, yy, amp, emax1, emax2, pv1, pv2, k0, qz, t, tpeak, polangle);
+    uxp[i] = charge * amp[0] * coeff;
+    uyp[i] = charge * amp[1] * coeff;
+    uzp[i] = charge * amp[2] * coeff;
+    gaminv[i] = 1.0;
+    xp[i] = xp[i] + dt * uxp[i];
+    yp[i] = yp[i] + dt * uyp[i];
+    zp[i] = zp[i] + dt * uzp[i];
+  }
+}

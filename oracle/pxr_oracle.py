@@ -91,6 +91,10 @@ class Oracle:
         L.orc_field_bc_periodic.argtypes = [P] + [I] * 6
         L.orc_current_bc_periodic.restype = None
         L.orc_current_bc_periodic.argtypes = [P] + [I] * 6
+        L.orc_laser_profile.restype = None
+        L.orc_laser_profile.argtypes = [I, I, P, P, P, F, F, P, P, F, F, F, F, F, F, I, F]
+        L.orc_laserp_pusher.restype = None
+        L.orc_laserp_pusher.argtypes = [I, I, I] + [P] * 8 + [F] * 4 + [P, P] + [F] * 7 + [I, F]
 
     # -- gather ------------------------------------------------------------
     def geteb3d_energy_conserving(self, xp, yp, zp, ex, ey, ez, bx, by, bz, xmin, ymin, zmin,
@@ -209,6 +213,34 @@ class Oracle:
     def current_bc_periodic(self, a, nx, ny, nz, gx, gy, gz):
         _chk_grid(a)
         self.lib.orc_current_bc_periodic(_ptr(a), nx, ny, nz, gx, gy, gz)
+
+    # -- laser antenna -------------------------------------------------------
+    def laser_profile(self, window, xx, yy, emax1, emax2, pv1, pv2, k0, q_z, tau, t, t_peak,
+                      temporal_order, polangle):
+        """(n, 3) amplitudes: window 0 gaussian_profile, 1 hanning_profile."""
+        _chk_part(xx, yy)
+        amp = np.zeros((len(xx), 3))
+        pv1 = np.ascontiguousarray(pv1, dtype=np.float64)
+        pv2 = np.ascontiguousarray(pv2, dtype=np.float64)
+        self.lib.orc_laser_profile(window, len(xx), _ptr(xx), _ptr(yy), _ptr(amp), emax1, emax2,
+                                   _ptr(pv1), _ptr(pv2), k0, complex(q_z).real,
+                                   complex(q_z).imag, tau, t, t_peak, temporal_order, polangle)
+        return amp
+
+    def laserp_pusher(self, window, pid, xp, yp, zp, uxp, uyp, uzp, gaminv, dt, emax, emax1,
+                      emax2, pv1, pv2, k0, q_z, charge, tau, t, t_peak, temporal_order,
+                      polangle):
+        """laserp_pusher_gaussian (window 0) / laserp_pusher_hanning (1);
+        pid is (np, npid) Fortran-ordered."""
+        _chk_part(xp, yp, zp, uxp, uyp, uzp, gaminv)
+        assert pid.flags.f_contiguous and pid.dtype == np.float64 and pid.shape[0] == len(xp)
+        pv1 = np.ascontiguousarray(pv1, dtype=np.float64)
+        pv2 = np.ascontiguousarray(pv2, dtype=np.float64)
+        self.lib.orc_laserp_pusher(window, len(xp), pid.shape[1], _ptr(pid),
+                                   *map(_ptr, (xp, yp, zp, uxp, uyp, uzp, gaminv)), dt, emax,
+                                   emax1, emax2, _ptr(pv1), _ptr(pv2), k0, complex(q_z).real,
+                                   complex(q_z).imag, charge, tau, t, t_peak, temporal_order,
+                                   polangle)
 
 
 # ---------------------------------------------------------------------------
@@ -371,6 +403,27 @@ class Reference:
         c = np.zeros(max(norder // 2, 1))
         self._f("fd_weights_")(_ptr(c), _ri(norder), _ri(1 if l_nodal else 0))
         return c[: norder // 2]
+
+    def laser_profile(self, window, xx, yy, emax1, emax2, pv1, pv2, k0, q_z, tau, t, t_peak,
+                      temporal_order, polangle, emax=0.0):
+        """gaussian_profile_ / hanning_profile_ point by point: (n, 3)."""
+        amp = np.zeros((len(xx), 3))
+        a = [_f64(0.0) for _ in range(3)]
+        pv1 = np.ascontiguousarray(pv1, dtype=np.float64)
+        pv2 = np.ascontiguousarray(pv2, dtype=np.float64)
+        qz = np.array([complex(q_z).real, complex(q_z).imag])
+        for i in range(len(xx)):
+            common = (_rf(xx[i]), _rf(yy[i]), C.byref(a[0]), C.byref(a[1]), C.byref(a[2]),
+                      _rf(emax), _rf(emax1), _rf(emax2), _ptr(pv1), _ptr(pv2), _rf(k0),
+                      _ptr(qz))
+            if window == 0:
+                self._f("gaussian_profile_")(*common, _rf(tau), _rf(t), _rf(t_peak),
+                                             _ri(temporal_order), _rf(polangle))
+            else:
+                self._f("hanning_profile_")(*common, _rf(t), _rf(t_peak), _ri(temporal_order),
+                                            _rf(polangle))
+            amp[i] = [a[0].value, a[1].value, a[2].value]
+        return amp
 
     def particle_bin_sorting(self, xp, yp, zp, ux, uy, uz, gam, pid, xmin2, ymin2, zmin2,
                              xmax2, ymax2, zmax2, dxf, dyf, dzf):

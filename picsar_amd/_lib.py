@@ -63,6 +63,10 @@ SIGNATURES = {
     "pxrhip_field_bc_periodic": [_P] + [_I] * 6,
     "pxrhip_current_bc_periodic": [_P] + [_I] * 6,
     "pxrhip_particle_bin_sorting": [_I] + [_P] * 8 + [_I] + [_F] * 9,
+    "pxrhip_laserp_pusher_gaussian": [_I, _I] + [_P] * 8 + [_F, _I] + [_F] * 3 + [_P, _P, _F, _P]
+                                     + [_F] * 4 + [_I, _F],
+    "pxrhip_laserp_pusher_hanning": [_I, _I] + [_P] * 8 + [_F, _I] + [_F] * 3 + [_P, _P, _F, _P]
+                                    + [_F] * 3 + [_I, _F],
     "pxrhip_tiles_nbins": [_P, _P],
     "pxrhip_tiles_check": [],
     "pxrhip_tiles_stats": [C.POINTER(C.c_int64), C.POINTER(C.c_int64)],

@@ -103,6 +103,109 @@ __global__ __launch_bounds__(256) void k_pushxyz(i64 np, double *xp, double *yp,
   zp[ip] = zp[ip] + uz[ip] * gi[ip] * dt;
 }
 
+// ---------------------------------------------------------------------------
+// laser antenna (laser_pusher_manager_3d.F90)
+// ---------------------------------------------------------------------------
+// Complex numbers as (re, im) pairs, evaluated in the reference's order of
+// operations; the division is Smith's (1962) algorithm.
+struct Cx {
+  double re, im;
+};
+PXR_DEV Cx cx_mul(Cx a, Cx b) { return Cx{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+PXR_DEV Cx cx_scale(Cx a, double s) { return Cx{a.re * s, a.im * s}; }
+PXR_DEV Cx cx_sub(Cx a, Cx b) { return Cx{a.re - b.re, a.im - b.im}; }
+PXR_DEV Cx cx_add(Cx a, Cx b) { return Cx{a.re + b.re, a.im + b.im}; }
+PXR_DEV Cx cx_div(Cx a, Cx b) {
+  if (fabs(b.re) >= fabs(b.im)) {
+    const double t = b.im / b.re, den = b.re + b.im * t;
+    return Cx{(a.re + a.im * t) / den, (a.im - a.re * t) / den};
+  }
+  const double t = b.re / b.im, den = b.im + b.re * t;
+  return Cx{(a.re * t + a.im) / den, (a.im * t - a.re) / den};
+}
+PXR_DEV Cx cx_exp(Cx a) {
+  const double e = exp(a.re);
+  double sn, cs;
+  sincos(a.im, &sn, &cs);
+  return Cx{e * cs, e * sn};
+}
+// x**n, n >= 0, by binary powering (how a real**integer power is expanded)
+PXR_DEV double ipow(double x, i64 n) {
+  double r = 1.0;
+  while (n > 0) {
+    if (n & 1) r *= x;
+    x *= x;
+    n >>= 1;
+  }
+  return r;
+}
+
+struct LaserArgs {
+  i64 np;
+  const double *xx, *yy;  // antenna-frame coordinates (pid columns 2 and 3)
+  double *x, *y, *z, *ux, *uy, *uz, *gi;
+  double dt, coeff, charge;
+  double emax1, emax2, pv1[3], pv2[3], k0;
+  Cx qz;
+  double tau, t, tpeak, polangle;
+  i64 torder;
+};
+
+constexpr double PXR_PI_F90 = 3.14159265358979323;  // constants module value of pi
+
+// WINDOW 0: laserp_pusher_gaussian + gaussian_profile (:195-271, :368-415);
+// WINDOW 1: laserp_pusher_hanning + hanning_profile (:281-356, :426-478).
+// clight = 1.  u = charge * amp * coeff, gaminv = 1, x += dt * u.
+template <int WINDOW>
+__global__ __launch_bounds__(256) void k_laser_push(LaserArgs a) {
+  const i64 ip = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= a.np) return;
+  const double xx = a.xx[ip], yy = a.yy[ip];
+  const Cx j = {0.0, 1.0};
+  const Cx jk = cx_scale(j, a.k0);
+  double amp[3];
+  if (WINDOW == 0) {
+    const Cx ph = cx_scale(cx_scale(jk, 1.0), a.t - a.tpeak);
+    const Cx b = cx_div(cx_scale(jk, xx * xx + yy * yy), cx_scale(a.qz, 2.0));
+    Cx u1, u2;
+    if (a.torder == 0) {
+      u1 = cx_sub(ph, b);
+      u2 = u1;
+    } else {
+      const Cx e = {ipow((a.t - a.tpeak) / a.tau, a.torder), 0.0};
+      u1 = cx_sub(cx_sub(ph, b), e);
+      u2 = cx_add(cx_sub(cx_sub(ph, b), e), cx_scale(j, a.polangle * 2.0 * PXR_PI_F90));
+    }
+    u1 = cx_scale(cx_exp(u1), a.emax1);
+    u2 = cx_scale(cx_exp(u2), a.emax2);
+    for (int d = 0; d < 3; d++) amp[d] = cx_add(cx_scale(u1, a.pv1[d]), cx_scale(u2, a.pv2[d])).im;
+  } else {
+    if (!(a.t <= 2.0 * a.tpeak)) {
+      amp[0] = amp[1] = amp[2] = 0.0;
+    } else {
+      const Cx x = cx_div(Cx{xx * xx + yy * yy, 0.0}, cx_scale(a.qz, 2.0));
+      const Cx arg = cx_sub(x, Cx{1.0 * (a.t - a.tpeak), 0.0});
+      Cx u1 = cx_scale(cx_mul(jk, arg), -1.0);
+      Cx u2 = cx_add(cx_scale(cx_mul(jk, arg), -1.0),
+                     cx_scale(cx_scale(cx_scale(j, a.polangle), 2.0), PXR_PI_F90));
+      const double win = 0.5 - 0.5 * cos(a.t * 2.0 * PXR_PI_F90 / (2.0 * a.tpeak));
+      u1 = cx_scale(cx_scale(cx_scale(cx_exp(u1), 1.0), a.emax1), win);
+      u2 = cx_scale(cx_scale(cx_scale(cx_exp(u2), 1.0), a.emax2), win);
+      for (int d = 0; d < 3; d++) amp[d] = cx_add(cx_scale(u1, a.pv1[d]), cx_scale(u2, a.pv2[d])).re;
+    }
+  }
+  const double ux = a.charge * amp[0] * a.coeff;
+  const double uy = a.charge * amp[1] * a.coeff;
+  const double uz = a.charge * amp[2] * a.coeff;
+  a.ux[ip] = ux;
+  a.uy[ip] = uy;
+  a.uz[ip] = uz;
+  a.gi[ip] = 1.0;
+  a.x[ip] = a.x[ip] + a.dt * ux;
+  a.y[ip] = a.y[ip] + a.dt * uy;
+  a.z[ip] = a.z[ip] + a.dt * uz;
+}
+
 __global__ __launch_bounds__(256) void k_set_gamma(i64 np, const double *ux, const double *uy,
                                                    const double *uz, double *gi) {
   const i64 ip = (i64)blockIdx.x * blockDim.x + threadIdx.x;
@@ -664,6 +767,77 @@ int pxrhip_pushxyz(int64_t np, double *xp, double *yp, double *zp, const double 
                      uxp, uyp, uzp, gaminv, dt);
   PXR_LAUNCHED();
   return PXRHIP_OK;
+}
+
+static int laser_push(int window, int64_t np, int64_t npidd, const double *pid, double *xp,
+                      double *yp, double *zp, double *uxp, double *uyp, double *uzp,
+                      double *gaminv, double dtt, double emax, double emax1, double emax2,
+                      const double *polvector1, const double *polvector2, double k0_laser,
+                      const double *q_z, double charge, double laser_tau, double real_time,
+                      double t_peak, int64_t temporal_order, double polangle) {
+  PXR_INIT();
+  if (np < 0) return set_error(PXRHIP_ERR_ARG, "np < 0");
+  if (np == 0) return PXRHIP_OK;
+  if (npidd < 3) return set_error(PXRHIP_ERR_ARG, "npidd must be >= 3 (pid columns 2 and 3 hold the antenna frame)");
+  if (!pid || !xp || !yp || !zp || !uxp || !uyp || !uzp || !gaminv)
+    return set_error(PXRHIP_ERR_ARG, "null particle array");
+  if (!polvector1 || !polvector2 || !q_z) return set_error(PXRHIP_ERR_ARG, "null laser parameter");
+  if (temporal_order < 0) return set_error(PXRHIP_ERR_ARG, "temporal_order < 0");
+  if (!(emax != 0.0)) return set_error(PXRHIP_ERR_ARG, "emax must be non-zero");
+  LaserArgs a;
+  a.np = np;
+  a.xx = pid + np;
+  a.yy = pid + 2 * np;
+  a.x = xp; a.y = yp; a.z = zp;
+  a.ux = uxp; a.uy = uyp; a.uz = uzp; a.gi = gaminv;
+  a.dt = dtt;
+  a.coeff = (0.01 * 1.0) / emax;  // disp_max / emax, disp_max = 0.01 c
+  a.charge = charge;
+  a.emax1 = emax1;
+  a.emax2 = emax2;
+  for (int d = 0; d < 3; d++) {
+    a.pv1[d] = polvector1[d];
+    a.pv2[d] = polvector2[d];
+  }
+  a.k0 = k0_laser;
+  a.qz = Cx{q_z[0], q_z[1]};
+  a.tau = laser_tau;
+  a.t = real_time;
+  a.tpeak = t_peak;
+  a.polangle = polangle;
+  a.torder = temporal_order;
+  if (window == 0)
+    hipLaunchKernelGGL(k_laser_push<0>, dim3(grid1d(np, 256)), dim3(256), 0, rt().stream, a);
+  else
+    hipLaunchKernelGGL(k_laser_push<1>, dim3(grid1d(np, 256)), dim3(256), 0, rt().stream, a);
+  PXR_LAUNCHED();
+  return PXRHIP_OK;
+}
+
+int pxrhip_laserp_pusher_gaussian(int64_t np, int64_t npidd, const double *pid, double *xp,
+                                  double *yp, double *zp, double *uxp, double *uyp, double *uzp,
+                                  double *gaminv, double dtt, int64_t lvect, double emax,
+                                  double emax1, double emax2, const double *polvector1,
+                                  const double *polvector2, double k0_laser, const double *q_z,
+                                  double charge, double laser_tau, double real_time,
+                                  double t_peak, int64_t temporal_order, double polangle) {
+  (void)lvect;
+  return laser_push(0, np, npidd, pid, xp, yp, zp, uxp, uyp, uzp, gaminv, dtt, emax, emax1,
+                    emax2, polvector1, polvector2, k0_laser, q_z, charge, laser_tau, real_time,
+                    t_peak, temporal_order, polangle);
+}
+
+int pxrhip_laserp_pusher_hanning(int64_t np, int64_t npidd, const double *pid, double *xp,
+                                 double *yp, double *zp, double *uxp, double *uyp, double *uzp,
+                                 double *gaminv, double dtt, int64_t lvect, double emax,
+                                 double emax1, double emax2, const double *polvector1,
+                                 const double *polvector2, double k0_laser, const double *q_z,
+                                 double charge, double real_time, double t_peak,
+                                 int64_t temporal_order, double polangle) {
+  (void)lvect;
+  return laser_push(1, np, npidd, pid, xp, yp, zp, uxp, uyp, uzp, gaminv, dtt, emax, emax1,
+                    emax2, polvector1, polvector2, k0_laser, q_z, charge, 1.0, real_time, t_peak,
+                    temporal_order, polangle);
 }
 
 int pxrhip_set_gamma(int64_t np, const double *uxp, const double *uyp, const double *uzp,

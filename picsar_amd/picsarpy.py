@@ -220,6 +220,44 @@ class _Picsar:
         _lib.call("pxrhip_pushxyz", n, *X, *U, float(dt))
         c.finish()
 
+    # laser_pusher_manager_3d.F90:195 and :281
+    def _laser(self, entry, np_, npidd, pid, xp, yp, zp, uxp, uyp, uzp, gaminv, dtt, lvect,
+               emax, emax1, emax2, polvector1, polvector2, k0_laser, q_z, charge, tail):
+        n, npid = int(np_), int(npidd)
+        if isinstance(pid, np.ndarray) and pid.ndim == 2 and not pid.flags.f_contiguous:
+            raise ValueError("pid: expected a Fortran-ordered (np, npid) array")
+        c = _Call()
+        P = c.arr(pid, n * npid, "pid")
+        X = [c.arr(a, n, nm, write=True) for a, nm in
+             ((xp, "xp"), (yp, "yp"), (zp, "zp"), (uxp, "uxp"), (uyp, "uyp"), (uzp, "uzp"),
+              (gaminv, "gaminv"))]
+        (P,), X = c.ptrs([P]), c.ptrs(X)
+        pv1 = np.ascontiguousarray(polvector1, dtype=np.float64).reshape(3)
+        pv2 = np.ascontiguousarray(polvector2, dtype=np.float64).reshape(3)
+        qz = np.array([complex(q_z).real, complex(q_z).imag])
+        c.before()
+        _lib.call(entry, n, npid, P, *X, float(dtt), int(lvect), float(emax), float(emax1),
+                  float(emax2), pv1.ctypes.data, pv2.ctypes.data, float(k0_laser),
+                  qz.ctypes.data, float(charge), *tail)
+        c.finish()
+
+    def laserp_pusher_gaussian(self, np_, npidd, pid, xp, yp, zp, uxp, uyp, uzp, gaminv, dtt,
+                               lvect, emax, emax1, emax2, polvector1, polvector2, k0_laser, q_z,
+                               charge, laser_tau, real_time, t_peak, temporal_order, polangle):
+        self._laser("pxrhip_laserp_pusher_gaussian", np_, npidd, pid, xp, yp, zp, uxp, uyp,
+                    uzp, gaminv, dtt, lvect, emax, emax1, emax2, polvector1, polvector2,
+                    k0_laser, q_z, charge,
+                    (float(laser_tau), float(real_time), float(t_peak), int(temporal_order),
+                     float(polangle)))
+
+    def laserp_pusher_hanning(self, np_, npidd, pid, xp, yp, zp, uxp, uyp, uzp, gaminv, dtt,
+                              lvect, emax, emax1, emax2, polvector1, polvector2, k0_laser, q_z,
+                              charge, real_time, t_peak, temporal_order, polangle):
+        self._laser("pxrhip_laserp_pusher_hanning", np_, npidd, pid, xp, yp, zp, uxp, uyp,
+                    uzp, gaminv, dtt, lvect, emax, emax1, emax2, polvector1, polvector2,
+                    k0_laser, q_z, charge,
+                    (float(real_time), float(t_peak), int(temporal_order), float(polangle)))
+
     # boris_3d.F90:1059
     def pxr_set_gamma(self, np_, uxp, uyp, uzp, gaminv):
         n = int(np_)

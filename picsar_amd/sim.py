@@ -171,6 +171,7 @@ class Simulation:
         else:
             self.coefs = None
         self.species: List[Species] = []
+        self.antennas = []
 
     # ------------------------------------------------------------------
     # particles
@@ -179,20 +180,23 @@ class Simulation:
                     lfreeze=False, ldodepos=True) -> Species:
         """Add a species from host arrays.  gaminv defaults to
         1/sqrt(1+u^2) (pxr_set_gamma)."""
-        n = int(len(x))
+        src = {k: np.asarray(v, dtype=np.float64).ravel()
+               for k, v in zip(("x", "y", "z", "ux", "uy", "uz", "w"), (x, y, z, ux, uy, uz, w))}
+        if gaminv is not None:
+            src["gaminv"] = np.asarray(gaminv, dtype=np.float64).ravel()
+        return self.add_species_chunks(name, q, m, len(src["x"]), [src], capacity=capacity,
+                                       lfreeze=lfreeze, ldodepos=ldodepos)
+
+    def add_species_chunks(self, name, q, m, npart, chunks, capacity=None, lfreeze=False,
+                           ldodepos=True) -> Species:
+        """Add a species of ``npart`` particles given as an iterable of dicts
+        of host arrays (x, y, z, ux, uy, uz, w and optionally gaminv), each
+        holding the next run of particles; large species can so be built
+        slab by slab without holding every array in host memory at once."""
+        n = int(npart)
         cap = int(capacity) if capacity is not None else max(n, 1)
         if cap < n:
             raise ValueError("capacity < number of particles")
-        src = {k: np.ascontiguousarray(np.asarray(v, dtype=np.float64).ravel())
-               for k, v in zip(("x", "y", "z", "ux", "uy", "uz", "w"), (x, y, z, ux, uy, uz, w))}
-        if gaminv is None:
-            u = src
-            src["gaminv"] = 1.0 / np.sqrt(1.0 + u["ux"] ** 2 + u["uy"] ** 2 + u["uz"] ** 2)
-        else:
-            src["gaminv"] = np.ascontiguousarray(np.asarray(gaminv, dtype=np.float64).ravel())
-        for k, v in src.items():
-            if v.size != n:
-                raise ValueError(f"{k}: {v.size} values, expected {n}")
         sp = Species(name=name, q=float(q), m=float(m), capacity=cap, npart=n,
                      lfreeze=lfreeze, ldodepos=ldodepos, precision=self.precision)
         for _ in range(2):
@@ -205,8 +209,27 @@ class Simulation:
             b.bin_start.fill(0)
             sp.bufs.append(b)
         # stage into buffer 1 as one unsorted run, then bin into buffer 0
-        for k in COMPONENTS:
-            sp.bufs[1].arrays[k].upload(src[k])
+        done = 0
+        for ch in chunks:
+            src = {k: np.ascontiguousarray(np.asarray(ch[k], dtype=np.float64).ravel())
+                   for k in ("x", "y", "z", "ux", "uy", "uz", "w")}
+            if ch.get("gaminv") is None:
+                src["gaminv"] = 1.0 / np.sqrt(1.0 + src["ux"] ** 2 + src["uy"] ** 2 +
+                                              src["uz"] ** 2)
+            else:
+                src["gaminv"] = np.ascontiguousarray(np.asarray(ch["gaminv"],
+                                                                dtype=np.float64).ravel())
+            k0 = len(src["x"])
+            for k, v in src.items():
+                if v.size != k0:
+                    raise ValueError(f"{k}: {v.size} values, expected {k0}")
+            if done + k0 > n:
+                raise ValueError(f"the chunks hold more than {n} particles")
+            for k in COMPONENTS:
+                sp.bufs[1].arrays[k].upload(src[k], offset=done)
+            done += k0
+        if done != n:
+            raise ValueError(f"the chunks hold {done} particles, expected {n}")
         off = np.zeros(self.ntiles + 1, dtype=np.int64)
         off[-1] = n
         sp.bufs[1].tile_off.upload(off)
@@ -215,6 +238,21 @@ class Simulation:
         self._rebin(sp)
         self.check()
         return sp
+
+    def add_laser_antenna(self, **kw) -> "LaserAntenna":
+        """A laser antenna (picsar_amd.laser.LaserAntenna): its particles are
+        pushed by the laser profile and deposit current every step."""
+        from .laser import LaserAntenna
+        a = LaserAntenna(self, **kw)
+        self.antennas.append(a)
+        return a
+
+    def laser_push_deposit(self):
+        """push_laser_particles (laser_pusher_manager_3d.F90:15) and the
+        antennas' current deposition."""
+        for a in self.antennas:
+            a.push(self)
+            a.deposit(self)
 
     def _rebin(self, sp: Species):
         s_src = sp.struct(sp.cur)
@@ -237,6 +275,7 @@ class Simulation:
             self.fields[k].fill(0.0)
         for sp in self.species:
             self.gather_push_deposit_species(sp)
+        self.laser_push_deposit()
 
     def gather_push_deposit_species(self, sp: Species):
         """One launch of the fused kernel for one species (J accumulates)."""
@@ -340,6 +379,9 @@ class Simulation:
         nst = int(nst)
         if nst < 2 or nst % 2:
             raise ValueError("capture an even number of steps")
+        if self.antennas:
+            raise RuntimeError("steps with a laser antenna cannot be captured: its push "
+                               "depends on the time of each step")
         if self.it == 0:
             raise RuntimeError("run one step before capturing (scratch buffers are sized then)")
         curs = [sp.cur for sp in self.species]

@@ -184,11 +184,44 @@ def gen_sort(ref, rng):
     np.savez_compressed(os.path.join(HERE, "sort.npz"), **d)
 
 
+LASER_CASES = [
+    # window, temporal_order, polangle, time, q_z (complex beam parameter)
+    (0, 2, 0.0, 30.0, 1j * 64.0 ** 2 * np.pi / 32.0),
+    (0, 2, 0.125, 61.5, 1j * 64.0 ** 2 * np.pi / 32.0),
+    (0, 4, 0.3, 12.0, 15.0 + 1j * 400.0),
+    (0, 0, 0.0, 5.0, 1j * 100.0),
+    (1, 2, 0.0, 20.0, 1j * 64.0 ** 2 * np.pi / 32.0),
+    (1, 2, 0.25, 47.0, -8.0 + 1j * 250.0),
+    (1, 2, 0.0, 85.0, 1j * 100.0),  # after 2 t_peak: zero
+]
+
+
+def gen_laser(ref, rng):
+    """gaussian_profile and hanning_profile of the laser antenna
+    (laser_pusher_manager_3d.F90:368-478) at antenna-frame points, including
+    the axis, for several windows, time orders, polarisation angles and
+    complex beam parameters."""
+    n = 96
+    xx = rng.standard_normal(n) * 40.0
+    yy = rng.standard_normal(n) * 40.0
+    xx[:4] = [0.0, 1.0, -3.5, 0.0]
+    yy[:4] = [0.0, 0.0, 2.0, -64.0]
+    pv1 = np.array([0.6, 0.0, 0.8])
+    pv2 = np.array([0.0, 1.0, 0.0])
+    d = dict(xx=xx, yy=yy, pv1=pv1, pv2=pv2, emax1=2.0, emax2=0.7, k0=2 * np.pi / 32.0,
+             tau=25.0, t_peak=40.0)
+    for c, (win, tord, pol, t, qz) in enumerate(LASER_CASES):
+        d[f"case{c}"] = np.array([win, tord, pol, t, qz.real, qz.imag])
+        d[f"amp{c}"] = ref.laser_profile(win, xx, yy, d["emax1"], d["emax2"], pv1, pv2, d["k0"],
+                                         qz, d["tau"], t, d["t_peak"], tord, pol)
+    np.savez_compressed(os.path.join(HERE, "laser.npz"), **d)
+
+
 def main():
     ref = load_reference()
     if ref is None:
         sys.exit("oracle/_ref/libpxrref.so is missing: run bash oracle/build_ref.sh first")
-    which = set(sys.argv[1:]) or {"gather", "push", "deposit", "fdtd", "sort", "rho"}
+    which = set(sys.argv[1:]) or {"gather", "push", "deposit", "fdtd", "sort", "rho", "laser"}
     rng = np.random.default_rng(20241202)  # one stream for the first five families
     for name, fn in (("gather", gen_gather), ("push", gen_push), ("deposit", gen_deposit),
                      ("fdtd", gen_fdtd), ("sort", gen_sort)):
@@ -196,6 +229,8 @@ def main():
             fn(ref, rng)
     if "rho" in which:
         gen_rho(ref, np.random.default_rng(20241203))
+    if "laser" in which:
+        gen_laser(ref, np.random.default_rng(20241204))
     meta = dict(nx=NX, ny=NY, nz=NZ, ng=NG, dx=DX, dy=DY, dz=DZ, xmin=XMIN, ymin=YMIN, zmin=ZMIN)
     np.savez(os.path.join(HERE, "grid.npz"), **meta)
     for f in sorted(os.listdir(HERE)):

@@ -28,6 +28,15 @@ class OracleState:
         self.f = {k: np.zeros(shape, order="F") for k in
                   ("ex", "ey", "ez", "bx", "by", "bz", "jx", "jy", "jz")}
         self.species = []
+        self.antennas = []
+        self.it = 0
+
+    def add_antenna(self, a):
+        """a: dict of x, y, z, pid ((n, 3) Fortran order: weight, antenna-frame
+        x and y) and the laser parameters of orc.laserp_pusher."""
+        n = len(a["x"])
+        self.antennas.append(dict(a, ux=np.zeros(n), uy=np.zeros(n), uz=np.zeros(n),
+                                  gaminv=np.ones(n)))
 
     def add_species(self, q, m, x, y, z, ux, uy, uz, w):
         g = 1.0 / np.sqrt(1.0 + ux ** 2 + uy ** 2 + uz ** 2)
@@ -41,6 +50,7 @@ class OracleState:
         self.gather_push_deposit(zero_j=True)
         self.fold_currents_and_advance_fields()
         self.wrap()
+        self.it += 1
 
     def gather_push_deposit(self, zero_j=True):
         """Gather + push of every species, then deposition of the pushed
@@ -70,6 +80,19 @@ class OracleState:
             args = (f["jx"], f["jy"], f["jz"], s["x"], s["y"], s["z"], s["ux"], s["uy"], s["uz"],
                     s["gaminv"], s["w"], s["q"], *self.geom(), dt, *d, nx, ny, nz, ng, ng, ng,
                     self.order)
+            bad = o.depose_esirkepov(*args) if self.depo == "esirkepov" else \
+                o.depose_direct(*args, False)
+            assert bad == 0
+        # push_laser_particles at it*dt - dt/2 (laser_pusher_manager_3d.F90:37),
+        # then the antennas' current
+        for a in self.antennas:
+            o.laserp_pusher(a["window"], a["pid"], a["x"], a["y"], a["z"], a["ux"], a["uy"],
+                            a["uz"], a["gaminv"], dt, a["emax"], a["emax1"], a["emax2"], a["pv1"],
+                            a["pv2"], a["k0"], a["q_z"], a["charge"], a["tau"],
+                            self.it * dt - dt / 2.0, a["t_peak"], a["torder"], a["polangle"])
+            args = (f["jx"], f["jy"], f["jz"], a["x"], a["y"], a["z"], a["ux"], a["uy"], a["uz"],
+                    a["gaminv"], np.ascontiguousarray(a["pid"][:, 0]), a["charge"], *self.geom(),
+                    dt, *d, nx, ny, nz, ng, ng, ng, self.order)
             bad = o.depose_esirkepov(*args) if self.depo == "esirkepov" else \
                 o.depose_direct(*args, False)
             assert bad == 0

@@ -146,6 +146,46 @@ def test_pushers_golden_bitexact(dev, pxr, golden):
     np.testing.assert_array_equal(np.stack([H(t) for t in x]), d["pushxyz"])
 
 
+@pytest.mark.parametrize("case", range(7))
+def test_laser_pusher(dev, pxr, orc, golden, case):
+    """laserp_pusher_gaussian / laserp_pusher_hanning on the GPU: the momenta
+    are charge * amp * 0.01 / emax with amp from gaussian_profile /
+    hanning_profile (golden vectors of the reference's own routines) to a few
+    ulps, gaminv = 1 and x += dt u, against the oracle's pusher."""
+    d = golden("laser")
+    win, tord, pol, t, qr, qi = d[f"case{case}"]
+    xx, yy = d["xx"], d["yy"]
+    n = len(xx)
+    pid = np.asfortranarray(np.stack([np.full(n, 2.0), xx, yy], axis=1))
+    rng = np.random.default_rng(case)
+    x0 = [rng.random(n) * 8.0 for _ in range(3)]
+    emax, charge, dt = 2.4, -1.0, 0.55
+    args = (emax, float(d["emax1"]), float(d["emax2"]), d["pv1"], d["pv2"], float(d["k0"]),
+            complex(qr, qi), charge)
+    X = [T(dev, a) for a in x0]
+    U = [T(dev, np.zeros(n)) for _ in range(3)]
+    G = T(dev, np.zeros(n))
+    P = T(dev, pid.ravel(order="F"))
+    if win == 0:
+        pxr.laserp_pusher_gaussian(n, 3, P, *X, *U, G, dt, 8, *args, float(d["tau"]), float(t),
+                                   float(d["t_peak"]), int(tord), float(pol))
+    else:
+        pxr.laserp_pusher_hanning(n, 3, P, *X, *U, G, dt, 8, *args, float(t),
+                                  float(d["t_peak"]), int(tord), float(pol))
+    x = [a.copy() for a in x0]
+    u = [np.zeros(n) for _ in range(3)]
+    g = np.zeros(n)
+    orc.laserp_pusher(int(win), pid, *x, *u, g, dt, *args, float(d["tau"]), float(t),
+                      float(d["t_peak"]), int(tord), float(pol))
+    amp = d[f"amp{case}"]
+    scale = max(np.abs(amp).max() * 0.01 / emax, 1e-300)
+    for k in range(3):
+        assert np.abs(H(U[k]) - charge * amp[:, k] * (0.01 / emax)).max() <= 16e-16 * scale
+        assert np.abs(H(U[k]) - u[k]).max() <= 16e-16 * scale
+        assert np.abs(H(X[k]) - x[k]).max() <= 1e-15 * 8.0
+    np.testing.assert_array_equal(H(G), 1.0)
+
+
 def test_pushers_large_bitexact(dev, pxr, orc):
     rng = np.random.default_rng(7)
     n = 200000

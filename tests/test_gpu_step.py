@@ -162,6 +162,62 @@ def test_rebin_at_tile_boundaries(torch):
     np.testing.assert_array_equal(tile[o], it[0] + nt[0] * (it[1] + nt[1] * it[2]))
 
 
+def _antenna_oracle_dict(a):
+    """The oracle's copy of a GPU laser antenna (its initial state)."""
+    h = a.arrays()
+    return dict(x=h["x"], y=h["y"], z=h["z"],
+                pid=np.asfortranarray(np.stack([h["w"], h["xx"], h["yy"]], axis=1)),
+                window=a.time_window, emax=a.emax, emax1=a.emax1, emax2=a.emax2, pv1=a.pv1,
+                pv2=a.pv2, k0=a.k0, q_z=a.q_z, charge=a.charge, tau=a.laser_tau,
+                t_peak=a.t_peak, torder=a.temporal_order, polangle=a.polangle)
+
+
+@pytest.mark.parametrize("depo", ["esirkepov", "direct"])
+def test_laser_antennas_in_the_step(torch, orc, depo):
+    """A plasma with two laser antennas (a Gaussian one normal to z, a
+    Hanning-windowed one on a tilted plane): each step pushes the antenna
+    particles by the laser profile at it*dt - dt/2 and deposits their current
+    with the plasma's (submain.F90:113-135).  One step at 1e-12, three at
+    1e-9 against the oracle."""
+    sim, ref, tng = build(torch, orc, 3, True, "vay", depo, uth=(0.05, 0.005))
+    x0, y0, z0 = sim.xmin, sim.ymin, sim.zmin
+    ants = [sim.add_laser_antenna(spot=(x0 + 8.0, y0 + 6.1, z0 + 5.3), vector=(0, 0, 1),
+                                  polvector1=(1, 0, 0), lambda_laser=8.0, laser_w0=4.0,
+                                  laser_ctau=6.0, t_peak=1.5, emax1=0.05, emax2=0.02,
+                                  polangle=0.25, charge=1.0),
+            sim.add_laser_antenna(spot=(x0 + 7.0, y0 + 6.0, z0 + 6.0), vector=(0, 0.6, 0.8),
+                                  polvector1=(1, 0, 0), lambda_laser=6.0, laser_w0=3.0,
+                                  laser_ctau=5.0, t_peak=2.0, emax1=0.03, time_window=1,
+                                  charge=-1.0)]
+    for a in ants:
+        assert a.npart > 0
+        ref.add_antenna(_antenna_oracle_dict(a))
+    sim.step(1)
+    ref.step()
+    compare_fields(sim, ref, tng, 1e-12)
+    L = max(sim.nx * sim.dx, sim.ny * sim.dy, sim.nz * sim.dz)
+    compare_particles(sim, ref, 1e-12 * L, 1e-12)
+    for a, ra in zip(ants, ref.antennas):
+        h = a.arrays()
+        for k in ("x", "y", "z", "ux", "uy", "uz"):
+            assert np.abs(h[k] - ra[k]).max() <= 1e-14 * max(np.abs(ra[k]).max(), 1.0), k
+    assert np.linalg.norm(tng(sim.fields["jx"])) > 0
+    sim.step(2)
+    ref.step()
+    ref.step()
+    compare_fields(sim, ref, tng, 1e-9)
+
+
+def test_laser_antenna_refuses_graph_capture(torch, orc):
+    sim, ref, tng = build(torch, orc, 1, True, "boris", "esirkepov")
+    sim.add_laser_antenna(spot=(sim.xmin + 4.0, sim.ymin + 4.0, sim.zmin + 4.0),
+                          vector=(0, 0, 1), polvector1=(1, 0, 0), lambda_laser=8.0,
+                          laser_w0=4.0, laser_ctau=6.0, t_peak=2.0, emax1=0.05)
+    sim.step(1)
+    with pytest.raises(RuntimeError):
+        sim.capture_steps(2)
+
+
 @pytest.mark.parametrize("chunk", [37, 100000])
 def test_chunking_does_not_change_results(torch, orc, chunk):
     """Tiny chunks (runs cut inside rows, many chunks per tile) and one chunk

@@ -121,3 +121,18 @@ def test_rho(orc, golden, ggrid, order):
     assert bad == 0
     np.testing.assert_array_equal(rho, d[f"o{order}_n"])
     assert _rel(rho, d[f"o{order}_scalar"]) < 1e-15
+
+
+@pytest.mark.parametrize("case", range(7))
+def test_laser_profile(orc, golden, case):
+    """The oracle's antenna amplitudes are gaussian_profile / hanning_profile
+    (laser_pusher_manager_3d.F90:368-478) to the last few ulps (complex exp
+    and division may round differently)."""
+    d = golden("laser")
+    win, tord, pol, t, qr, qi = d[f"case{case}"]
+    amp = orc.laser_profile(int(win), d["xx"], d["yy"], float(d["emax1"]), float(d["emax2"]),
+                            d["pv1"], d["pv2"], float(d["k0"]), complex(qr, qi),
+                            float(d["tau"]), float(t), float(d["t_peak"]), int(tord), float(pol))
+    ref = d[f"amp{case}"]
+    scale = max(np.abs(ref).max(), 1e-300)
+    assert np.abs(amp - ref).max() <= 8 * np.finfo(float).eps * scale

@@ -206,3 +206,82 @@ def test_current_bc_periodic(orc):
     orc.current_bc_periodic(c, NX, NY, NZ, NG, NG, NG)
     assert c[NG + 0, NG + 2, NG + 2] == 11.0
     assert c[NG + NX, NG + 2, NG + 2] == 12.0
+
+
+LASER = dict(emax1=2.0, emax2=0.7, k0=2 * np.pi / 32.0, tau=25.0, t_peak=40.0)
+
+
+def _laser(orc, win, xx, yy, t, polangle=0.0, tord=2, w0=64.0, pv1=(1.0, 0, 0),
+           pv2=(0, 1.0, 0)):
+    qz = 1j * w0 ** 2 * np.pi / (2 * np.pi / LASER["k0"])
+    return orc.laser_profile(win, np.atleast_1d(np.asarray(xx, float)),
+                             np.atleast_1d(np.asarray(yy, float)), LASER["emax1"],
+                             LASER["emax2"], np.array(pv1), np.array(pv2), LASER["k0"], qz,
+                             LASER["tau"], t, LASER["t_peak"], tord, polangle)
+
+
+def test_laser_gaussian_closed_forms(orc):
+    """Gaussian antenna (laser_pusher_manager_3d.F90:368-415), q_z = i pi w0^2
+    / lambda: amp = Im[e^{i k0 (t - t_peak) - r^2/w0^2 - ((t - t_peak)/tau)^n}
+    (E1 p1 + E2 e^{2 pi i polangle} p2)]."""
+    k0, tau, tp = LASER["k0"], LASER["tau"], LASER["t_peak"]
+    # on axis at the peak: the phase is 0, only the polangle-shifted term is left
+    a = _laser(orc, 0, 0.0, 0.0, tp, polangle=0.25)[0]
+    np.testing.assert_allclose(a, [0.0, LASER["emax2"], 0.0], atol=1e-15)
+    # a quarter period later, off axis, fourth-order time envelope
+    dtq = np.pi / (2 * k0)
+    r2 = 10.0 ** 2 + 7.0 ** 2
+    a = _laser(orc, 0, 10.0, 7.0, tp + dtq, tord=4)[0]
+    env = np.exp(-r2 / 64.0 ** 2 - (dtq / tau) ** 4)
+    np.testing.assert_allclose(a, [LASER["emax1"] * env, LASER["emax2"] * env, 0.0],
+                               rtol=1e-14, atol=1e-15)
+    # temporal_order 0: no time envelope
+    a = _laser(orc, 0, 0.0, 0.0, tp + dtq + 500.0 * 2 * np.pi / k0, tord=0)[0]
+    np.testing.assert_allclose(a, [LASER["emax1"], LASER["emax2"], 0.0], rtol=1e-12)
+
+
+def test_laser_hanning_closed_forms(orc):
+    """Hanning antenna (laser_pusher_manager_3d.F90:426-478): amp =
+    Re[e^{i k0 (t - t_peak) - r^2/w0^2} (E1 p1 + E2 e^{2 pi i polangle} p2)]
+    (1 - cos(pi t / t_peak)) / 2 for t <= 2 t_peak, 0 after."""
+    tp = LASER["t_peak"]
+    a = _laser(orc, 1, 0.0, 0.0, tp, polangle=0.125)[0]
+    np.testing.assert_allclose(a, [LASER["emax1"], LASER["emax2"] * np.cos(np.pi / 4), 0.0],
+                               rtol=1e-15)
+    t = 10.0
+    r2 = 3.0 ** 2 + 5.0 ** 2
+    a = _laser(orc, 1, 3.0, -5.0, t)[0]
+    ph = LASER["k0"] * (t - tp)
+    win = 0.5 - 0.5 * np.cos(np.pi * t / tp)
+    np.testing.assert_allclose(a, np.array([LASER["emax1"], LASER["emax2"], 0.0]) *
+                               np.exp(-r2 / 64.0 ** 2) * np.cos(ph) * win, rtol=1e-13,
+                               atol=1e-15)
+    assert not _laser(orc, 1, 0.0, 0.0, 2 * tp + 1e-9).any()
+
+
+def test_laser_pusher_moves_by_the_profile(orc):
+    """laserp_pusher_* (laser_pusher_manager_3d.F90:195-356): u = charge *
+    amp * 0.01 c / emax from the profile at pid columns 2 and 3, gaminv = 1,
+    x += dt u."""
+    rng = np.random.default_rng(11)
+    n = 40
+    pid = np.asfortranarray(np.stack([np.full(n, 3.0), rng.standard_normal(n) * 30,
+                                      rng.standard_normal(n) * 30], axis=1))
+    x0 = [rng.random(n) * 10 for _ in range(3)]
+    pv1, pv2 = np.array([1.0, 0, 0]), np.array([0, 1.0, 0])
+    qz = 1j * 64.0 ** 2 * np.pi / 32.0
+    for win in (0, 1):
+        x = [a.copy() for a in x0]
+        u = [np.zeros(n) for _ in range(3)]
+        g = np.zeros(n)
+        emax = 2.5
+        orc.laserp_pusher(win, pid, *x, *u, g, 0.5, emax, LASER["emax1"], LASER["emax2"], pv1,
+                          pv2, LASER["k0"], qz, -1.0, LASER["tau"], 33.0, LASER["t_peak"], 2,
+                          0.0)
+        amp = orc.laser_profile(win, pid[:, 1].copy(), pid[:, 2].copy(), LASER["emax1"],
+                                LASER["emax2"], pv1, pv2, LASER["k0"], qz, LASER["tau"], 33.0,
+                                LASER["t_peak"], 2, 0.0)
+        for d in range(3):
+            np.testing.assert_array_equal(u[d], -1.0 * amp[:, d] * (0.01 / emax))
+            np.testing.assert_array_equal(x[d], x0[d] + 0.5 * u[d])
+        np.testing.assert_array_equal(g, 1.0)
