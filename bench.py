@@ -49,7 +49,8 @@ def profiled_traffic(args):
     (FETCH_SIZE and WRITE_SIZE passes scaled by a copy-kernel calibration),
     valid for the default workload and launch parameters only."""
     if (args.n, args.ppc, args.tiles, args.chunk) != (128, 16, 8, 0) or args.tune or \
-            args.precision != "fp64" or args.workload != "thermal" or args.autotune:
+            args.precision != "fp64" or args.workload != "thermal" or args.autotune or \
+            (args.order, args.pusher, args.depo) != (3, "vay", "esirkepov"):
         return None
     path = os.path.join(ROOT, "profiles", "fused_c2_hbm_traffic.md")
     try:
@@ -75,12 +76,13 @@ def workload_name(args):
                 f"to 4 ppc (immobile ions neither move nor deposit and are not carried), "
                 f"Gaussian laser a0=2, lambda 32 cells, waist 64 cells, antenna at z=0, "
                 f"order-4 FDTD, order-3 gather, Vay, order-3 Esirkepov, 16^3-node tiles, {prec}")
+    meth = (f"order-{args.order} gather, {args.pusher.capitalize()}, order-{args.order} "
+            f"{'Esirkepov' if args.depo == 'esirkepov' else 'direct'}")
     if args.workload == "bunch":
         return (f"Gaussian electron bunch (sigma 20 cells, peak 64 ppc, gamma 10) in a 1 ppc "
-                f"background, {args.n}^3 periodic, order-3 gather, Vay, order-3 Esirkepov, Yee, "
-                f"{tiles}, {prec}")
-    return (f"thermal plasma {args.n}^3 periodic, {args.ppc} ppc electrons + ions, order-3 "
-            f"gather, Vay, order-3 Esirkepov, Yee, {tiles}, {prec}")
+                f"background, {args.n}^3 periodic, {meth}, Yee, {tiles}, {prec}")
+    return (f"thermal plasma {args.n}^3 periodic, {args.ppc} ppc electrons + ions, {meth}, "
+            f"Yee, {tiles}, {prec}")
 
 
 def build_laser_workload(args):
@@ -133,7 +135,8 @@ def build_workload(args):
         return build_laser_workload(args)
     n = args.n
     sim = Simulation(n, n, n, 1.0, 1.0, 1.0, 0.0, 0.0, 0.0, args.tiles, args.tiles, args.tiles,
-                     order=3, l_lower_order_in_v=True, pusher="vay", depo="esirkepov", norder=2,
+                     order=args.order, l_lower_order_in_v=True, pusher=args.pusher,
+                     depo=args.depo, norder=2,
                      cfl=0.95, chunk=args.chunk, precision=args.precision)
     rng = np.random.default_rng(12345)
     if args.workload == "bunch":
@@ -358,6 +361,10 @@ def main():
                     help="thermal: periodic thermal plasma (electrons + ions); bunch: Gaussian "
                          "gamma=10 electron bunch in a 1 ppc background (uneven tiles); laser: "
                          "laser-driven underdense plasma on n x n x 2n cells (512: the large run)")
+    ap.add_argument("--order", type=int, choices=(1, 2, 3), default=3,
+                    help="shape order of the gather and the deposition (thermal and bunch)")
+    ap.add_argument("--pusher", choices=("vay", "boris"), default="vay")
+    ap.add_argument("--depo", choices=("esirkepov", "direct"), default="esirkepov")
     ap.add_argument("--autotune", action="store_true",
                     help="pick the launch parameters with picsar_amd.autotune before timing")
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",

@@ -87,9 +87,12 @@ int pxrhip_timer_elapsed(int start, int stop, double *ms);
  * compute_kin_energy (kin_energy.F90:33-143) for one species */
 int pxrhip_kinetic_energy(int64_t np, const double *gaminv, const double *w,
                           double *out);
-/* the same for a mixed-precision species (float32 gaminv and w, FP64 sum) */
-int pxrhip_kinetic_energy_f32(int64_t np, const float *gaminv, const float *w,
-                              double *out);
+/* the same for a mixed-precision species (float32 momenta and w): the sum
+ * of (gamma - 1)*w with gamma - 1 = u^2/(1 + sqrt(1 + u^2)) evaluated in
+ * FP64 from the momenta, since a float32 gaminv cannot resolve 1/gaminv - 1
+ * for a cold species (for u ~ 1e-3 it is a few float32 ulps). */
+int pxrhip_kinetic_energy_f32(int64_t np, const float *ux, const float *uy,
+                              const float *uz, const float *w, double *out);
 /* sum of f^2 over the nodes 0..n-1 of each axis; times dV/2 this is one
  * component of the field energy of diags.F90:1072-1114 */
 int pxrhip_field_energy(const double *f, int64_t nx, int64_t ny, int64_t nz,

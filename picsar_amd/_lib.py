@@ -47,7 +47,7 @@ SIGNATURES = {
     "pxrhip_graph_end": [C.POINTER(C.c_void_p)],
     "pxrhip_graph_launch": [_P],
     "pxrhip_graph_destroy": [_P],
-    "pxrhip_kinetic_energy_f32": [_I, _P, _P, C.POINTER(C.c_double)],
+    "pxrhip_kinetic_energy_f32": [_I, _P, _P, _P, _P, C.POINTER(C.c_double)],
     "pxrhip_field_energy": [_P] + [_I] * 6 + [C.POINTER(C.c_double)],
     "pxrhip_geteb3d_energy_conserving": [_I] + [_P] * 9 + [_F] * 6 + [_I] * 9 + [_P] * 6 + [_I] * 5,
     "pxrhip_boris_push_u_3d": [_I] + [_P] * 10 + [_F] * 3,

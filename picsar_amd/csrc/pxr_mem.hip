@@ -59,14 +59,23 @@ __global__ __launch_bounds__(256) void k_kin_partial(i64 np, const double *gi, c
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
-__global__ __launch_bounds__(256) void k_kin_partial_f32(i64 np, const float *gi, const float *w,
+// gamma - 1 = u^2 / (1 + sqrt(1 + u^2)) in FP64 from FP32 momenta: an FP32
+// gaminv is too coarse for the kinetic energy of a cold species (for
+// u ~ 1e-3, 1/gaminv - 1 ~ 5e-7 is a few FP32 ulps, and the rounding of an
+// FP32 pusher's 1 + u^2 biases the sum by percents)
+__global__ __launch_bounds__(256) void k_kin_partial_f32(i64 np, const float *ux, const float *uy,
+                                                         const float *uz, const float *w,
                                                          double *part) {
   __shared__ double s[256];
   const i64 b0 = (i64)blockIdx.x * RED_B;
   double acc = 0.0;
   for (int k = threadIdx.x; k < RED_B; k += 256) {
     const i64 i = b0 + k;
-    if (i < np) acc += (1.0 / (double)gi[i] - 1.0) * (double)w[i];
+    if (i < np) {
+      const double a = ux[i], b = uy[i], c = uz[i];
+      const double usq = a * a + b * b + c * c;
+      acc += usq / (1.0 + sqrt(1.0 + usq)) * (double)w[i];
+    }
   }
   const double t = block_sum(acc, s);
   if (threadIdx.x == 0) part[blockIdx.x] = t;
@@ -219,16 +228,18 @@ int pxrhip_kinetic_energy(int64_t np, const double *gaminv, const double *w, dou
   return reduce_finish(part, nblk, out);
 }
 
-int pxrhip_kinetic_energy_f32(int64_t np, const float *gaminv, const float *w, double *out) {
+int pxrhip_kinetic_energy_f32(int64_t np, const float *ux, const float *uy, const float *uz,
+                              const float *w, double *out) {
   PXR_INIT();
-  if (np < 0 || !out || (np && (!gaminv || !w))) return set_error(PXRHIP_ERR_ARG, "bad arguments");
+  if (np < 0 || !out || (np && (!ux || !uy || !uz || !w)))
+    return set_error(PXRHIP_ERR_ARG, "bad arguments");
   *out = 0.0;
   if (!np) return PXRHIP_OK;
   const i64 nblk = (np + RED_B - 1) / RED_B;
   double *part = (double *)scratch(11, sizeof(double) * (size_t)(nblk + 1));
   if (!part) return set_error(PXRHIP_ERR_HIP, "scratch allocation failed");
-  hipLaunchKernelGGL(k_kin_partial_f32, dim3((unsigned)nblk), dim3(256), 0, rt().stream, np, gaminv,
-                     w, part);
+  hipLaunchKernelGGL(k_kin_partial_f32, dim3((unsigned)nblk), dim3(256), 0, rt().stream, np, ux,
+                     uy, uz, w, part);
   PXR_LAUNCHED();
   return reduce_finish(part, nblk, out);
 }

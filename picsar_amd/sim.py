@@ -418,13 +418,20 @@ class Simulation:
     # diagnostics (diags.F90:847-1114, kin_energy.F90:33-143)
     # ------------------------------------------------------------------
     def kinetic_energy(self) -> float:
-        """Sum over species of m c^2 sum (1/gaminv - 1) w (kin_energy.F90:33-143)."""
+        """Sum over species of m c^2 sum (1/gaminv - 1) w (kin_energy.F90:33-143);
+        for a mixed-precision species gamma - 1 is taken from the momenta
+        in FP64 (pxrhip_kinetic_energy_f32)."""
         tot = 0.0
         for sp in self.species:
             out = C.c_double(0.0)
             b = sp.active
-            fn = "pxrhip_kinetic_energy_f32" if sp.precision == "mixed" else "pxrhip_kinetic_energy"
-            _lib.call(fn, sp.npart, b.arrays["gaminv"].ptr, b.arrays["w"].ptr, C.byref(out))
+            if sp.precision == "mixed":
+                _lib.call("pxrhip_kinetic_energy_f32", sp.npart, b.arrays["ux"].ptr,
+                          b.arrays["uy"].ptr, b.arrays["uz"].ptr, b.arrays["w"].ptr,
+                          C.byref(out))
+            else:
+                _lib.call("pxrhip_kinetic_energy", sp.npart, b.arrays["gaminv"].ptr,
+                          b.arrays["w"].ptr, C.byref(out))
             tot += out.value * sp.m
         return tot
 

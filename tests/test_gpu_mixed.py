@@ -145,3 +145,24 @@ def test_mixed_esirkepov_continuity(torch, orc, order):
     res = (r1 - r0) / sim.dt + div
     scale = np.abs(r1).max() / sim.dt
     assert np.abs(res[1:, 1:, 1:]).max() <= 1e-6 * scale, np.abs(res[1:, 1:, 1:]).max() / scale
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_kinetic_energy_of_a_cold_species(torch, orc, precision):
+    """compute_kin_energy (kin_energy.F90:33-143): m c^2 sum (gamma - 1) w.  A
+    mixed-precision species sums gamma - 1 = u^2 / (1 + sqrt(1 + u^2)) in FP64
+    from its FP32 momenta, which stays exact for ions at u_th = 0.001 c (from
+    an FP32 gaminv the sum would be off by percents)."""
+    kw = dict(precision="mixed", id_scale=2.0 ** -15, ppc=2) if precision == "mixed" else {}
+    sim, ref, _ = build(torch, orc, 3, True, "vay", "esirkepov", uth=(0.01, 0.001), **kw)
+    sim.step(2)
+    want = 0.0
+    for sp in sim.species:
+        a = sp.arrays()
+        usq = sum(a[k].astype(np.float64) ** 2 for k in ("ux", "uy", "uz"))
+        want += sp.m * (usq / (1.0 + np.sqrt(1.0 + usq)) * a["w"].astype(np.float64)).sum()
+    got = sim.kinetic_energy()
+    # FP64: the reference's 1/gaminv - 1 resolves gamma - 1 ~ 5e-7 to ~1e-16/5e-7;
+    # mixed: the same FP64 formula on the same momenta, summed in another order
+    tol = 1e-8 if precision == "fp64" else 1e-12
+    assert abs(got - want) <= tol * want, (got, want)
