@@ -245,7 +245,16 @@ typedef struct {
    * tile's lower edge (particle_sorting_sub, sorting.F90:150-160) */
   double sorting_dx, sorting_dy, sorting_dz;
   double sorting_shiftx, sorting_shifty, sorting_shiftz;
+  /* particle boundary at x_min, x_max, y_min, y_max, z_min, z_max
+   * (pbound_x_min ... of particle_bcs_mpi_blocking,
+   * particle_boundaries.F90:1106-1300): PXRHIP_PBOUND_* */
+  int64_t pbound[6];
 } pxrhip_grid;
+
+#define PXRHIP_PBOUND_PERIODIC 0
+#define PXRHIP_PBOUND_ABSORBING 1   /* the particle is removed */
+#define PXRHIP_PBOUND_REFLECTING 2  /* one cell back in, normal momentum reversed */
+#define PXRHIP_PBOUND_REINJECTING 3 /* mirrored in, at rest, kept in its tile */
 
 typedef struct {
   double *x, *y, *z, *ux, *uy, *uz, *gaminv, *w; /* SoA particle arrays */
@@ -306,14 +315,16 @@ int pxrhip_tiles_gather_push_deposit(
     int64_t order, int64_t l_lower_order_in_v, int64_t pusher,
     int64_t depo_algo, int64_t chunk);
 
-/* particle_bcs on one periodic rank (particle_boundaries.F90:83-426 and
- * the periodic case of particle_bcs_mpi_blocking, :1040-1300) followed by
- * particle_sorting_sub (sorting.F90:106-215): every particle is wrapped into
- * the periodic box, assigned to the tile that now holds it (tile index of
+/* particle_bcs on one rank (particle_boundaries.F90:83-426 and the domain
+ * boundaries of particle_bcs_mpi_blocking, :1040-1300, per grid->pbound;
+ * absorbed particles are dropped, so dst may hold fewer) followed by
+ * particle_sorting_sub (sorting.F90:106-215): every particle that left the
+ * box is wrapped, reflected, reinjected or removed (in src, in place), then
+ * assigned to the tile that now holds it (tile index of
  * particle_boundaries.F90:397) and the tiles are ordered by sorting-grid
- * cell (pxr_particle_bin_sorting's key).  src is read and left as it is;
- * the result, with wrapped positions, is written to dst, which must have
- * the same capacity; dst's tile_off, tile_np and bin_start are filled in.
+ * cell (pxr_particle_bin_sorting's key).  The result is written to dst,
+ * which must have the same capacity; dst's tile_off, tile_np and bin_start
+ * are filled in (tile_off[ntiles] is the number of particles kept).
  * Particles of one cell keep no particular order among themselves. */
 int pxrhip_tiles_rebin_sort(const pxrhip_grid *grid, pxrhip_species *src,
                             pxrhip_species *dst);

@@ -116,6 +116,10 @@ static int check_grid_desc(const pxrhip_grid *g) {
   if (!(g->dx > 0) || !(g->dy > 0) || !(g->dz > 0)) return set_error(PXRHIP_ERR_ARG, "dx, dy, dz must be > 0");
   if (!(g->sorting_dx > 0) || !(g->sorting_dy > 0) || !(g->sorting_dz > 0))
     return set_error(PXRHIP_ERR_ARG, "sorting cell sizes must be > 0");
+  for (int k = 0; k < 6; k++)
+    if (g->pbound[k] < PXRHIP_PBOUND_PERIODIC || g->pbound[k] > PXRHIP_PBOUND_REINJECTING)
+      return set_error(PXRHIP_ERR_ARG, "pbound[%d] = %lld is not a particle boundary (0..3)", k,
+                       (long long)g->pbound[k]);
   double tot = (double)(g->nx + 1 + 2 * g->ngx) * (double)(g->ny + 1 + 2 * g->ngy) *
                (double)(g->nz + 1 + 2 * g->ngz);
   if (tot > 4.0e9) return set_error(PXRHIP_ERR_ARG, "grid too large");
@@ -203,31 +207,66 @@ static int get_table(const pxrhip_grid *g, TileTable **out) {
 struct RebinGeom {
   double xmin[3], xmax[3], len[3], dx[3], sdi[3];
   int nt[3], n0[3];
+  int pbound[6];            // particle boundary at x_min, x_max, y_min, ... (PXRHIP_PBOUND_*)
   const double *sorg;       // sorting-grid origin of every tile
   const int *n3;            // sorting cells of every tile
   const int64_t *bin_base;  // first bin of every tile
+  const double *tbox;       // membership [tmin, tmax) of every tile
 };
 
-// periodic wrap of coordinate d into [xmin, xmax) (particle_bcs_mpi_blocking,
-// periodic case)
-PXR_DEV double wrap_coord(const RebinGeom &r, int d, double v) {
-  double w = v;
-  if (v < r.xmin[d]) w = v + r.len[d];
-  if (v >= r.xmax[d]) w = v - r.len[d];
-  return w;
+// Domain boundaries of one particle that left the box
+// (particle_bcs_mpi_blocking, particle_boundaries.F90:1106-1300), axis by
+// axis in x, y, z order as the reference examines them:
+//   periodic:    x +/- L;
+//   absorbing:   the particle is removed (returns false), no further axis;
+//   reflecting:  x +/- dx (one cell back in) and u_d = -u_d;
+//   reinjecting: x = 2 x_edge - x, u = 0, then every coordinate clamped to
+//                [tmin + dx/2, tmax - dx/2] of the tile the particle is in
+//                (old_tile()), and no further axis is examined.
+// Each side is tested on the coordinate as it was before that axis.
+template <typename OldTile>
+PXR_DEV bool domain_bcs(const RebinGeom &r, double *p, double *u, bool &uchanged,
+                        OldTile old_tile) {
+  for (int d = 0; d < 3; d++) {
+    const double v = p[d];
+    for (int side = 0; side < 2; side++) {
+      if (side == 0 ? !(v < r.xmin[d]) : !(v >= r.xmax[d])) continue;
+      switch (r.pbound[2 * d + side]) {
+        case PXRHIP_PBOUND_ABSORBING:
+          return false;
+        case PXRHIP_PBOUND_REFLECTING:
+          p[d] = side == 0 ? v + r.dx[d] : v - r.dx[d];
+          u[d] = -u[d];
+          uchanged = true;
+          break;
+        case PXRHIP_PBOUND_REINJECTING: {
+          p[d] = 2.0 * (side == 0 ? r.xmin[d] : r.xmax[d]) - p[d];
+          u[0] = u[1] = u[2] = 0.0;
+          uchanged = true;
+          const double *tb = r.tbox + 6 * old_tile();
+          for (int e = 0; e < 3; e++) {
+            p[e] = fmax(p[e], tb[e] + 0.5 * r.dx[e]);
+            p[e] = fmin(p[e], tb[3 + e] - 0.5 * r.dx[e]);
+          }
+          return true;
+        }
+        default:  // periodic
+          p[d] = side == 0 ? v + r.len[d] : v - r.len[d];
+      }
+    }
+  }
+  return true;
 }
 
-// Sorting bin of the particle at p: wrapped position, tile index
-// (particle_boundaries.F90:397) and its cell in that tile's sorting grid
-// (sorting.F90:294-304).  ok is false (cell 0 of the tile) if the cell lies
-// outside that grid.
+// Sorting bin of the particle at p (inside the box after domain_bcs): tile
+// index (particle_boundaries.F90:397) and its cell in that tile's sorting
+// grid (sorting.F90:294-304).  ok is false (cell 0 of the tile) if the cell
+// lies outside that grid.
 PXR_DEV int rebin_bin(const RebinGeom &r, const double *p, bool &ok) {
-  double pw[3];
   int it[3];
 #pragma unroll
   for (int d = 0; d < 3; d++) {
-    pw[d] = wrap_coord(r, d, p[d]);
-    const int t = (int)floor(((pw[d] - r.xmin[d]) + r.dx[d] / 2.0) / ((double)r.n0[d] * r.dx[d]));
+    const int t = (int)floor(((p[d] - r.xmin[d]) + r.dx[d] / 2.0) / ((double)r.n0[d] * r.dx[d]));
     it[d] = min(max(t, 0), r.nt[d] - 1);
   }
   const int t = it[0] + r.nt[0] * (it[1] + r.nt[1] * it[2]);
@@ -235,7 +274,7 @@ PXR_DEV int rebin_bin(const RebinGeom &r, const double *p, bool &ok) {
   ok = true;
 #pragma unroll
   for (int d = 0; d < 3; d++) {
-    const double f = floor((pw[d] - r.sorg[3 * t + d]) * r.sdi[d]);
+    const double f = floor((p[d] - r.sorg[3 * t + d]) * r.sdi[d]);
     const int n3 = r.n3[3 * t + d];
     if (!(f >= 0.0) || f >= (double)n3) ok = false;  // also catches NaN
     c[d] = ok ? (int)f : 0;
@@ -245,26 +284,66 @@ PXR_DEV int rebin_bin(const RebinGeom &r, const double *p, bool &ok) {
 }
 
 struct RebinArgs {
-  const double *x, *y, *z;
+  double *x, *y, *z;
+  void *ux, *uy, *uz;       // FP64, or FP32 when mixed
+  int mixed;
   const int64_t *tile_off;  // src, ntiles+1
   int ntiles;
   RebinGeom rg;
-  int *key;       // global bin of every particle
+  int *key;       // global bin of every particle (-1: absorbed)
   int *hist;      // nbins counters
   unsigned long long *bad;
 };
 
-// Sorting bin of every particle, counted into the bin histogram.  The
-// particles come cell-sorted from the previous re-binning and few change
-// cell in a step, so equal bins sit on neighbouring lanes: one atomic per
-// run of them.
+// Domain boundaries and sorting bin of every particle, counted into the bin
+// histogram.  A particle that left the box is updated in place (the rare
+// case; others are only read).  The particles come cell-sorted from the
+// previous re-binning and few change cell in a step, so equal bins sit on
+// neighbouring lanes: one atomic per run of them.
 __global__ __launch_bounds__(256) void k_rb_keys(RebinArgs a, i64 cap) {
   const i64 i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   const i64 n = a.tile_off[a.ntiles];
   if (i >= n || i >= cap) return;
-  const double p[3] = {a.x[i], a.y[i], a.z[i]};
+  double p[3] = {a.x[i], a.y[i], a.z[i]};
+  const RebinGeom &r = a.rg;
+  bool keep = true;
+  if (p[0] < r.xmin[0] || p[0] >= r.xmax[0] || p[1] < r.xmin[1] || p[1] >= r.xmax[1] ||
+      p[2] < r.xmin[2] || p[2] >= r.xmax[2] || p[0] != p[0] || p[1] != p[1] || p[2] != p[2]) {
+    double u[3];
+    if (a.mixed) {
+      u[0] = ((const float *)a.ux)[i]; u[1] = ((const float *)a.uy)[i]; u[2] = ((const float *)a.uz)[i];
+    } else {
+      u[0] = ((const double *)a.ux)[i]; u[1] = ((const double *)a.uy)[i]; u[2] = ((const double *)a.uz)[i];
+    }
+    bool uch = false;
+    auto old_tile = [&]() {  // the tile whose run holds particle i
+      int lo = 0, hi = a.ntiles;  // last t with tile_off[t] <= i
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (a.tile_off[mid] <= i) lo = mid; else hi = mid;
+      }
+      return lo;
+    };
+    keep = domain_bcs(r, p, u, uch, old_tile);
+    if (keep) {
+      a.x[i] = p[0];
+      a.y[i] = p[1];
+      a.z[i] = p[2];
+      if (uch) {
+        if (a.mixed) {
+          ((float *)a.ux)[i] = (float)u[0]; ((float *)a.uy)[i] = (float)u[1]; ((float *)a.uz)[i] = (float)u[2];
+        } else {
+          ((double *)a.ux)[i] = u[0]; ((double *)a.uy)[i] = u[1]; ((double *)a.uz)[i] = u[2];
+        }
+      }
+    }
+  }
+  if (!keep) {
+    a.key[i] = -1;
+    return;
+  }
   bool ok;
-  const int gbin = rebin_bin(a.rg, p, ok);
+  const int gbin = rebin_bin(r, p, ok);
   if (!ok) count_bad(a.bad);
   a.key[i] = gbin;
   int head, cnt;
@@ -365,7 +444,6 @@ struct ScatterArgs {
   const void *srcq[5];   // ux, uy, uz, gaminv, w: FP64, or FP32 when mixed
   void *dstq[5];
   int mixed;
-  RebinGeom rg;          // the wrap of the positions
   const int64_t *src_off;
   int ntiles;
   const int *key;
@@ -373,12 +451,13 @@ struct ScatterArgs {
 };
 
 // Counting-sort scatter: each run of equal bins on neighbouring lanes takes
-// its slots with one atomic; positions are wrapped as they are copied.
+// its slots with one atomic; absorbed particles (key -1) are dropped.
 __global__ __launch_bounds__(256) void k_rb_scatter(ScatterArgs a, i64 cap) {
   const i64 i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   const i64 n = a.src_off[a.ntiles];
   if (i >= n || i >= cap) return;
   const int k = a.key[i];
+  if (k < 0) return;
   int head, cnt;
   wave_runs(k, head, cnt);
   unsigned long long slot = 0;
@@ -386,7 +465,7 @@ __global__ __launch_bounds__(256) void k_rb_scatter(ScatterArgs a, i64 cap) {
   slot = __shfl(slot, head);
   const i64 d = (i64)slot + (__lane_id() - head);
 #pragma unroll
-  for (int c = 0; c < 3; c++) a.dst[c][d] = wrap_coord(a.rg, c, a.src[c][i]);
+  for (int c = 0; c < 3; c++) a.dst[c][d] = a.src[c][i];
   if (a.mixed) {
 #pragma unroll
     for (int c = 0; c < 5; c++) ((float *)a.dstq[c])[d] = ((const float *)a.srcq[c])[i];
@@ -719,9 +798,11 @@ static RebinGeom make_rebin_geom(const pxrhip_grid *grid, const TileTable *tt) {
     rg.nt[k] = tt->nt[k];
     rg.n0[k] = tt->n0[k];
   }
+  for (int k = 0; k < 6; k++) rg.pbound[k] = (int)grid->pbound[k];
   rg.sorg = tt->sorg;
   rg.n3 = tt->n3;
   rg.bin_base = tt->bin_base;
+  rg.tbox = tt->tbox;
   return rg;
 }
 
@@ -749,6 +830,10 @@ int pxrhip_tiles_rebin_sort(const pxrhip_grid *grid, pxrhip_species *src, pxrhip
   a.x = src->x;
   a.y = src->y;
   a.z = src->z;
+  a.ux = src->ux;
+  a.uy = src->uy;
+  a.uz = src->uz;
+  a.mixed = src->precision == 1;
   a.tile_off = src->tile_off;
   a.ntiles = tt->ntiles;
   a.rg = rg;
@@ -774,7 +859,6 @@ int pxrhip_tiles_rebin_sort(const pxrhip_grid *grid, pxrhip_species *src, pxrhip
     sa.dstq[k] = DQ[k];
   }
   sa.mixed = src->precision == 1;
-  sa.rg = rg;
   sa.src_off = src->tile_off;
   sa.ntiles = tt->ntiles;
   sa.key = key;

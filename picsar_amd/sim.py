@@ -45,7 +45,8 @@ class PxrGrid(C.Structure):
                [(n, C.c_double) for n in ("xmin", "ymin", "zmin", "dx", "dy", "dz",
                                           "sorting_dx", "sorting_dy", "sorting_dz",
                                           "sorting_shiftx", "sorting_shifty",
-                                          "sorting_shiftz")]
+                                          "sorting_shiftz")] + \
+               [("pbound", C.c_int64 * 6)]
 
 
 class PxrSpecies(C.Structure):
@@ -56,6 +57,9 @@ class PxrSpecies(C.Structure):
 
 
 PUSHERS = {"boris": 0, "vay": 1}
+# particle_bcs_mpi_blocking's pbound values (particle_boundaries.F90:1106-1300)
+PBOUND_CODES = {"periodic": 0, "absorbing": 1, "reflecting": 2, "reinjecting": 3}
+PBOUNDS_DEFAULT = ("periodic",) * 6
 DEPOS = {"esirkepov": 0, "direct": 3}
 COMPONENTS = ("x", "y", "z", "ux", "uy", "uz", "gaminv", "w")
 PRECISIONS = {"fp64": 0, "mixed": 1}
@@ -126,7 +130,7 @@ class Simulation:
     def __init__(self, nx, ny, nz, dx=1.0, dy=1.0, dz=1.0, xmin=0.0, ymin=0.0, zmin=0.0,
                  ntx=1, nty=1, ntz=1, order=1, l_lower_order_in_v=True, pusher="boris",
                  depo="esirkepov", norder=2, dt=None, cfl=0.95, nguard=None, sorting_dx=1.0,
-                 sorting_dy=1.0, sorting_dz=1.0, chunk=0, precision="fp64"):
+                 sorting_dy=1.0, sorting_dz=1.0, chunk=0, precision="fp64", pbound=None):
         _lib.call("pxrhip_init", -1)
         if precision not in PRECISIONS:
             raise ValueError("precision must be 'fp64' or 'mixed'")
@@ -159,6 +163,18 @@ class Simulation:
                             self.xmin, self.ymin, self.zmin, self.dx, self.dy, self.dz,
                             sorting_dx * self.dx, sorting_dy * self.dy, sorting_dz * self.dz,
                             0.0, 0.0, 0.0)
+        # particle boundaries at x_min, x_max, y_min, y_max, z_min, z_max:
+        # "periodic", "absorbing", "reflecting", "reinjecting" (or 0..3);
+        # the fields stay periodic
+        pb = PBOUNDS_DEFAULT if pbound is None else tuple(pbound)
+        if len(pb) != 6:
+            raise ValueError("pbound needs six entries (x_min, x_max, y_min, ..., z_max)")
+        codes = [PBOUND_CODES[b] if isinstance(b, str) else int(b) for b in pb]
+        for c in codes:
+            if c not in PBOUND_CODES.values():
+                raise ValueError(f"unknown particle boundary {c}")
+        self.grid.pbound[:] = codes
+        self.absorbing = PBOUND_CODES["absorbing"] in codes
         self.ntiles = int(ntx) * int(nty) * int(ntz)
         nb = C.c_int64(0)
         _lib.call("pxrhip_tiles_nbins", C.byref(self.grid), C.byref(nb))
@@ -259,6 +275,8 @@ class Simulation:
         s_dst = sp.struct(1 - sp.cur)
         _lib.call("pxrhip_tiles_rebin_sort", C.byref(self.grid), C.byref(s_src), C.byref(s_dst))
         sp.cur = 1 - sp.cur
+        if self.absorbing:  # particles may have been removed (this synchronises)
+            sp.npart = int(sp.active.tile_off.numpy()[-1])
 
     # ------------------------------------------------------------------
     def check(self):
@@ -382,6 +400,9 @@ class Simulation:
         if self.antennas:
             raise RuntimeError("steps with a laser antenna cannot be captured: its push "
                                "depends on the time of each step")
+        if self.absorbing:
+            raise RuntimeError("steps with absorbing particle boundaries cannot be captured: "
+                               "the particle counts change")
         if self.it == 0:
             raise RuntimeError("run one step before capturing (scratch buffers are sized then)")
         curs = [sp.cur for sp in self.species]

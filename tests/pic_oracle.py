@@ -30,6 +30,11 @@ class OracleState:
         self.species = []
         self.antennas = []
         self.it = 0
+        # particle boundaries (x_min, x_max, y_min, y_max, z_min, z_max):
+        # 0 periodic, 1 absorbing, 2 reflecting, 3 reinjecting; nt: tiles per
+        # axis (reinjection keeps a particle inside the tile it was in)
+        self.pbound = (0,) * 6
+        self.nt = (1, 1, 1)
 
     def add_antenna(self, a):
         """a: dict of x, y, z, pid ((n, 3) Fortran order: weight, antenna-frame
@@ -49,7 +54,7 @@ class OracleState:
     def step(self):
         self.gather_push_deposit(zero_j=True)
         self.fold_currents_and_advance_fields()
-        self.wrap()
+        self.particle_bcs()
         self.it += 1
 
     def gather_push_deposit(self, zero_j=True):
@@ -61,6 +66,7 @@ class OracleState:
         dt = self.dt
         for s in self.species:
             n = len(s["x"])
+            s["old"] = (s["x"].copy(), s["y"].copy(), s["z"].copy())  # for particle_bcs
             e = [np.zeros(n) for _ in range(6)]
             bad = o.geteb3d_energy_conserving(s["x"], s["y"], s["z"], *e, *self.geom(), *d, nx,
                                               ny, nz, ng, ng, ng, self.order, self.order,
@@ -109,18 +115,83 @@ class OracleState:
         self._bhalf()
         self._bcs(("bx", "by", "bz"))
 
-    def wrap(self):
-        """Periodic wrap of the positions (particle_bcs)."""
-        nx, ny, nz = self.nx, self.ny, self.nz
-        L = (nx * self.dx, ny * self.dy, nz * self.dz)
+    def tile_bounds(self, a):
+        """Membership [tmin, tmax) of every tile along axis a
+        (set_tile_split_for_species, tiling.F90:94-271)."""
+        n = (self.nx, self.ny, self.nz)[a]
+        d = (self.dx, self.dy, self.dz)[a]
+        x0 = self.geom()[a]
+        nt = self.nt[a]
+        n0 = (n + 1) // nt
+        out = []
+        for i in range(nt):
+            gmin = x0 + i * n0 * d
+            gmax = gmin + (n0 - 1) * d
+            lo = x0 if i == 0 else gmin - d / 2.0
+            hi = x0 + n * d if i == nt - 1 else gmax + d / 2.0
+            out.append((lo, hi))
+        return out, n0
+
+    def _tile_of(self, p):
+        """Tile (per axis) of an in-box position (particle_boundaries.F90:397)."""
+        it = []
+        for a in range(3):
+            d = (self.dx, self.dy, self.dz)[a]
+            _, n0 = self.tile_bounds(a)
+            t = int(np.floor(((p[a] - self.geom()[a]) + d / 2.0) / (n0 * d)))
+            it.append(min(max(t, 0), self.nt[a] - 1))
+        return it
+
+    def particle_bcs(self):
+        """Domain boundaries of the particles that left the box
+        (particle_bcs_mpi_blocking, particle_boundaries.F90:1106-1300), axis by
+        axis: periodic x +/- L; absorbing: removed; reflecting: x +/- dx and the
+        normal momentum reversed; reinjecting: x = 2 x_edge - x, u = 0, all
+        coordinates clamped to [tmin + d/2, tmax - d/2] of the particle's tile,
+        no further axis."""
+        L = (self.nx * self.dx, self.ny * self.dy, self.nz * self.dz)
+        D = (self.dx, self.dy, self.dz)
         x0 = self.geom()
+        bounds = [self.tile_bounds(a)[0] for a in range(3)]
         for s in self.species:
-            for a, k in enumerate(("x", "y", "z")):
-                v = s[k]
-                lo = v < x0[a]
-                hi = v >= x0[a] + L[a]
-                v[lo] = v[lo] + L[a]
-                v[hi] = v[hi] - L[a]
+            X = [s["x"], s["y"], s["z"]]
+            U = [s["ux"], s["uy"], s["uz"]]
+            out = np.zeros(len(X[0]), dtype=bool)
+            for a in range(3):
+                out |= (X[a] < x0[a]) | (X[a] >= x0[a] + L[a])
+            keep = np.ones(len(X[0]), dtype=bool)
+            for i in np.nonzero(out)[0]:
+                for a in range(3):
+                    v = X[a][i]
+                    done = False
+                    for side in (0, 1):
+                        if not ((v < x0[a]) if side == 0 else (v >= x0[a] + L[a])):
+                            continue
+                        b = self.pbound[2 * a + side]
+                        if b == 1:
+                            keep[i] = False
+                            done = True
+                        elif b == 2:
+                            X[a][i] = v + D[a] if side == 0 else v - D[a]
+                            U[a][i] = -U[a][i]
+                        elif b == 3:
+                            edge = x0[a] if side == 0 else x0[a] + L[a]
+                            X[a][i] = 2.0 * edge - X[a][i]
+                            for e in range(3):
+                                U[e][i] = 0.0
+                            it = self._tile_of([s["old"][e][i] for e in range(3)])
+                            for e in range(3):
+                                lo, hi = bounds[e][it[e]]
+                                X[e][i] = max(X[e][i], lo + 0.5 * D[e])
+                                X[e][i] = min(X[e][i], hi - 0.5 * D[e])
+                            done = True
+                        else:
+                            X[a][i] = v + L[a] if side == 0 else v - L[a]
+                    if done:
+                        break
+            if not keep.all():
+                for k in ("x", "y", "z", "ux", "uy", "uz", "gaminv", "w"):
+                    s[k] = s[k][keep]
 
     def _bcs(self, keys):
         for k in keys:

@@ -28,7 +28,7 @@ def rel(a, b):
 
 def build(torch, orc, order, lo, pusher, depo, norder=2, ntiles=(2, 2, 2), n=(16, 14, 12),
           ppc=4, uth=(0.2, 0.02), chunk=0, field_amp=0.05, seed=1, d=(1.0, 0.9, 1.1),
-          precision="fp64", id_scale=1e-9):
+          precision="fp64", id_scale=1e-9, pbound=None):
     from picsar_amd.sim import Simulation, _fd_weights
     nx, ny, nz = n
     dx, dy, dz = d
@@ -36,9 +36,11 @@ def build(torch, orc, order, lo, pusher, depo, norder=2, ntiles=(2, 2, 2), n=(16
     dt = default_dt(dx, dy, dz)
     sim = Simulation(nx, ny, nz, dx, dy, dz, xmin, ymin, zmin, *ntiles, order=order,
                      l_lower_order_in_v=lo, pusher=pusher, depo=depo, norder=norder, dt=dt,
-                     chunk=chunk, precision=precision)
+                     chunk=chunk, precision=precision, pbound=pbound)
     ref = OracleState(orc, nx, ny, nz, sim.ng, dx, dy, dz, xmin, ymin, zmin, dt, order, lo,
                       pusher, depo, norder, _fd_weights(norder) if norder > 2 else None)
+    ref.pbound = tuple(sim.grid.pbound)
+    ref.nt = tuple(ntiles)
     rng = np.random.default_rng(seed)
     shape = ref.f["ex"].shape
     for k in ("ex", "ey", "ez", "bx", "by", "bz"):
@@ -206,6 +208,43 @@ def test_laser_antennas_in_the_step(torch, orc, depo):
     ref.step()
     ref.step()
     compare_fields(sim, ref, tng, 1e-9)
+
+
+@pytest.mark.parametrize("pbound", [
+    ("absorbing",) * 6,
+    ("reflecting", "reflecting", "absorbing", "absorbing", "periodic", "periodic"),
+    ("periodic", "periodic", "periodic", "periodic", "reinjecting", "reinjecting"),
+    ("reinjecting", "absorbing", "reflecting", "periodic", "periodic", "reflecting"),
+])
+def test_particle_boundaries(torch, orc, pbound):
+    """Absorbing, reflecting and reinjecting domain boundaries
+    (particle_bcs_mpi_blocking, particle_boundaries.F90:1106-1300) with fast
+    particles (u_th = 0.6 c): the particles kept, their positions and momenta
+    and the fields match the oracle after one step (1e-12) and three (1e-9)."""
+    sim, ref, tng = build(torch, orc, 3, True, "vay", "esirkepov", uth=(0.6, 0.05),
+                          pbound=pbound)
+    n0 = sim.total_particles()
+    sim.step(1)
+    ref.step()
+    check_tiles(sim)
+    compare_fields(sim, ref, tng, 1e-12)
+    L = max(sim.nx * sim.dx, sim.ny * sim.dy, sim.nz * sim.dz)
+    compare_particles(sim, ref, 1e-12 * L, 1e-12)
+    if "absorbing" in pbound:
+        assert sim.total_particles() < n0
+    for _ in range(2):
+        sim.step(1)
+        ref.step()
+    check_tiles(sim)
+    compare_fields(sim, ref, tng, 1e-9)
+    assert sim.total_particles() == sum(len(s["x"]) for s in ref.species)
+
+
+def test_absorbing_boundaries_refuse_graph_capture(torch, orc):
+    sim, ref, tng = build(torch, orc, 1, True, "boris", "esirkepov", pbound=("absorbing",) * 6)
+    sim.step(1)
+    with pytest.raises(RuntimeError):
+        sim.capture_steps(2)
 
 
 def test_laser_antenna_refuses_graph_capture(torch, orc):
