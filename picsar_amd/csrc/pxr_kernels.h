@@ -306,32 +306,56 @@ PXR_DEV void esirkepov_deposit(const EsirkepovIn<T> &in, const Acc &acc) {
     cy[r] = in.wqy * sy;
     cz[r] = in.wqz * sz;
   }
+  // transverse weights (esirkepov_3d.F90:3427-3450): for Jx
+  // (S_y + dS_y/2) S_z + (S_y/2 + dS_y/3) dS_z, and likewise for Jy and Jz.
+  // The two factors of each are formed once per node of their axis, and
+  // each component runs its own loop nest, so every transverse weight is
+  // evaluated once per pair of transverse nodes.
   const T third = Cst<T>::onethird;
+  T hx[W], tx[W], hy[W], ty[W], hz[W], tz[W];
+#pragma unroll
+  for (int r = 0; r < W; r++) {
+    hx[r] = X.sn[r] + T(0.5) * dx_[r];
+    tx[r] = T(0.5) * X.sn[r] + third * dx_[r];
+    hy[r] = Y.sn[r] + T(0.5) * dy_[r];
+    ty[r] = T(0.5) * Y.sn[r] + third * dy_[r];
+    hz[r] = Z.sn[r] + T(0.5) * dz_[r];
+    tz[r] = T(0.5) * Z.sn[r] + third * dz_[r];
+  }
 #pragma unroll
   for (int rk = 0; rk < W; rk++) {
     if (rk >= Z.w) continue;
 #pragma unroll
     for (int rj = 0; rj < W; rj++) {
       if (rj >= Y.w) continue;
-      // transverse weights (esirkepov_3d.F90:3427-3450)
-      const T wx = (Y.sn[rj] + T(0.5) * dy_[rj]) * Z.sn[rk] +
-                   (T(0.5) * Y.sn[rj] + third * dy_[rj]) * dz_[rk];
+      const T wx = hy[rj] * Z.sn[rk] + ty[rj] * dz_[rk];
 #pragma unroll
-      for (int ri = 0; ri < W; ri++) {
-        if (ri >= X.w) continue;
-        const int i = X.base + ri, j = Y.base + rj, k = Z.base + rk;
-        if (ri < X.w - 1) acc.add(0, i, j, k, cx[ri] * wx);
-        if (rj < Y.w - 1) {
-          const T wy = (Z.sn[rk] + T(0.5) * dz_[rk]) * X.sn[ri] +
-                       (T(0.5) * Z.sn[rk] + third * dz_[rk]) * dx_[ri];
-          acc.add(1, i, j, k, cy[rj] * wy);
-        }
-        if (rk < Z.w - 1) {
-          const T wz = (X.sn[ri] + T(0.5) * dx_[ri]) * Y.sn[rj] +
-                       (T(0.5) * X.sn[ri] + third * dx_[ri]) * dy_[rj];
-          acc.add(2, i, j, k, cz[rk] * wz);
-        }
-      }
+      for (int ri = 0; ri < W - 1; ri++)
+        if (ri < X.w - 1) acc.add(0, X.base + ri, Y.base + rj, Z.base + rk, cx[ri] * wx);
+    }
+  }
+#pragma unroll
+  for (int rk = 0; rk < W; rk++) {
+    if (rk >= Z.w) continue;
+#pragma unroll
+    for (int ri = 0; ri < W; ri++) {
+      if (ri >= X.w) continue;
+      const T wy = hz[rk] * X.sn[ri] + tz[rk] * dx_[ri];
+#pragma unroll
+      for (int rj = 0; rj < W - 1; rj++)
+        if (rj < Y.w - 1) acc.add(1, X.base + ri, Y.base + rj, Z.base + rk, cy[rj] * wy);
+    }
+  }
+#pragma unroll
+  for (int rj = 0; rj < W; rj++) {
+    if (rj >= Y.w) continue;
+#pragma unroll
+    for (int ri = 0; ri < W; ri++) {
+      if (ri >= X.w) continue;
+      const T wz = hx[ri] * Y.sn[rj] + tx[ri] * dy_[rj];
+#pragma unroll
+      for (int rk = 0; rk < W - 1; rk++)
+        if (rk < Z.w - 1) acc.add(2, X.base + ri, Y.base + rj, Z.base + rk, cz[rk] * wz);
     }
   }
 }
