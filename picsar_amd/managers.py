@@ -1,0 +1,139 @@
+"""The reference's zero-argument step managers on a Simulation's state.
+
+PICSAR's drivers advance a step by calling managers that act on module state
+(submain.F90:99-199):
+
+    field_gathering_plus_particle_pusher   particle_pusher_manager_3d.F90:431
+    push_laser_particles                   laser_pusher_manager_3d.F90:15
+    particle_bcs                           particle_boundaries.F90:83
+    pxr_particle_sorting                   sorting.F90:67
+    pxrdepose_currents_on_grid_jxjyjz      current_deposition_manager_3d.F90:224
+    current_bcs, efield_bcs, bfield_bcs    field_boundaries.F90:1861, 1757, 1808
+    push_bfield, push_efield               maxwell_solver_manager.F90:44, 393
+
+``Managers(sim)`` exposes them under those names, acting on the fields and
+tiled species that ``sim`` keeps on the GPU, so a driver written against the
+managers runs unchanged, in the reference's order: the current is deposited
+after the particle boundaries, from the wrapped positions.
+
+The gather, push and deposition managers run the leaf kernels over each
+species' arrays (FP64 species); ``Simulation.step`` is the fast form of the
+same step, with one fused kernel per species.  The re-binning that
+``particle_bcs`` performs also sorts every tile, so ``pxr_particle_sorting``
+has nothing left to do.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _lib
+from .devmem import DeviceArray
+
+
+class Managers:
+    def __init__(self, sim):
+        self.sim = sim
+        self._e = None  # per-particle E and B of the largest species
+
+    def _fields_at_particles(self, n):
+        if self._e is None or self._e[0].n < n:
+            self._e = [DeviceArray(max(n, 1)) for _ in range(6)]
+        for a in self._e:
+            a.fill(0.0)  # the gather adds into them
+        return self._e
+
+    def _species(self):
+        for sp in self.sim.species:
+            if sp.precision != "fp64":
+                raise NotImplementedError("the manager-level step runs FP64 species; use "
+                                          "Simulation.step for mixed precision")
+            yield sp
+
+    def field_gathering_plus_particle_pusher(self):
+        """Gather, push and position push of every species
+        (particle_pusher_manager_3d.F90:431-672)."""
+        s = self.sim
+        f = s.fields
+        g = s.ng
+        for sp in self._species():
+            if sp.lfreeze or sp.npart == 0:
+                continue
+            n = sp.npart
+            a = sp.active.arrays
+            E = self._fields_at_particles(n)
+            _lib.call("pxrhip_geteb3d_energy_conserving", n, a["x"].ptr, a["y"].ptr, a["z"].ptr,
+                      *(e.ptr for e in E), s.xmin, s.ymin, s.zmin, s.dx, s.dy, s.dz, s.nx, s.ny,
+                      s.nz, g, g, g, s.order, s.order, s.order, f["ex"].ptr, f["ey"].ptr,
+                      f["ez"].ptr, f["bx"].ptr, f["by"].ptr, f["bz"].ptr, 0,
+                      int(s.l_lower_order_in_v), 8, 0, 0)
+            U = (a["ux"].ptr, a["uy"].ptr, a["uz"].ptr, a["gaminv"].ptr)
+            if s.pusher == 1:
+                _lib.call("pxrhip_ebcancelpush3d", n, *U, *(e.ptr for e in E), sp.q, sp.m, s.dt,
+                          0)
+            else:
+                _lib.call("pxrhip_boris_push_u_3d", n, *U, *(e.ptr for e in E), sp.q, sp.m,
+                          s.dt)
+            _lib.call("pxrhip_pushxyz", n, a["x"].ptr, a["y"].ptr, a["z"].ptr, *U, s.dt)
+
+    def push_laser_particles(self):
+        for ant in self.sim.antennas:
+            ant.push(self.sim)
+
+    def particle_bcs(self):
+        """Domain boundaries, tile re-binning and the per-tile sort."""
+        self.sim.particle_bcs_and_sorting()
+
+    def pxr_particle_sorting(self):
+        """Done by particle_bcs: the re-binning writes every tile sorted."""
+
+    def pxrdepose_currents_on_grid_jxjyjz(self):
+        """J = 0, then the current of every species and antenna
+        (current_deposition_manager_3d.F90:224-505)."""
+        s = self.sim
+        f = s.fields
+        g = s.ng
+        for k in ("jx", "jy", "jz"):
+            f[k].fill(0.0)
+        for sp in self._species():
+            if not sp.ldodepos or sp.npart == 0:
+                continue
+            a = sp.active.arrays
+            _lib.call("pxrhip_depose_jxjyjz", f["jx"].ptr, f["jy"].ptr, f["jz"].ptr, sp.npart,
+                      a["x"].ptr, a["y"].ptr, a["z"].ptr, a["ux"].ptr, a["uy"].ptr, a["uz"].ptr,
+                      a["gaminv"].ptr, a["w"].ptr, sp.q, s.xmin, s.ymin, s.zmin, s.dt, s.dx, s.dy,
+                      s.dz, s.nx, s.ny, s.nz, g, g, g, s.order, s.order, s.order, s.depo, 0)
+        for ant in s.antennas:
+            ant.deposit(s)
+
+    def current_bcs(self):
+        self.sim.current_bcs()
+
+    def push_bfield(self):
+        self.sim.push_bfield()
+
+    def bfield_bcs(self):
+        self.sim.bfield_bcs()
+
+    def push_efield(self):
+        self.sim.push_efield()
+
+    def efield_bcs(self):
+        self.sim.efield_bcs()
+
+    def step(self, nst: int = 1):
+        """submain.F90:99-199 with these managers."""
+        for _ in range(int(nst)):
+            self.field_gathering_plus_particle_pusher()
+            self.push_laser_particles()
+            self.particle_bcs()
+            self.pxr_particle_sorting()
+            self.pxrdepose_currents_on_grid_jxjyjz()
+            self.current_bcs()
+            self.push_bfield()
+            self.bfield_bcs()
+            self.push_efield()
+            self.efield_bcs()
+            self.push_bfield()
+            self.bfield_bcs()
+            self.sim.it += 1
+        self.sim.check()

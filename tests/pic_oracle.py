@@ -57,9 +57,27 @@ class OracleState:
         self.particle_bcs()
         self.it += 1
 
+    def step_reference_order(self):
+        """submain.F90:99-199 in the reference's order: gather + push, laser
+        push, particle boundaries and sort, then the deposition from the new
+        (wrapped) positions."""
+        self.gather_push()
+        self.push_antennas()
+        self.particle_bcs()
+        self.deposit(zero_j=True)
+        self.deposit_antennas()
+        self.fold_currents_and_advance_fields()
+        self.it += 1
+
     def gather_push_deposit(self, zero_j=True):
         """Gather + push of every species, then deposition of the pushed
-        particles into J (zeroed first when zero_j)."""
+        particles into J (zeroed first when zero_j); then the antennas."""
+        self.gather_push()
+        self.deposit(zero_j)
+        self.push_antennas()
+        self.deposit_antennas()
+
+    def gather_push(self):
         o, f = self.orc, self.f
         nx, ny, nz, ng = self.nx, self.ny, self.nz, self.ng
         d = (self.dx, self.dy, self.dz)
@@ -79,29 +97,37 @@ class OracleState:
             else:
                 o.boris_push_u_3d(*u, s["gaminv"], *e, s["q"], s["m"], dt)
             o.pushxyz(s["x"], s["y"], s["z"], *u, s["gaminv"], dt)
+
+    def _depose(self, s, w, q):
+        o, f = self.orc, self.f
+        d = (self.dx, self.dy, self.dz)
+        args = (f["jx"], f["jy"], f["jz"], s["x"], s["y"], s["z"], s["ux"], s["uy"], s["uz"],
+                s["gaminv"], w, q, *self.geom(), self.dt, *d, self.nx, self.ny, self.nz, self.ng,
+                self.ng, self.ng, self.order)
+        bad = o.depose_esirkepov(*args) if self.depo == "esirkepov" else \
+            o.depose_direct(*args, False)
+        assert bad == 0
+
+    def deposit(self, zero_j=True):
         if zero_j:
             for k in ("jx", "jy", "jz"):
-                f[k][...] = 0.0
+                self.f[k][...] = 0.0
         for s in self.species:
-            args = (f["jx"], f["jy"], f["jz"], s["x"], s["y"], s["z"], s["ux"], s["uy"], s["uz"],
-                    s["gaminv"], s["w"], s["q"], *self.geom(), dt, *d, nx, ny, nz, ng, ng, ng,
-                    self.order)
-            bad = o.depose_esirkepov(*args) if self.depo == "esirkepov" else \
-                o.depose_direct(*args, False)
-            assert bad == 0
-        # push_laser_particles at it*dt - dt/2 (laser_pusher_manager_3d.F90:37),
-        # then the antennas' current
+            self._depose(s, s["w"], s["q"])
+
+    def push_antennas(self):
+        """push_laser_particles at it*dt - dt/2 (laser_pusher_manager_3d.F90:37)."""
+        dt = self.dt
         for a in self.antennas:
-            o.laserp_pusher(a["window"], a["pid"], a["x"], a["y"], a["z"], a["ux"], a["uy"],
-                            a["uz"], a["gaminv"], dt, a["emax"], a["emax1"], a["emax2"], a["pv1"],
-                            a["pv2"], a["k0"], a["q_z"], a["charge"], a["tau"],
-                            self.it * dt - dt / 2.0, a["t_peak"], a["torder"], a["polangle"])
-            args = (f["jx"], f["jy"], f["jz"], a["x"], a["y"], a["z"], a["ux"], a["uy"], a["uz"],
-                    a["gaminv"], np.ascontiguousarray(a["pid"][:, 0]), a["charge"], *self.geom(),
-                    dt, *d, nx, ny, nz, ng, ng, ng, self.order)
-            bad = o.depose_esirkepov(*args) if self.depo == "esirkepov" else \
-                o.depose_direct(*args, False)
-            assert bad == 0
+            self.orc.laserp_pusher(a["window"], a["pid"], a["x"], a["y"], a["z"], a["ux"],
+                                   a["uy"], a["uz"], a["gaminv"], dt, a["emax"], a["emax1"],
+                                   a["emax2"], a["pv1"], a["pv2"], a["k0"], a["q_z"],
+                                   a["charge"], a["tau"], self.it * dt - dt / 2.0, a["t_peak"],
+                                   a["torder"], a["polangle"])
+
+    def deposit_antennas(self):
+        for a in self.antennas:
+            self._depose(a, np.ascontiguousarray(a["pid"][:, 0]), a["charge"])
 
     def fold_currents_and_advance_fields(self):
         o, f = self.orc, self.f

@@ -240,6 +240,42 @@ def test_particle_boundaries(torch, orc, pbound):
     assert sim.total_particles() == sum(len(s["x"]) for s in ref.species)
 
 
+@pytest.mark.parametrize("order,pusher,depo", [(3, "vay", "esirkepov"), (2, "boris", "direct"),
+                                               (1, "boris", "esirkepov")])
+def test_managers_in_reference_order(torch, orc, order, pusher, depo):
+    """The zero-argument managers (picsar_amd.managers) called in
+    submain.F90's order -- gather + push, particle boundaries and sort, then
+    the deposition from the wrapped positions -- with an antenna and
+    reflecting x faces: one step at 1e-12, three at 1e-9 against the oracle
+    in the same order."""
+    from picsar_amd.managers import Managers
+    pb = ("reflecting", "reflecting", "periodic", "periodic", "periodic", "periodic")
+    sim, ref, tng = build(torch, orc, order, True, pusher, depo, pbound=pb)
+    a = sim.add_laser_antenna(spot=(sim.xmin + 8.0, sim.ymin + 6.1, sim.zmin + 5.3),
+                              vector=(0, 0, 1), polvector1=(1, 0, 0), lambda_laser=8.0,
+                              laser_w0=4.0, laser_ctau=6.0, t_peak=1.5, emax1=0.05)
+    ref.add_antenna(_antenna_oracle_dict(a))
+    mgr = Managers(sim)
+    mgr.step(1)
+    ref.step_reference_order()
+    check_tiles(sim)
+    compare_fields(sim, ref, tng, 1e-12)
+    L = max(sim.nx * sim.dx, sim.ny * sim.dy, sim.nz * sim.dz)
+    compare_particles(sim, ref, 1e-12 * L, 1e-12)
+    mgr.step(2)
+    ref.step_reference_order()
+    ref.step_reference_order()
+    compare_fields(sim, ref, tng, 1e-9)
+
+
+def test_managers_refuse_mixed_precision(torch, orc):
+    from picsar_amd.managers import Managers
+    sim, ref, tng = build(torch, orc, 3, True, "vay", "esirkepov", precision="mixed",
+                          id_scale=2.0 ** -15, ppc=2)
+    with pytest.raises(NotImplementedError):
+        Managers(sim).field_gathering_plus_particle_pusher()
+
+
 def test_absorbing_boundaries_refuse_graph_capture(torch, orc):
     sim, ref, tng = build(torch, orc, 1, True, "boris", "esirkepov", pbound=("absorbing",) * 6)
     sim.step(1)
