@@ -503,7 +503,8 @@ struct ChunkArgs {
   i64 *start;       // exclusive scan of count, nunits+1 entries
   Chunk *out;
   int *nchunks, *queue;
-  i64 maxchunks;
+  i64 maxchunks;    // bound on the chunks of a launch; beyond it ...
+  unsigned long long *bad;  // ... chunks would be dropped, and are counted here
 };
 
 // plane roles in ChunkArgs::group (a value >= 0 is the last plane of the
@@ -580,6 +581,7 @@ template <bool WRITE>
 PXR_DEV int walk_unit(const ChunkArgs &a, int t, int z, i64 pos) {
   int nch = 0;
   auto emit = [&](int64_t b, int64_t n, int zz, int zz1, int y0, int y1) {
+    if (WRITE && pos + nch >= a.maxchunks) count_bad(a.bad);  // cannot happen: see the bound
     if (WRITE && pos + nch < a.maxchunks) {
       Chunk c;
       c.begin = b;
@@ -924,6 +926,10 @@ int pxrhip_tiles_gather_push_deposit(const pxrhip_grid *grid, const pxrhip_speci
   const int jacc = mx ? g_tune.jacc : 2;  // FP64 species: FP64 LDS current
   const int jnode = jacc == 0 ? 12 : 24;  // 3 FP32 | 3 FP32 + 3 compensations | 3 FP64
   const i64 nunits = (i64)tt->ntiles * nzmax;
+  // A plane gives one chunk if it heads a group of whole planes, none if it
+  // lies inside one, else its row walk: a chunk per row that opens one
+  // (<= its rows), per cut of a row denser than ch (<= cap/ch over all
+  // planes) and a last one.  Hence at most nrows + cap/ch + nunits.
   const i64 maxchunks = tt->nrows + (cap / ch + 1) + nunits + 16;
   const i64 nblk = (nunits + SCAN_B - 1) / SCAN_B;
   int *cnt = (int *)scratch(8, sizeof(int) * (size_t)nunits);
@@ -955,6 +961,7 @@ int pxrhip_tiles_gather_push_deposit(const pxrhip_grid *grid, const pxrhip_speci
   ca.nchunks = ctl;
   ca.queue = ctl + 1;
   ca.maxchunks = maxchunks;
+  ca.bad = rt().d_bad;
   hipLaunchKernelGGL(k_chunk_groups, dim3(grid1d(tt->ntiles, 64)), dim3(64), 0, st, ca);
   hipLaunchKernelGGL(k_chunk_count, dim3(grid1d(nunits, 256)), dim3(256), 0, st, ca);
   hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nblk), dim3(256), 0, st, cnt, nunits, bsum);
