@@ -167,6 +167,21 @@ def build_workload(args):
     return sim
 
 
+def gauss_state(sim):
+    """div E - rho on the nodes 0..n-1 of each axis (host copy) and the norm
+    of rho there, or None where rho is not deposited here (mixed-precision
+    species) or where the discrete Gauss law is not that of the solver
+    (FDTD order above 2)."""
+    if sim.norder != 2 or any(sp.precision != "fp64" for sp in sim.species):
+        return None
+    sim.gauss_residual()
+    g = sim.ng
+    cut = (slice(g, g + sim.nx), slice(g, g + sim.ny), slice(g, g + sim.nz))
+    dv = sim.diag_grid("dive").numpy()[cut]
+    rho = sim.diag_grid("rho").numpy()[cut]
+    return dv - rho, float(np.linalg.norm(rho))
+
+
 def run_gpu(args):
     from picsar_amd import _lib
     from picsar_amd.devmem import synchronize
@@ -186,6 +201,7 @@ def run_gpu(args):
         tuned = autotune(sim)
         log(f"autotune: {tuned['best']} ({tuned['ms_per_step']:.2f} ms/step gather+push+deposit)")
     ke_init, fe_init = sim.kinetic_energy(), sim.field_energy()
+    gauss0 = gauss_state(sim)
     for _ in range(args.warmup):
         sim.step(1, check=False)
     sim.check()
@@ -271,7 +287,15 @@ def run_gpu(args):
     if extra:
         log(f"{extra} more untimed steps for the energy drift over {done + extra} steps")
     ke_d, fe_d = sim.kinetic_energy(), sim.field_energy()
-    return dict(sim=sim, N=N, wall=wall, gpu_ms=total_ms, stage_ms={k: v / K for k, v in stage.items()},
+    gauss = None
+    if gauss0 is not None:
+        r1, rho1 = gauss_state(sim)
+        gauss = {"steps": done + extra, "rho_norm_initial": gauss0[1], "rho_norm_final": rho1,
+                 "residual_change_rel": float(np.linalg.norm(r1 - gauss0[0])) / gauss0[1],
+                 "what": "|(div E - rho)_final - (div E - rho)_initial| / |rho_initial| on the "
+                         "nodes 0..n-1 of each axis"}
+        del r1
+    return dict(gauss=gauss, sim=sim, N=N, wall=wall, gpu_ms=total_ms, stage_ms={k: v / K for k, v in stage.items()},
                 per_launch_ms=per_launch_ms, achieved=achieved, bytes_launch=bytes_launch,
                 ke0=ke0, fe0=fe0, ke=ke, fe=fe, ke_init=ke_init, fe_init=fe_init,
                 drift_steps=done + extra, ke_d=ke_d, fe_d=fe_d, tuned=tuned, graph=graph)
@@ -447,6 +471,7 @@ def main():
                    (r["ke_init"] + r["fe_init"]),
                    "relative_drift_timed_region": (r["ke"] + r["fe"] - r["ke0"] - r["fe0"]) /
                    (r["ke0"] + r["fe0"])},
+        "gauss_law": r["gauss"],
     }
     if args.calibrate_copy:
         from picsar_amd.devmem import DeviceArray, synchronize

@@ -97,6 +97,30 @@ int pxrhip_kinetic_energy_f32(int64_t np, const float *ux, const float *uy,
  * component of the field energy of diags.F90:1072-1114 */
 int pxrhip_field_energy(const double *f, int64_t nx, int64_t ny, int64_t nz,
                         int64_t ngx, int64_t ngy, int64_t ngz, double *out);
+/* calc_field_div, diags.F90:129-156: divee = (ex(j)-ex(j-1))/dx +
+ * (ey(k)-ey(k-1))/dy + (ez(l)-ez(l-1))/dz at the nodes 0..n of each axis;
+ * the other nodes of divee are left as they are.  Guards >= 1. */
+int pxrhip_calc_field_div(double *divee, const double *eex, const double *eey,
+                          const double *eez, int64_t nx, int64_t ny, int64_t nz,
+                          int64_t nxguard, int64_t nyguard, int64_t nzguard,
+                          double dx, double dy, double dz);
+/* calc_field_divB, diags.F90:170-197: forward differences of B at the
+ * nodes 1..n of each axis.  Guards >= 1. */
+int pxrhip_calc_field_divb(double *divbb, const double *bbx, const double *bby,
+                           const double *bbz, int64_t nx, int64_t ny, int64_t nz,
+                           int64_t nxguard, int64_t nyguard, int64_t nzguard,
+                           double dx, double dy, double dz);
+/* get_norm_divErho, diags.F90:1305-1345, on one domain: sqrt of the sum
+ * over the nodes 1..n of each axis of (divee2*eps0 - rho2)^2, eps0 = 1
+ * (normalised units). */
+int pxrhip_get_norm_diverho(const double *divee2, const double *rho2, int64_t nx2,
+                            int64_t ny2, int64_t nz2, int64_t nxguard,
+                            int64_t nyguard, int64_t nzguard, double *norm);
+/* get_loc_norm_2, diags.F90:1265-1293: the sum over the nodes 1..n of each
+ * axis of array^2 (not square-rooted, as in the reference). */
+int pxrhip_get_loc_norm_2(const double *array, int64_t nx2, int64_t ny2, int64_t nz2,
+                          int64_t nxguard, int64_t nyguard, int64_t nzguard,
+                          double *norm);
 
 /* ---- field gathering ---------------------------------------------------- */
 /* geteb3d_energy_conserving, field_gathering_manager_3d.F90:295-351.

@@ -14,6 +14,11 @@
 #   submain.F90           -> FD_weights
 #   laser_pusher_manager_3d.F90 -> gaussian_profile, hanning_profile (the
 #                            pushers around them read module state)
+#   diags.F90             -> calc_field_div, calc_field_divB, placed in a
+#                            module that gives them the host context of
+#                            their own module (USE PICSAR_precision,
+#                            constants; the MPI module it also uses is not
+#                            needed by these two)
 # -DLIBRARY selects normalised units (clight = mu0 = eps0 = 1).
 set -euo pipefail
 REF=${PXR_REF:-/root/reference}
@@ -48,6 +53,10 @@ extract_sub "$SRC/housekeeping/sorting.F90" pxr_particle_bin_sorting > "$TMP/bin
 extract_sub "$SRC/submain.F90" FD_weights > "$TMP/fdweights.F90"
 L="$SRC/particle_pushers/laser_pusher_manager_3d.F90"
 { extract_sub "$L" gaussian_profile; extract_sub "$L" hanning_profile; } > "$TMP/laserprof.F90"
+G="$SRC/diags/diags.F90"
+{ printf 'MODULE pxr_ref_diags\n  USE PICSAR_precision\n  USE constants\n  IMPLICIT NONE\nCONTAINS\n'
+  extract_sub "$G" calc_field_div; extract_sub "$G" calc_field_divB
+  printf 'END MODULE pxr_ref_diags\n'; } > "$TMP/divdiags.F90"
 
 cd "$TMP"
 $FC $FFLAGS -c pxr_consts.F90 -o pxr_consts.o
@@ -64,7 +73,7 @@ for f in field_gathering/energy_conserving/field_gathering_o1_3d.F90 \
   $FC $FFLAGS -c "$SRC/$f" -o "$o"
   OBJS="$OBJS $o"
 done
-for f in direct_scalar binsort fdweights laserprof; do
+for f in direct_scalar binsort fdweights laserprof divdiags; do
   $FC $FFLAGS -c "$f.F90" -o "$f.o"
   OBJS="$OBJS $f.o"
 done

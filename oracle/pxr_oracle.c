@@ -826,6 +826,54 @@ void orc_current_bc_periodic(double *a, i64 nx, i64 ny, i64 nz, i64 gx, i64 gy,
     }
 }
 
+/* ---- divergence diagnostics (diags.F90) -------------------------------- */
+/* calc_field_div (diags.F90:129-156): backward differences of E at the
+ * nodes 0..n of each axis, summed as (x + y) + z.                         */
+void orc_calc_field_div(double *dv, const double *ex, const double *ey, const double *ez,
+                        i64 nx, i64 ny, i64 nz, i64 gx, i64 gy, i64 gz, double dx,
+                        double dy, double dz) {
+  grid_t g = mkgrid(nx, ny, nz, gx, gy, gz);
+  const double ix = 1.0 / dx, iy = 1.0 / dy, iz = 1.0 / dz;
+  for (i64 l = 0; l <= nz; l++)
+    for (i64 k = 0; k <= ny; k++)
+      for (i64 j = 0; j <= nx; j++)
+        dv[gidx(&g, j, k, l)] = ix * (ex[gidx(&g, j, k, l)] - ex[gidx(&g, j - 1, k, l)]) +
+                                iy * (ey[gidx(&g, j, k, l)] - ey[gidx(&g, j, k - 1, l)]) +
+                                iz * (ez[gidx(&g, j, k, l)] - ez[gidx(&g, j, k, l - 1)]);
+}
+
+/* calc_field_divB (diags.F90:170-197): forward differences of B at the
+ * nodes 1..n of each axis.                                                 */
+void orc_calc_field_divb(double *dv, const double *bx, const double *by, const double *bz,
+                         i64 nx, i64 ny, i64 nz, i64 gx, i64 gy, i64 gz, double dx,
+                         double dy, double dz) {
+  grid_t g = mkgrid(nx, ny, nz, gx, gy, gz);
+  const double ix = 1.0 / dx, iy = 1.0 / dy, iz = 1.0 / dz;
+  for (i64 l = 1; l <= nz; l++)
+    for (i64 k = 1; k <= ny; k++)
+      for (i64 j = 1; j <= nx; j++)
+        dv[gidx(&g, j, k, l)] = ix * (bx[gidx(&g, j + 1, k, l)] - bx[gidx(&g, j, k, l)]) +
+                                iy * (by[gidx(&g, j, k + 1, l)] - by[gidx(&g, j, k, l)]) +
+                                iz * (bz[gidx(&g, j, k, l + 1)] - bz[gidx(&g, j, k, l)]);
+}
+
+/* get_loc_norm_divErho / get_loc_norm_2 (diags.F90:1222-1293): the sum over
+ * the nodes 1..n of each axis of (a*eps0 - b)^2 (eps0 = 1, normalised
+ * units), or of a^2 when b is NULL; summed in the reference's loop order. */
+double orc_node_norm2(const double *a, const double *b, i64 nx, i64 ny, i64 nz, i64 gx, i64 gy,
+                      i64 gz) {
+  grid_t g = mkgrid(nx, ny, nz, gx, gy, gz);
+  double s = 0.0;
+  for (i64 l = 1; l <= nz; l++)
+    for (i64 k = 1; k <= ny; k++)
+      for (i64 j = 1; j <= nx; j++) {
+        const i64 p = gidx(&g, j, k, l);
+        const double v = b ? a[p] * 1.0 - b[p] : a[p];
+        s = s + v * v;
+      }
+  return s;
+}
+
 /* ---- laser antenna (laser_pusher_manager_3d.F90) ---------------------- */
 /* Complex numbers as (re, im) pairs, in the reference's evaluation order;
  * the division is Smith's (1962) algorithm.                              */

@@ -91,6 +91,12 @@ class Oracle:
         L.orc_field_bc_periodic.argtypes = [P] + [I] * 6
         L.orc_current_bc_periodic.restype = None
         L.orc_current_bc_periodic.argtypes = [P] + [I] * 6
+        L.orc_calc_field_div.restype = None
+        L.orc_calc_field_div.argtypes = [P] * 4 + [I] * 6 + [F] * 3
+        L.orc_calc_field_divb.restype = None
+        L.orc_calc_field_divb.argtypes = [P] * 4 + [I] * 6 + [F] * 3
+        L.orc_node_norm2.restype = F
+        L.orc_node_norm2.argtypes = [P, P] + [I] * 6
         L.orc_laser_profile.restype = None
         L.orc_laser_profile.argtypes = [I, I, P, P, P, F, F, P, P, F, F, F, F, F, F, I, F]
         L.orc_laserp_pusher.restype = None
@@ -213,6 +219,27 @@ class Oracle:
     def current_bc_periodic(self, a, nx, ny, nz, gx, gy, gz):
         _chk_grid(a)
         self.lib.orc_current_bc_periodic(_ptr(a), nx, ny, nz, gx, gy, gz)
+
+    # -- divergence diagnostics ----------------------------------------------
+    def calc_field_div(self, dv, ex, ey, ez, nx, ny, nz, gx, gy, gz, dx, dy, dz):
+        _chk_grid(dv, ex, ey, ez)
+        self.lib.orc_calc_field_div(*map(_ptr, (dv, ex, ey, ez)), nx, ny, nz, gx, gy, gz, dx, dy,
+                                    dz)
+
+    def calc_field_divb(self, dv, bx, by, bz, nx, ny, nz, gx, gy, gz, dx, dy, dz):
+        _chk_grid(dv, bx, by, bz)
+        self.lib.orc_calc_field_divb(*map(_ptr, (dv, bx, by, bz)), nx, ny, nz, gx, gy, gz, dx, dy,
+                                     dz)
+
+    def norm_diverho(self, dv, rho, nx, ny, nz, gx, gy, gz):
+        """get_norm_divErho on one domain (square-rooted)."""
+        _chk_grid(dv, rho)
+        return float(np.sqrt(self.lib.orc_node_norm2(_ptr(dv), _ptr(rho), nx, ny, nz, gx, gy,
+                                                     gz)))
+
+    def loc_norm_2(self, a, nx, ny, nz, gx, gy, gz):
+        _chk_grid(a)
+        return self.lib.orc_node_norm2(_ptr(a), None, nx, ny, nz, gx, gy, gz)
 
     # -- laser antenna -------------------------------------------------------
     def laser_profile(self, window, xx, yy, emax1, emax2, pv1, pv2, k0, q_z, tau, t, t_peak,
@@ -403,6 +430,21 @@ class Reference:
         c = np.zeros(max(norder // 2, 1))
         self._f("fd_weights_")(_ptr(c), _ri(norder), _ri(1 if l_nodal else 0))
         return c[: norder // 2]
+
+    def _div(self, name, dv, fx, fy, fz, nx, ny, nz, gx, gy, gz, dx, dy, dz):
+        _chk_grid(dv, fx, fy, fz)
+        self._f(name)(*map(_ptr, (dv, fx, fy, fz)), _ri(nx), _ri(ny), _ri(nz), _ri(gx), _ri(gy),
+                      _ri(gz), _rf(dx), _rf(dy), _rf(dz))
+
+    # diags.F90:129 and :170, compiled inside a module (flang's module
+    # procedure names)
+    def calc_field_div(self, dv, ex, ey, ez, nx, ny, nz, gx, gy, gz, dx, dy, dz):
+        self._div("_QMpxr_ref_diagsPcalc_field_div", dv, ex, ey, ez, nx, ny, nz, gx, gy, gz, dx,
+                  dy, dz)
+
+    def calc_field_divb(self, dv, bx, by, bz, nx, ny, nz, gx, gy, gz, dx, dy, dz):
+        self._div("_QMpxr_ref_diagsPcalc_field_divb", dv, bx, by, bz, nx, ny, nz, gx, gy, gz, dx,
+                  dy, dz)
 
     def laser_profile(self, window, xx, yy, emax1, emax2, pv1, pv2, k0, q_z, tau, t, t_peak,
                       temporal_order, polangle, emax=0.0):

@@ -49,6 +49,10 @@ SIGNATURES = {
     "pxrhip_graph_destroy": [_P],
     "pxrhip_kinetic_energy_f32": [_I, _P, _P, _P, _P, C.POINTER(C.c_double)],
     "pxrhip_field_energy": [_P] + [_I] * 6 + [C.POINTER(C.c_double)],
+    "pxrhip_calc_field_div": [_P] * 4 + [_I] * 6 + [_F] * 3,
+    "pxrhip_calc_field_divb": [_P] * 4 + [_I] * 6 + [_F] * 3,
+    "pxrhip_get_norm_diverho": [_P, _P] + [_I] * 6 + [C.POINTER(C.c_double)],
+    "pxrhip_get_loc_norm_2": [_P] + [_I] * 6 + [C.POINTER(C.c_double)],
     "pxrhip_geteb3d_energy_conserving": [_I] + [_P] * 9 + [_F] * 6 + [_I] * 9 + [_P] * 6 + [_I] * 5,
     "pxrhip_boris_push_u_3d": [_I] + [_P] * 10 + [_F] * 3,
     "pxrhip_ebcancelpush3d": [_I] + [_P] * 10 + [_F] * 3 + [_I],

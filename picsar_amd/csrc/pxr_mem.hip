@@ -99,6 +99,50 @@ __global__ __launch_bounds__(256) void k_sq_partial(const double *f, GridGeom g,
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
+// divergence diagnostics, one thread per node (x fastest):
+// calc_field_div (diags.F90:129-156), backward differences of E at the nodes
+// 0..n of each axis; calc_field_divB (diags.F90:170-197), forward
+// differences of B at the nodes 1..n.  The sum is grouped as the reference's
+// (x + y) + z.
+template <bool FORWARD>
+__global__ __launch_bounds__(256) void k_div(double *__restrict__ out, const double *__restrict__ fx,
+                                             const double *__restrict__ fy,
+                                             const double *__restrict__ fz, GridGeom g, int lo,
+                                             int mx, int my, i64 tot, double idx, double idy,
+                                             double idz) {
+  const i64 c = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= tot) return;
+  const int i = lo + (int)(c % mx), j = lo + (int)((c / mx) % my), l = lo + (int)(c / ((i64)mx * my));
+  const i64 p = g.idx(i, j, l);
+  const i64 sy = g.NX, sz = g.NX * g.NY;
+  if (FORWARD)
+    out[p] = idx * (fx[p + 1] - fx[p]) + idy * (fy[p + sy] - fy[p]) + idz * (fz[p + sz] - fz[p]);
+  else
+    out[p] = idx * (fx[p] - fx[p - 1]) + idy * (fy[p] - fy[p - sy]) + idz * (fz[p] - fz[p - sz]);
+}
+
+// sum over the nodes 1..n of each axis of (a*eps0 - b)^2, or of a^2 when b
+// is null (get_loc_norm_divErho and get_loc_norm_2, diags.F90:1222-1293)
+__global__ __launch_bounds__(256) void k_norm_partial(const double *a, const double *b,
+                                                      double eps0, GridGeom g, double *part) {
+  __shared__ double s[256];
+  const i64 tot = (i64)g.nx * g.ny * g.nz;
+  const i64 b0 = (i64)blockIdx.x * RED_B;
+  double acc = 0.0;
+  for (int k = threadIdx.x; k < RED_B; k += 256) {
+    const i64 c = b0 + k;
+    if (c < tot) {
+      const int i = 1 + (int)(c % g.nx), j = 1 + (int)((c / g.nx) % g.ny),
+                l = 1 + (int)(c / ((i64)g.nx * g.ny));
+      const i64 p = g.idx(i, j, l);
+      const double v = b ? a[p] * eps0 - b[p] : a[p];
+      acc += v * v;
+    }
+  }
+  const double t = block_sum(acc, s);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
 __global__ __launch_bounds__(256) void k_sum_final(const double *part, i64 n, double *out) {
   __shared__ double s[256];
   double acc = 0.0;
@@ -261,6 +305,84 @@ int pxrhip_field_energy(const double *f, int64_t nx, int64_t ny, int64_t nz, int
   hipLaunchKernelGGL(k_sq_partial, dim3((unsigned)nblk), dim3(256), 0, rt().stream, f, g, part);
   PXR_LAUNCHED();
   return reduce_finish(part, nblk, out);
+}
+
+static bool grid_args(int64_t nx, int64_t ny, int64_t nz, int64_t ngx, int64_t ngy, int64_t ngz,
+                      int64_t min_guard, GridGeom &g) {
+  if (nx < 1 || ny < 1 || nz < 1 || ngx < min_guard || ngy < min_guard || ngz < min_guard)
+    return false;
+  g.nx = (int)nx; g.ny = (int)ny; g.nz = (int)nz;
+  g.gx = (int)ngx; g.gy = (int)ngy; g.gz = (int)ngz;
+  g.NX = nx + 1 + 2 * ngx;
+  g.NY = ny + 1 + 2 * ngy;
+  return true;
+}
+
+static int field_div(bool forward, double *out, const double *fx, const double *fy,
+                     const double *fz, int64_t nx, int64_t ny, int64_t nz, int64_t ngx,
+                     int64_t ngy, int64_t ngz, double dx, double dy, double dz) {
+  PXR_INIT();
+  GridGeom g;
+  if (!out || !fx || !fy || !fz || !grid_args(nx, ny, nz, ngx, ngy, ngz, 1, g))
+    return set_error(PXRHIP_ERR_ARG, "bad arguments (the differences need guards >= 1)");
+  if (!(dx > 0.0) || !(dy > 0.0) || !(dz > 0.0))
+    return set_error(PXRHIP_ERR_ARG, "cell sizes must be positive");
+  // nodes 0..n (E) or 1..n (B) of each axis
+  const int lo = forward ? 1 : 0;
+  const int mx = (int)nx + 1 - lo, my = (int)ny + 1 - lo, mz = (int)nz + 1 - lo;
+  const i64 tot = (i64)mx * my * mz;
+  const double idx = 1.0 / dx, idy = 1.0 / dy, idz = 1.0 / dz;
+  if (forward)
+    hipLaunchKernelGGL(k_div<true>, dim3(grid1d(tot, 256)), dim3(256), 0, rt().stream, out, fx, fy,
+                       fz, g, lo, mx, my, tot, idx, idy, idz);
+  else
+    hipLaunchKernelGGL(k_div<false>, dim3(grid1d(tot, 256)), dim3(256), 0, rt().stream, out, fx,
+                       fy, fz, g, lo, mx, my, tot, idx, idy, idz);
+  PXR_LAUNCHED();
+  return PXRHIP_OK;
+}
+
+int pxrhip_calc_field_div(double *divee, const double *eex, const double *eey, const double *eez,
+                          int64_t nx, int64_t ny, int64_t nz, int64_t nxguard, int64_t nyguard,
+                          int64_t nzguard, double dx, double dy, double dz) {
+  return field_div(false, divee, eex, eey, eez, nx, ny, nz, nxguard, nyguard, nzguard, dx, dy, dz);
+}
+
+int pxrhip_calc_field_divb(double *divbb, const double *bbx, const double *bby, const double *bbz,
+                           int64_t nx, int64_t ny, int64_t nz, int64_t nxguard, int64_t nyguard,
+                           int64_t nzguard, double dx, double dy, double dz) {
+  return field_div(true, divbb, bbx, bby, bbz, nx, ny, nz, nxguard, nyguard, nzguard, dx, dy, dz);
+}
+
+static int node_norm(const double *a, const double *b, int64_t nx, int64_t ny, int64_t nz,
+                     int64_t ngx, int64_t ngy, int64_t ngz, double *out) {
+  PXR_INIT();
+  GridGeom g;
+  if (!a || !out || !grid_args(nx, ny, nz, ngx, ngy, ngz, 0, g))
+    return set_error(PXRHIP_ERR_ARG, "bad arguments");
+  const i64 tot = nx * ny * nz;
+  const i64 nblk = (tot + RED_B - 1) / RED_B;
+  double *part = (double *)scratch(11, sizeof(double) * (size_t)(nblk + 1));
+  if (!part) return set_error(PXRHIP_ERR_HIP, "scratch allocation failed");
+  // eps0 = 1: normalised units, the reference's LIBRARY build (modules.F90)
+  hipLaunchKernelGGL(k_norm_partial, dim3((unsigned)nblk), dim3(256), 0, rt().stream, a, b, 1.0, g,
+                     part);
+  PXR_LAUNCHED();
+  return reduce_finish(part, nblk, out);
+}
+
+int pxrhip_get_norm_diverho(const double *divee2, const double *rho2, int64_t nx2, int64_t ny2,
+                            int64_t nz2, int64_t nxguard, int64_t nyguard, int64_t nzguard,
+                            double *norm) {
+  if (!rho2) return set_error(PXRHIP_ERR_ARG, "rho2 is null");
+  const int rc = node_norm(divee2, rho2, nx2, ny2, nz2, nxguard, nyguard, nzguard, norm);
+  if (rc == PXRHIP_OK) *norm = sqrt(*norm);
+  return rc;
+}
+
+int pxrhip_get_loc_norm_2(const double *array, int64_t nx2, int64_t ny2, int64_t nz2,
+                          int64_t nxguard, int64_t nyguard, int64_t nzguard, double *norm) {
+  return node_norm(array, nullptr, nx2, ny2, nz2, nxguard, nyguard, nzguard, norm);
 }
 
 }  // extern "C"

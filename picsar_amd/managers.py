@@ -10,6 +10,9 @@ PICSAR's drivers advance a step by calling managers that act on module state
     pxrdepose_currents_on_grid_jxjyjz      current_deposition_manager_3d.F90:224
     current_bcs, efield_bcs, bfield_bcs    field_boundaries.F90:1861, 1757, 1808
     push_bfield, push_efield               maxwell_solver_manager.F90:44, 393
+    pxrdepose_rho_on_grid, charge_bcs      charge_deposition_manager.F90:63,
+                                           field_boundaries.F90:1910
+    calc_diags                             diags.F90:63
 
 ``Managers(sim)`` exposes them under those names, acting on the fields and
 tiled species that ``sim`` keeps on the GPU, so a driver written against the
@@ -120,8 +123,32 @@ class Managers:
     def efield_bcs(self):
         self.sim.efield_bcs()
 
+    def pxrdepose_rho_on_grid(self):
+        """rho = 0, then the charge density of every species that deposits
+        (charge_deposition_manager.F90:63-250)."""
+        self.sim.pxrdepose_rho_on_grid()
+
+    def charge_bcs(self):
+        """Periodic fold of the rho guards (field_boundaries.F90:1910-1930)."""
+        self.sim.charge_bcs()
+
+    def calc_diags(self):
+        """calc_diags (diags.F90:63-115): the charge density and its guard
+        fold (the reference's div E branch there is switched off)."""
+        self.pxrdepose_rho_on_grid()
+        self.charge_bcs()
+
+    def calc_field_div(self):
+        """div E into the simulation's ``dive`` array (diags.F90:129)."""
+        return self.sim.calc_field_div()
+
+    def get_norm_divErho(self) -> float:
+        """||div E * eps0 - rho|| of the dive and rho arrays (diags.F90:1305)."""
+        return self.sim.get_norm_divErho()
+
     def step(self, nst: int = 1):
-        """submain.F90:99-199 with these managers."""
+        """submain.F90:99-199 with these managers (calc_diags included, as
+        the reference calls it every step)."""
         for _ in range(int(nst)):
             self.field_gathering_plus_particle_pusher()
             self.push_laser_particles()
@@ -135,5 +162,6 @@ class Managers:
             self.efield_bcs()
             self.push_bfield()
             self.bfield_bcs()
+            self.calc_diags()
             self.sim.it += 1
         self.sim.check()

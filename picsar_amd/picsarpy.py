@@ -435,5 +435,52 @@ class _Picsar:
                   int(nzg))
         c.finish()
 
+    def _div(self, entry, out, fx, fy, fz, nx, ny, nz, nxguard, nyguard, nzguard, dx, dy, dz,
+             names):
+        shp = _gshape(nx, ny, nz, nxguard, nyguard, nzguard)
+        tot = shp[0] * shp[1] * shp[2]
+        c = _Call()
+        A = [c.arr(out, tot, names[0], write=True, fortran_shape=shp)]
+        A += [c.arr(a, tot, nm, fortran_shape=shp) for a, nm in zip((fx, fy, fz), names[1:])]
+        A = c.ptrs(A)
+        c.before()
+        _lib.call(entry, *A, int(nx), int(ny), int(nz), int(nxguard), int(nyguard), int(nzguard),
+                  float(dx), float(dy), float(dz))
+        c.finish()
+
+    # diags.F90:129
+    def calc_field_div(self, divee, eex, eey, eez, nx, ny, nz, nxguard, nyguard, nzguard, dx, dy,
+                       dz):
+        self._div("pxrhip_calc_field_div", divee, eex, eey, eez, nx, ny, nz, nxguard, nyguard,
+                  nzguard, dx, dy, dz, ("divee", "eex", "eey", "eez"))
+
+    # diags.F90:170
+    def calc_field_divB(self, divbb, bbx, bby, bbz, nx, ny, nz, nxguard, nyguard, nzguard, dx, dy,
+                        dz):
+        self._div("pxrhip_calc_field_divb", divbb, bbx, bby, bbz, nx, ny, nz, nxguard, nyguard,
+                  nzguard, dx, dy, dz, ("divbb", "bbx", "bby", "bbz"))
+
+    def _norm(self, entry, arrays, names, nx2, ny2, nz2, nxguard, nyguard, nzguard):
+        shp = _gshape(nx2, ny2, nz2, nxguard, nyguard, nzguard)
+        tot = shp[0] * shp[1] * shp[2]
+        c = _Call()
+        A = c.ptrs([c.arr(a, tot, nm, fortran_shape=shp) for a, nm in zip(arrays, names)])
+        out = C.c_double(0.0)
+        c.before()
+        _lib.call(entry, *A, int(nx2), int(ny2), int(nz2), int(nxguard), int(nyguard),
+                  int(nzguard), C.byref(out))
+        c.finish()
+        return out.value
+
+    # diags.F90:1305 (the norm, intent(out) in the reference, is returned)
+    def get_norm_divErho(self, divee2, rho2, nx2, ny2, nz2, nxguard, nyguard, nzguard):
+        return self._norm("pxrhip_get_norm_diverho", (divee2, rho2), ("divee2", "rho2"), nx2, ny2,
+                          nz2, nxguard, nyguard, nzguard)
+
+    # diags.F90:1265 (returned, not square-rooted)
+    def get_loc_norm_2(self, array, nx2, ny2, nz2, nxguard, nyguard, nzguard):
+        return self._norm("pxrhip_get_loc_norm_2", (array,), ("array",), nx2, ny2, nz2, nxguard,
+                          nyguard, nzguard)
+
 
 picsar = _Picsar()

@@ -188,6 +188,7 @@ class Simulation:
             self.coefs = None
         self.species: List[Species] = []
         self.antennas = []
+        self.diag_fields = {}  # rho, dive, divb: allocated on first use
 
     # ------------------------------------------------------------------
     # particles
@@ -469,6 +470,85 @@ class Simulation:
 
     def total_particles(self) -> int:
         return sum(sp.npart for sp in self.species)
+
+    # ------------------------------------------------------------------
+    # charge density and Gauss's law (charge_deposition_manager.F90:63-743,
+    # field_boundaries.F90:1910-1930, diags.F90:63-197, 1222-1345)
+    # ------------------------------------------------------------------
+    def diag_grid(self, name: str) -> DeviceGrid:
+        """The diagnostic grid array ``name`` (rho, dive, divb), zeroed when
+        it is first used."""
+        g = self.diag_fields.get(name)
+        if g is None:
+            g = DeviceGrid(self.nx, self.ny, self.nz, self.ng)
+            g.fill(0.0)
+            self.diag_fields[name] = g
+        return g
+
+    def pxrdepose_rho_on_grid(self):
+        """rho = 0, then the charge density of every species that deposits
+        (frozen ones included) and of the laser antennas, which the
+        reference holds as species that deposit (tiling.F90:1684-1697), at
+        the simulation's shape order."""
+        rho = self.diag_grid("rho")
+        rho.fill(0.0)
+        g = self.ng
+        parts = []
+        for sp in self.species:
+            if not sp.ldodepos or sp.npart == 0:
+                continue
+            if sp.precision != "fp64":
+                raise NotImplementedError("the charge density is deposited from FP64 species")
+            a = sp.active.arrays
+            parts.append((sp.npart, a["x"], a["y"], a["z"], a["w"], sp.q))
+        for ant in self.antennas:
+            if ant.npart:  # pid column 1 holds the weights
+                parts.append((ant.npart, ant.x, ant.y, ant.z, ant.pid, ant.charge))
+        for n, x, y, z, w, q in parts:
+            _lib.call("pxrhip_depose_rho_n", rho.ptr, n, x.ptr, y.ptr, z.ptr, w.ptr, q, self.xmin,
+                      self.ymin, self.zmin, self.dx, self.dy, self.dz, self.nx, self.ny, self.nz,
+                      g, g, g, self.order, self.order, self.order, 1, 0)
+
+    def charge_bcs(self):
+        """Periodic fold of the rho guards (summation_bcs on rho)."""
+        g = self.ng
+        _pxr.summation_bcs(self.diag_grid("rho"), g, g, g, self.nx, self.ny, self.nz)
+
+    def calc_field_div(self) -> DeviceGrid:
+        """div E at the nodes 0..n (backward differences)."""
+        f, g = self.fields, self.ng
+        d = self.diag_grid("dive")
+        _pxr.calc_field_div(d, f["ex"], f["ey"], f["ez"], self.nx, self.ny, self.nz, g, g, g,
+                            self.dx, self.dy, self.dz)
+        return d
+
+    def calc_field_divB(self) -> DeviceGrid:
+        """div B at the nodes 1..n (forward differences)."""
+        f, g = self.fields, self.ng
+        d = self.diag_grid("divb")
+        _pxr.calc_field_divB(d, f["bx"], f["by"], f["bz"], self.nx, self.ny, self.nz, g, g, g,
+                             self.dx, self.dy, self.dz)
+        return d
+
+    def get_norm_divErho(self) -> float:
+        """sqrt(sum over the nodes 1..n of (div E * eps0 - rho)^2) of the
+        current dive and rho arrays."""
+        g = self.ng
+        return _pxr.get_norm_divErho(self.diag_grid("dive"), self.diag_grid("rho"), self.nx,
+                                     self.ny, self.nz, g, g, g)
+
+    def gauss_residual(self) -> float:
+        """Gauss's law residual of the present state: rho deposited and
+        folded, div E computed, then the norm of div E*eps0 - rho over the
+        nodes 1..n.  With the Esirkepov current and the Yee solver the
+        residual array keeps its initial value to round-off on the nodes
+        0..n-1 of each axis; node n of rho holds the node-n deposit twice
+        (the periodic fold of one rank, field_boundaries.F90:431-441), as in
+        the reference, and enters the norm."""
+        self.pxrdepose_rho_on_grid()
+        self.charge_bcs()
+        self.calc_field_div()
+        return self.get_norm_divErho()
 
 
 class StepGraph:

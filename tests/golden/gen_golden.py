@@ -217,11 +217,27 @@ def gen_laser(ref, rng):
     np.savez_compressed(os.path.join(HERE, "laser.npz"), **d)
 
 
+def gen_div(ref, rng):
+    """calc_field_div and calc_field_divB (diags.F90:129-197) of random
+    fields on a small grid with non-cubic cells."""
+    nx, ny, nz, g = 7, 5, 6, 2
+    d = dict(nx=nx, ny=ny, nz=nz, ng=g, dx=0.9, dy=1.1, dz=1.3)
+    shp = (nx + 1 + 2 * g, ny + 1 + 2 * g, nz + 1 + 2 * g)
+    for k in ("fx", "fy", "fz"):
+        d[k] = np.asfortranarray(rng.standard_normal(shp))
+    for name, fn in (("dive", ref.calc_field_div), ("divb", ref.calc_field_divb)):
+        out = np.asfortranarray(np.full(shp, 7.0))  # nodes outside the loops keep their value
+        fn(out, d["fx"], d["fy"], d["fz"], nx, ny, nz, g, g, g, d["dx"], d["dy"], d["dz"])
+        d[name] = out
+    np.savez_compressed(os.path.join(HERE, "div.npz"), **d)
+
+
 def main():
     ref = load_reference()
     if ref is None:
         sys.exit("oracle/_ref/libpxrref.so is missing: run bash oracle/build_ref.sh first")
-    which = set(sys.argv[1:]) or {"gather", "push", "deposit", "fdtd", "sort", "rho", "laser"}
+    which = set(sys.argv[1:]) or {"gather", "push", "deposit", "fdtd", "sort", "rho", "laser",
+                                  "div"}
     rng = np.random.default_rng(20241202)  # one stream for the first five families
     for name, fn in (("gather", gen_gather), ("push", gen_push), ("deposit", gen_deposit),
                      ("fdtd", gen_fdtd), ("sort", gen_sort)):
@@ -231,6 +247,8 @@ def main():
         gen_rho(ref, np.random.default_rng(20241203))
     if "laser" in which:
         gen_laser(ref, np.random.default_rng(20241204))
+    if "div" in which:
+        gen_div(ref, np.random.default_rng(20241205))
     meta = dict(nx=NX, ny=NY, nz=NZ, ng=NG, dx=DX, dy=DY, dz=DZ, xmin=XMIN, ymin=YMIN, zmin=ZMIN)
     np.savez(os.path.join(HERE, "grid.npz"), **meta)
     for f in sorted(os.listdir(HERE)):

@@ -351,6 +351,54 @@ def test_guards_bitexact(dev, pxr, orc):
     np.testing.assert_array_equal(HG(t), b)
 
 
+def test_field_divergence_bitexact(dev, pxr, orc, golden):
+    """calc_field_div / calc_field_divB: bit for bit against the golden
+    vectors of the reference's routines and against the oracle; the nodes
+    outside the loops keep their value."""
+    d = golden("div")
+    nx, ny, nz, g = (int(d[k]) for k in ("nx", "ny", "nz", "ng"))
+    h = tuple(float(d[k]) for k in ("dx", "dy", "dz"))
+    F = [TG(dev, np.asfortranarray(d[k])) for k in ("fx", "fy", "fz")]
+    for name, fn in (("dive", pxr.calc_field_div), ("divb", pxr.calc_field_divB)):
+        out = TG(dev, np.asfortranarray(np.full(d["fx"].shape, 7.0)))
+        fn(out, *F, nx, ny, nz, g, g, g, *h)
+        np.testing.assert_array_equal(HG(out), d[name])
+    rng = np.random.default_rng(12)
+    A = [np.asfortranarray(rng.standard_normal(SHAPE)) for _ in range(3)]
+    h = (0.7, 1.2, 0.95)
+    for ofn, gfn in ((orc.calc_field_div, pxr.calc_field_div),
+                     (orc.calc_field_divb, pxr.calc_field_divB)):
+        want = np.asfortranarray(np.zeros(SHAPE))
+        ofn(want, *A, NX, NY, NZ, NG, NG, NG, *h)
+        got = np.asfortranarray(np.zeros(SHAPE))
+        gfn(got, *A, NX, NY, NZ, NG, NG, NG, *h)  # numpy arrays: staged and written back
+        np.testing.assert_array_equal(got, want)
+
+
+def test_divergence_norms(dev, pxr, orc):
+    """get_norm_divErho and get_loc_norm_2 against the oracle's sums (the
+    GPU sums in another order)."""
+    rng = np.random.default_rng(13)
+    dv, rho = (np.asfortranarray(rng.standard_normal(SHAPE)) for _ in range(2))
+    D, R = TG(dev, dv), TG(dev, rho)
+    got = pxr.get_norm_divErho(D, R, NX, NY, NZ, NG, NG, NG)
+    assert got == pytest.approx(orc.norm_diverho(dv, rho, NX, NY, NZ, NG, NG, NG), rel=1e-13)
+    got = pxr.get_loc_norm_2(D, NX, NY, NZ, NG, NG, NG)
+    assert got == pytest.approx(orc.loc_norm_2(dv, NX, NY, NZ, NG, NG, NG), rel=1e-13)
+
+
+def test_divergence_argument_errors(dev, pxr):
+    from picsar_amd import PxrHipError
+    from picsar_amd._lib import ERR_ARG
+    a = [TG(dev, np.zeros((6, 6, 6), order="F")) for _ in range(4)]
+    with pytest.raises(PxrHipError) as ei:  # no guard: the differences would leave the array
+        pxr.calc_field_div(*a, 5, 5, 5, 0, 0, 0, 1.0, 1.0, 1.0)
+    assert ei.value.code == ERR_ARG
+    with pytest.raises(PxrHipError) as ei:
+        pxr.calc_field_divB(*a, 5, 5, 5, 0, 0, 0, 1.0, 1.0, 1.0)
+    assert ei.value.code == ERR_ARG
+
+
 @pytest.mark.parametrize("case", ["", "2"])
 def test_sort_golden_bitexact(dev, pxr, golden, case):
     d = golden("sort")

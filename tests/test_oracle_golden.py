@@ -136,3 +136,16 @@ def test_laser_profile(orc, golden, case):
     ref = d[f"amp{case}"]
     scale = max(np.abs(ref).max(), 1e-300)
     assert np.abs(amp - ref).max() <= 8 * np.finfo(float).eps * scale
+
+
+def test_field_divergence(orc, golden):
+    """calc_field_div and calc_field_divB (diags.F90:129-197) bit for bit,
+    including the nodes outside their loops, which keep their value."""
+    d = golden("div")
+    nx, ny, nz, g = (int(d[k]) for k in ("nx", "ny", "nz", "ng"))
+    h = tuple(float(d[k]) for k in ("dx", "dy", "dz"))
+    F = [np.asfortranarray(d[k]) for k in ("fx", "fy", "fz")]
+    for name, fn in (("dive", orc.calc_field_div), ("divb", orc.calc_field_divb)):
+        out = np.asfortranarray(np.full(F[0].shape, 7.0))
+        fn(out, *F, nx, ny, nz, g, g, g, *h)
+        np.testing.assert_array_equal(out, d[name])

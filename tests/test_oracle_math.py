@@ -285,3 +285,66 @@ def test_laser_pusher_moves_by_the_profile(orc):
             np.testing.assert_array_equal(u[d], -1.0 * amp[:, d] * (0.01 / emax))
             np.testing.assert_array_equal(x[d], x0[d] + 0.5 * u[d])
         np.testing.assert_array_equal(g, 1.0)
+
+
+def _interior(a, lo, hi):
+    """nodes lo..hi[d] of each axis d of a grid array"""
+    return a[NG + lo:NG + hi[0] + 1, NG + lo:NG + hi[1] + 1, NG + lo:NG + hi[2] + 1]
+
+
+def test_divergence_of_a_yee_curl_vanishes(orc):
+    """div E by backward differences (calc_field_div) is unchanged by the Yee
+    E push without current, and div B by forward differences
+    (calc_field_divB) by the B push: the discrete divergence of the discrete
+    curl is zero.  A difference taken the other way round breaks both."""
+    rng = np.random.default_rng(5)
+    d = (0.9, 1.1, 1.3)
+    dt = 0.4
+    F = {k: np.asfortranarray(rng.standard_normal(SHAPE)) for k in
+         ("ex", "ey", "ez", "bx", "by", "bz")}
+    J = [zeros() for _ in range(3)]
+    lo, hi = (0, 0, 0), (NX, NY, NZ)
+    before, after = zeros(), zeros()
+    orc.calc_field_div(before, F["ex"], F["ey"], F["ez"], NX, NY, NZ, NG, NG, NG, *d)
+    orc.push_em3d_evec(lo, hi, *(F[k] for k in ("ex", "ey", "ez", "bx", "by", "bz")), *J, dt,
+                       dt / d[0], dt / d[1], dt / d[2], NX, NY, NZ, NG, NG, NG)
+    orc.calc_field_div(after, F["ex"], F["ey"], F["ez"], NX, NY, NZ, NG, NG, NG, *d)
+    # E moved at the nodes 0..n: div E at 1..n sees only updated values
+    diff = _interior(after - before, 1, (NX, NY, NZ))
+    assert np.abs(diff).max() < 1e-13 * np.abs(_interior(before, 1, (NX, NY, NZ))).max()
+    assert np.abs(_interior(after - before, 0, (NX, NY, NZ))).max() > 1e-3  # guards not moved
+    orc.calc_field_divb(before, F["bx"], F["by"], F["bz"], NX, NY, NZ, NG, NG, NG, *d)
+    orc.push_em3d_bvec(lo, hi, *(F[k] for k in ("ex", "ey", "ez", "bx", "by", "bz")),
+                       dt / d[0], dt / d[1], dt / d[2], NX, NY, NZ, NG, NG, NG)
+    orc.calc_field_divb(after, F["bx"], F["by"], F["bz"], NX, NY, NZ, NG, NG, NG, *d)
+    m = (NX - 1, NY - 1, NZ - 1)  # div B at n reads the guard node n+1
+    diff = _interior(after - before, 1, m)
+    assert np.abs(diff).max() < 1e-13 * np.abs(_interior(before, 1, m)).max()
+
+
+def test_divergence_of_linear_fields(orc):
+    """Fields linear in the node index have divergence a/dx + b/dy + c/dz at
+    every node of the loops; the nodes outside keep their value."""
+    d = (0.9, 1.1, 1.3)
+    i, j, k = node_coords()
+    a, b, c = 0.7, -1.9, 2.3
+    fx, fy, fz = (np.asfortranarray(v * 1.0) for v in (a * i, b * j, c * k))
+    want = a / d[0] + b / d[1] + c / d[2]
+    for fn, lo in ((orc.calc_field_div, 0), (orc.calc_field_divb, 1)):
+        out = np.asfortranarray(np.full(SHAPE, 3.5))
+        fn(out, fx, fy, fz, NX, NY, NZ, NG, NG, NG, *d)
+        inner = _interior(out, lo, (NX, NY, NZ))
+        np.testing.assert_allclose(inner, want, rtol=1e-13)
+        assert (out == 3.5).sum() == out.size - inner.size
+
+
+def test_divergence_norms(orc):
+    """get_norm_divErho and get_loc_norm_2 (diags.F90:1222-1345) against
+    numpy sums over the nodes 1..n of each axis."""
+    rng = np.random.default_rng(8)
+    dv, rho = (np.asfortranarray(rng.standard_normal(SHAPE)) for _ in range(2))
+    s = (slice(NG + 1, NG + NX + 1), slice(NG + 1, NG + NY + 1), slice(NG + 1, NG + NZ + 1))
+    want = np.sqrt(((dv[s] - rho[s]) ** 2).sum())
+    assert orc.norm_diverho(dv, rho, NX, NY, NZ, NG, NG, NG) == pytest.approx(want, rel=1e-14)
+    assert orc.loc_norm_2(dv, NX, NY, NZ, NG, NG, NG) == pytest.approx((dv[s] ** 2).sum(),
+                                                                       rel=1e-14)
