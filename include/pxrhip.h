@@ -367,8 +367,17 @@ int pxrhip_tiles_rebin_sort(const pxrhip_grid *grid, pxrhip_species *src,
  * (LDS current of a mixed-precision species: 0 FP32 atomics, 1 FP32
  * atomics compensated by TwoSum into a second array, 2 FP64 atomics),
  * "mixed.lds_bytes" and "mixed.wg_per_cu" (the launch of a mixed-precision
- * species; fused.lds_bytes and fused.wg_per_cu are those of an FP64 one).  Work items whose field or current block exceeds
- * its LDS budget are completed from global memory by a second kernel. */
+ * species; fused.lds_bytes and fused.wg_per_cu are those of an FP64 one).
+ * wg_per_cu 0 (the default) is automatic: three workgroups per CU with
+ * 52224 bytes each when a work item of four sorting rows of the widest tile
+ * fits that, else two with 79616 bytes; a wg_per_cu above 0 takes the
+ * matching lds_bytes (default 52224).  An FP64 species at order 3 with the
+ * direct deposition runs at most two workgroups per CU, since its kernel
+ * holds registers for two waves per SIMD; it then takes 2 and 79616 bytes.
+ * pxrhip_get_tuning also reads "last.wg_per_cu" and "last.lds_bytes", the
+ * launch of the last pxrhip_tiles_gather_push_deposit.  Work items whose
+ * field or current block exceeds its LDS budget are completed from global
+ * memory by a second kernel. */
 int pxrhip_set_tuning(const char *key, int64_t value);
 int pxrhip_get_tuning(const char *key, int64_t *value);
 

@@ -9,7 +9,8 @@ and the launch of the persistent kernel are set by the keys of
 * ``fused.chunk``     -- particles per work item (rows of the sorting grid
   are added while the LDS blocks fit);
 * ``fused.wg_per_cu`` with ``fused.lds_bytes`` -- workgroups per CU and the
-  LDS each may use (2 x 79616, 1 x 155648, 3 x 52224 bytes);
+  LDS each may use (2 x 79616, 1 x 155648, 3 x 52224 bytes; 0 workgroups:
+  the library's automatic choice between the first and the last);
 * ``fused.spread``    -- lanes on particles S apart in the deposition.
 
 ``autotune`` times the gather + push + deposit stage of a Simulation for
@@ -34,7 +35,8 @@ DEFAULT_CANDIDATES: List[Dict[str, int]] = [
              (ch, wg, lds, sp)))
     for ch, (wg, lds), sp in itertools.product((1024, 2048, 4096, 8192),
                                                ((2, 79616), (1, 155648), (3, 52224)), (1,))
-] + [{"fused.chunk": 4096, "fused.wg_per_cu": 2, "fused.lds_bytes": 79616, "fused.spread": 0}]
+] + [{"fused.chunk": 4096, "fused.wg_per_cu": 2, "fused.lds_bytes": 79616, "fused.spread": 0},
+      {"fused.chunk": 4096, "fused.wg_per_cu": 0, "fused.lds_bytes": 52224, "fused.spread": 1}]
 
 
 def get_tuning(keys: Iterable[str] = KEYS) -> Dict[str, int]:

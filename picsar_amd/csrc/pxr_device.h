@@ -10,14 +10,15 @@
 #include <stdint.h>
 
 #define PXR_DEV __device__ __forceinline__
+#define PXR_HD __host__ __device__ __forceinline__
 
 namespace pxr {
 
 typedef int64_t i64;
 
 // Fortran NINT (round half away from zero) and FLOOR.
-PXR_DEV int f_nint(double x) { return (int)round(x); }
-PXR_DEV int f_floor(double x) { return (int)floor(x); }
+PXR_HD int f_nint(double x) { return (int)round(x); }
+PXR_HD int f_floor(double x) { return (int)floor(x); }
 PXR_DEV int f_nint(float x) { return (int)roundf(x); }
 PXR_DEV int f_floor(float x) { return (int)floorf(x); }
 

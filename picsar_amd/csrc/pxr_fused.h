@@ -33,7 +33,7 @@ struct BoxGeom {
   const int *n3;         // sorting cells of every tile
 };
 
-PXR_DEV void gather_range(const BoxGeom &b, double ulo, double uhi, int &lo, int &hi) {
+PXR_HD void gather_range(const BoxGeom &b, double ulo, double uhi, int &lo, int &hi) {
   const int N = b.N, M = b.M;
   auto jn = [&](double v) { return (N == 2 ? f_nint(v) : f_floor(v)) - (N == 1 ? 0 : 1); };
   auto js = [&](double v) {
@@ -44,7 +44,7 @@ PXR_DEV void gather_range(const BoxGeom &b, double ulo, double uhi, int &lo, int
   hi = max(jn(uhi) + N, js(uhi) + M);
 }
 
-PXR_DEV void deposit_range(const BoxGeom &b, double ulo, double uhi, double m, int &lo, int &hi) {
+PXR_HD void deposit_range(const BoxGeom &b, double ulo, double uhi, double m, int &lo, int &hi) {
   const int N = b.N;
   if (b.depo == 0) {  // Esirkepov: old and new node index, support ESup<N>
     auto ei = [&](double v) { return N == 2 ? f_nint(v) : f_floor(v); };
@@ -74,7 +74,7 @@ PXR_DEV void rows_range(const BoxGeom &b, int t, int z0, int z1, int y0, int y1,
 
 // field block (FR0, fe) and current block (JR0, je) for that range, clamped
 // to the arrays; returns the doubles they take in LDS (6 + 3 arrays)
-PXR_DEV i64 blocks_for(const BoxGeom &b, const double *ulo, const double *uhi, int *FR0, int *fe,
+PXR_HD i64 blocks_for(const BoxGeom &b, const double *ulo, const double *uhi, int *FR0, int *fe,
                        int *JR0, int *je) {
   for (int d = 0; d < 3; d++) {
     int lo, hi;
@@ -148,12 +148,19 @@ PXR_DEV void wave_minmax(Box6 &b) {
   }
 }
 
+// Current accumulators: add(c, i, j, k, v) adds v to node (i, j, k) of
+// component c; row(c, i, j, k) is the address of that node, and
+// add_row(row, r, v) adds v to the node r further along x.
 struct LdsAcc {
   double *p[3];
   int i0, j0, k0, sy, sz;
   PXR_DEV void add(int c, int i, int j, int k, double v) const {
     atomicAdd(p[c] + ((i - i0) + sy * (j - j0) + sz * (k - k0)), v);
   }
+  PXR_DEV double *row(int c, int i, int j, int k) const {
+    return p[c] + ((i - i0) + sy * (j - j0) + sz * (k - k0));
+  }
+  PXR_DEV void add_row(double *r, int o, double v) const { atomicAdd(r + o, v); }
 };
 
 struct GAcc {
@@ -162,6 +169,8 @@ struct GAcc {
   PXR_DEV void add(int c, int i, int j, int k, double v) const {
     unsafeAtomicAdd(p[c] + g.idx(i, j, k), v);
   }
+  PXR_DEV double *row(int c, int i, int j, int k) const { return p[c] + g.idx(i, j, k); }
+  PXR_DEV void add_row(double *r, int o, double v) const { unsafeAtomicAdd(r + o, v); }
 };
 
 PXR_DEV bool in_block(const int *lo, const int *hi, const int *R0, const int *e) {
@@ -372,8 +381,12 @@ PXR_DEV bool gpd_deposit_mx(const FusedArgs &a, i64 p, const Acc &acc) {
 // A work item whose field block does not fit in LDS is left whole to
 // k_fused_overflow; one whose current block does not fit is left to it
 // after the push.
+// Registers are capped for three waves per SIMD (three workgroups of 256
+// lanes per CU), except for the order-3 direct deposition of an FP64
+// species, which would spill and keeps two.
 template <int BS, int N, bool LO, int PUSHER, int DEPO, bool MX, int JACC>
-__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void k_fused(FusedArgs a) {
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(
+    (N == 3 && DEPO == 3 && !MX) ? 2 : 3))) void k_fused(FusedArgs a) {
   typedef typename std::conditional<MX, float, double>::type FT;  // field block
   // current block: FP64, or per the mixed-precision accumulation mode
   typedef typename std::conditional<MX && JACC != 2, float, double>::type JT;

@@ -322,6 +322,8 @@ PXR_DEV void esirkepov_deposit(const EsirkepovIn<T> &in, const Acc &acc) {
     hz[r] = Z.sn[r] + T(0.5) * dz_[r];
     tz[r] = T(0.5) * Z.sn[r] + third * dz_[r];
   }
+  // Every component runs over rows of x nodes, so that a row's address is
+  // formed once and its nodes are constant offsets from it.
 #pragma unroll
   for (int rk = 0; rk < W; rk++) {
     if (rk >= Z.w) continue;
@@ -329,33 +331,40 @@ PXR_DEV void esirkepov_deposit(const EsirkepovIn<T> &in, const Acc &acc) {
     for (int rj = 0; rj < W; rj++) {
       if (rj >= Y.w) continue;
       const T wx = hy[rj] * Z.sn[rk] + ty[rj] * dz_[rk];
+      const auto row = acc.row(0, X.base, Y.base + rj, Z.base + rk);
 #pragma unroll
       for (int ri = 0; ri < W - 1; ri++)
-        if (ri < X.w - 1) acc.add(0, X.base + ri, Y.base + rj, Z.base + rk, cx[ri] * wx);
+        if (ri < X.w - 1) acc.add_row(row, ri, cx[ri] * wx);
     }
   }
 #pragma unroll
   for (int rk = 0; rk < W; rk++) {
     if (rk >= Z.w) continue;
+    T wy[W];
 #pragma unroll
-    for (int ri = 0; ri < W; ri++) {
-      if (ri >= X.w) continue;
-      const T wy = hz[rk] * X.sn[ri] + tz[rk] * dx_[ri];
+    for (int ri = 0; ri < W; ri++) wy[ri] = hz[rk] * X.sn[ri] + tz[rk] * dx_[ri];
 #pragma unroll
-      for (int rj = 0; rj < W - 1; rj++)
-        if (rj < Y.w - 1) acc.add(1, X.base + ri, Y.base + rj, Z.base + rk, cy[rj] * wy);
+    for (int rj = 0; rj < W - 1; rj++) {
+      if (rj >= Y.w - 1) continue;
+      const auto row = acc.row(1, X.base, Y.base + rj, Z.base + rk);
+#pragma unroll
+      for (int ri = 0; ri < W; ri++)
+        if (ri < X.w) acc.add_row(row, ri, cy[rj] * wy[ri]);
     }
   }
 #pragma unroll
   for (int rj = 0; rj < W; rj++) {
     if (rj >= Y.w) continue;
+    T wz[W];
 #pragma unroll
-    for (int ri = 0; ri < W; ri++) {
-      if (ri >= X.w) continue;
-      const T wz = hx[ri] * Y.sn[rj] + tx[ri] * dy_[rj];
+    for (int ri = 0; ri < W; ri++) wz[ri] = hx[ri] * Y.sn[rj] + tx[ri] * dy_[rj];
 #pragma unroll
-      for (int rk = 0; rk < W - 1; rk++)
-        if (rk < Z.w - 1) acc.add(2, X.base + ri, Y.base + rj, Z.base + rk, cz[rk] * wz);
+    for (int rk = 0; rk < W - 1; rk++) {
+      if (rk >= Z.w - 1) continue;
+      const auto row = acc.row(2, X.base, Y.base + rj, Z.base + rk);
+#pragma unroll
+      for (int ri = 0; ri < W; ri++)
+        if (ri < X.w) acc.add_row(row, ri, cz[rk] * wz[ri]);
     }
   }
 }

@@ -232,6 +232,8 @@ struct GlobalAcc {
   PXR_DEV void add(int c, int i, int j, int k, double v) const {
     unsafeAtomicAdd(p[c] + (i + sy * j + sz * k), v);
   }
+  PXR_DEV double *row(int c, int i, int j, int k) const { return p[c] + (i + sy * j + sz * k); }
+  PXR_DEV void add_row(double *r, int o, double v) const { unsafeAtomicAdd(r + o, v); }
 };
 
 template <int N>

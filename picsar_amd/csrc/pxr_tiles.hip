@@ -663,10 +663,34 @@ __global__ __launch_bounds__(256) void k_chunk_write(ChunkArgs a) {
 // C ABI
 // ===========================================================================
 namespace pxr {
+// dynamic LDS per workgroup when two or three workgroups (each with its
+// 2 KiB of static LDS) share a CU's 160 KiB
+constexpr int LDS_TWO_WG = 79616;
+constexpr int LDS_THREE_WG = 52224;
+
+// LDS that a work item of `rows` sorting rows in one plane of the widest
+// tile needs: the larger of its field and current blocks, counted as
+// blocks_fit counts them, for rows that start half a cell off the grid
+// nodes, as tile edges do
+static i64 rows_lds_bytes(const BoxGeom &bg, const TileTable &tt, int rows, int fnode,
+                          int jnode) {
+  const double w[3] = {tt.nxmax * bg.sd[0] * bg.di[0], rows * bg.sd[1] * bg.di[1],
+                       bg.sd[2] * bg.di[2]};
+  double ulo[3], uhi[3];
+  for (int d = 0; d < 3; d++) {
+    ulo[d] = -0.5 - 1e-7;
+    uhi[d] = -0.5 + w[d] + 1e-7;
+  }
+  int FR0[3], fe[3], JR0[3], je[3];
+  blocks_for(bg, ulo, uhi, FR0, fe, JR0, je);
+  const int j0 = je[0] + 2, j1 = je[1] + 2, j2 = je[2] + 2;
+  const i64 jpz = ((i64)(j0 | 1) * j1) | 1;
+  const i64 fb = (i64)fnode * fe[0] * fe[1] * fe[2], jb = (i64)jnode * jpz * j2;
+  return fb > jb ? fb : jb;
+}
 struct Tuning {
-  int lds_bytes = 79616;  // dynamic LDS per workgroup: two workgroups (with their 2 KiB
-                          // of static LDS) fit in a CU's 160 KiB
-  int wg_per_cu = 2;      // persistent workgroups per CU
+  int lds_bytes = LDS_THREE_WG;  // dynamic LDS per workgroup (with wg_per_cu > 0)
+  int wg_per_cu = 0;             // persistent workgroups per CU; 0: automatic
   int spread = 1;         // lane-spread mapping in the deposit phase
   int chunk = 4096;       // default particles per work item
   int j_lds_bytes = -1;   // cap on the LDS current block (negative: all of lds_bytes)
@@ -676,10 +700,12 @@ struct Tuning {
   int jacc = 2;           // mixed precision LDS current: 0 FP32, 1 compensated FP32, 2 FP64
   // launch of a mixed-precision species: its kernels take fewer registers,
   // so three workgroups with smaller LDS blocks fit on a CU
-  int mx_lds_bytes = 52224;
-  int mx_wg_per_cu = 3;
+  int mx_lds_bytes = LDS_THREE_WG;
+  int mx_wg_per_cu = 0;
 };
 static Tuning g_tune;
+// the launch chosen by the last pxrhip_tiles_gather_push_deposit
+static int g_last_wg_per_cu = 0, g_last_lds_bytes = 0;
 }  // namespace pxr
 
 extern "C" {
@@ -700,7 +726,8 @@ int pxrhip_set_tuning(const char *key, int64_t value) {
       return set_error(PXRHIP_ERR_ARG, "fused.lds_bytes must be in [4096, 159744]");
     g_tune.lds_bytes = (int)(value & ~7);
   } else if (!strcmp(key, "fused.wg_per_cu")) {
-    if (value < 1 || value > 16) return set_error(PXRHIP_ERR_ARG, "fused.wg_per_cu must be in [1, 16]");
+    if (value < 0 || value > 16)
+      return set_error(PXRHIP_ERR_ARG, "fused.wg_per_cu must be in [0, 16] (0: automatic)");
     g_tune.wg_per_cu = (int)value;
   } else if (!strcmp(key, "fused.spread")) {
     g_tune.spread = value != 0;
@@ -711,7 +738,8 @@ int pxrhip_set_tuning(const char *key, int64_t value) {
       return set_error(PXRHIP_ERR_ARG, "mixed.lds_bytes must be in [4096, 159744]");
     g_tune.mx_lds_bytes = (int)(value & ~7);
   } else if (!strcmp(key, "mixed.wg_per_cu")) {
-    if (value < 1 || value > 16) return set_error(PXRHIP_ERR_ARG, "mixed.wg_per_cu must be in [1, 16]");
+    if (value < 0 || value > 16)
+      return set_error(PXRHIP_ERR_ARG, "mixed.wg_per_cu must be in [0, 16] (0: automatic)");
     g_tune.mx_wg_per_cu = (int)value;
   } else if (!strcmp(key, "mixed.jacc")) {
     if (value < 0 || value > 2) return set_error(PXRHIP_ERR_ARG, "mixed.jacc must be 0, 1 or 2");
@@ -746,6 +774,8 @@ int pxrhip_get_tuning(const char *key, int64_t *value) {
   else if (!strcmp(key, "mixed.jacc")) *value = g_tune.jacc;
   else if (!strcmp(key, "mixed.lds_bytes")) *value = g_tune.mx_lds_bytes;
   else if (!strcmp(key, "mixed.wg_per_cu")) *value = g_tune.mx_wg_per_cu;
+  else if (!strcmp(key, "last.wg_per_cu")) *value = g_last_wg_per_cu;
+  else if (!strcmp(key, "last.lds_bytes")) *value = g_last_lds_bytes;
   else return set_error(PXRHIP_ERR_ARG, "unknown tuning key '%s'", key);
   return PXRHIP_OK;
 }
@@ -919,12 +949,26 @@ int pxrhip_tiles_gather_push_deposit(const pxrhip_grid *grid, const pxrhip_speci
     bg.n3 = tt->n3;
   }
   const bool mx = sp->precision == 1;
-  const i64 lds_bytes = mx ? g_tune.mx_lds_bytes : g_tune.lds_bytes;
-  const i64 j_bytes = g_tune.j_lds_bytes < 0 || g_tune.j_lds_bytes > lds_bytes ? lds_bytes
-                                                                              : g_tune.j_lds_bytes;
   const int esize = mx ? 4 : 8;
   const int jacc = mx ? g_tune.jacc : 2;  // FP64 species: FP64 LDS current
   const int jnode = jacc == 0 ? 12 : 24;  // 3 FP32 | 3 FP32 + 3 compensations | 3 FP64
+  i64 lds_bytes = mx ? g_tune.mx_lds_bytes : g_tune.lds_bytes;
+  int wg_per_cu = mx ? g_tune.mx_wg_per_cu : g_tune.wg_per_cu;
+  if (wg_per_cu == 0) {
+    // automatic: three workgroups per CU when a work item of four sorting
+    // rows of the widest tile fits their LDS, else two with more each
+    const bool three = rows_lds_bytes(bg, *tt, 4, 6 * esize, jnode) <= LDS_THREE_WG;
+    wg_per_cu = three ? 3 : 2;
+    lds_bytes = three ? LDS_THREE_WG : LDS_TWO_WG;
+  }
+  if (!mx && order == 3 && depo_algo && wg_per_cu > 2) {
+    // that kernel holds registers for two waves per SIMD: two workgroups
+    // per CU, each with the LDS of two
+    wg_per_cu = 2;
+    lds_bytes = LDS_TWO_WG;
+  }
+  const i64 j_bytes = g_tune.j_lds_bytes < 0 || g_tune.j_lds_bytes > lds_bytes ? lds_bytes
+                                                                              : g_tune.j_lds_bytes;
   const i64 nunits = (i64)tt->ntiles * nzmax;
   // A plane gives one chunk if it heads a group of whole planes, none if it
   // lies inside one, else its row walk: a chunk per row that opens one
@@ -1014,7 +1058,9 @@ int pxrhip_tiles_gather_push_deposit(const pxrhip_grid *grid, const pxrhip_speci
   a.ovf_count = ctl + 2;
   a.ovf = ovf;
   a.ovf_cap = maxchunks < INT_MAX ? (int)maxchunks : INT_MAX;
-  const int nwg = rt().num_cus * (mx ? g_tune.mx_wg_per_cu : g_tune.wg_per_cu);
+  const int nwg = rt().num_cus * wg_per_cu;
+  g_last_wg_per_cu = wg_per_cu;
+  g_last_lds_bytes = (int)lds_bytes;
   const size_t lds = (size_t)lds_bytes;
   const int key = (int)(order * 100 + (l_lower_order_in_v ? 10 : 0) + pusher * 2 + (depo_algo ? 1 : 0));
   if (!mx) return launch_fused_fp64(key, a, nwg, lds, st);

@@ -305,7 +305,10 @@ def test_chunking_does_not_change_results(torch, orc, chunk):
 
 
 @pytest.mark.parametrize("tune", [
-    {"fused.lds_bytes": 4096},        # no field block fits: gather and deposit from global memory
+    # no field block fits: gather and deposit from global memory
+    {"fused.wg_per_cu": 2, "fused.lds_bytes": 4096},
+    {"fused.wg_per_cu": 2, "fused.lds_bytes": 79616},
+    {"fused.wg_per_cu": 3, "fused.lds_bytes": 52224},
     {"fused.j_lds_bytes": 0},         # fields in LDS, every current block deposited globally
     {"fused.j_lds_bytes": 6000},      # some current blocks in LDS, the others globally
     {"fused.spread": 0},
@@ -331,6 +334,34 @@ def test_overflow_and_variants_match(torch, orc, tune, depo):
     finally:
         for k, v in saved.items():
             _lib.call("pxrhip_set_tuning", k.encode(), v)
+    compare_fields(sim, ref, tng, 1e-12)
+    compare_particles(sim, ref, 1e-12 * sim.nx * sim.dx, 1e-12)
+
+
+def _last_launch():
+    from picsar_amd import _lib
+    import ctypes as C
+    out = []
+    for k in ("last.wg_per_cu", "last.lds_bytes"):
+        v = C.c_int64(0)
+        _lib.call("pxrhip_get_tuning", k.encode(), C.byref(v))
+        out.append(v.value)
+    return tuple(out)
+
+
+@pytest.mark.parametrize("depo,n,ntiles,want", [
+    ("esirkepov", (16, 14, 12), (2, 2, 2), (3, 52224)),   # narrow tiles: three workgroups
+    ("esirkepov", (40, 12, 12), (1, 2, 2), (2, 79616)),   # a 40-cell-wide tile: two
+    ("direct", (16, 14, 12), (2, 2, 2), (2, 79616)),      # order-3 direct FP64: two
+])
+def test_automatic_launch_choice(torch, orc, depo, n, ntiles, want):
+    """fused.wg_per_cu = 0 picks three workgroups per CU when a four-row
+    work item of the widest tile fits 52224 bytes, else two; the step is the
+    oracle's either way."""
+    sim, ref, tng = build(torch, orc, 3, True, "vay", depo, n=n, ntiles=ntiles)
+    sim.step(1)
+    ref.step()
+    assert _last_launch() == want
     compare_fields(sim, ref, tng, 1e-12)
     compare_particles(sim, ref, 1e-12 * sim.nx * sim.dx, 1e-12)
 
