@@ -169,10 +169,9 @@ def build_workload(args):
 
 def gauss_state(sim):
     """div E - rho on the nodes 0..n-1 of each axis (host copy) and the norm
-    of rho there, or None where rho is not deposited here (mixed-precision
-    species) or where the discrete Gauss law is not that of the solver
-    (FDTD order above 2)."""
-    if sim.norder != 2 or any(sp.precision != "fp64" for sp in sim.species):
+    of rho there, or None where the discrete Gauss law is not that of the
+    solver (FDTD order above 2)."""
+    if sim.norder != 2:
         return None
     sim.gauss_residual()
     g = sim.ng

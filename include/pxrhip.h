@@ -76,6 +76,9 @@ int pxrhip_memcpy_dtoh(void *dst, const void *src, int64_t bytes);
 int pxrhip_memcpy_dtod(void *dst, const void *src, int64_t bytes);
 int pxrhip_fill_f64(double *p, int64_t n, double value);
 int pxrhip_fill_i64(int64_t *p, int64_t n, int64_t value);
+/* dst[i] = (double)src[i]: the weights of a mixed-precision species for
+ * routines that take FP64 arrays (the charge density) */
+int pxrhip_convert_f32_f64(const float *src, double *dst, int64_t n);
 /* HIP events on the library stream (slots 0..65535), the device
  * counterpart of the reference's localtimes() wall-clock slots
  * (modules.F90:918-941) */

@@ -40,6 +40,7 @@ SIGNATURES = {
     "pxrhip_memcpy_dtod": [_P, _P, _I],
     "pxrhip_fill_f64": [_P, _I, _F],
     "pxrhip_fill_i64": [_P, _I, _I],
+    "pxrhip_convert_f32_f64": [_P, _P, _I],
     "pxrhip_timer_record": [_i32],
     "pxrhip_timer_elapsed": [_i32, _i32, C.POINTER(C.c_double)],
     "pxrhip_kinetic_energy": [_I, _P, _P, C.POINTER(C.c_double)],

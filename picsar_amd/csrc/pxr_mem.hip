@@ -23,6 +23,11 @@ __global__ __launch_bounds__(256) void k_fill_i64(int64_t *p, i64 n, int64_t v) 
   if (i < n) p[i] = v;
 }
 
+__global__ __launch_bounds__(256) void k_f32_to_f64(const float *s, double *d, i64 n) {
+  const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] = (double)s[i];
+}
+
 // plain device copy in 16-byte words (grid-stride); also the reference
 // point for HBM traffic counters: it reads and writes each byte once
 __global__ __launch_bounds__(256) void k_copy16(uint4 *__restrict__ dst, const uint4 *__restrict__ src,
@@ -222,6 +227,15 @@ int pxrhip_fill_f64(double *p, int64_t n, double value) {
   if (n < 0 || (n && !p)) return set_error(PXRHIP_ERR_ARG, "bad fill");
   if (!n) return PXRHIP_OK;
   hipLaunchKernelGGL(k_fill_f64, dim3(grid1d(n, 256)), dim3(256), 0, rt().stream, p, n, value);
+  PXR_LAUNCHED();
+  return PXRHIP_OK;
+}
+
+int pxrhip_convert_f32_f64(const float *src, double *dst, int64_t n) {
+  PXR_INIT();
+  if (n < 0 || (n && (!src || !dst))) return set_error(PXRHIP_ERR_ARG, "bad conversion");
+  if (!n) return PXRHIP_OK;
+  hipLaunchKernelGGL(k_f32_to_f64, dim3(grid1d(n, 256)), dim3(256), 0, rt().stream, src, dst, n);
   PXR_LAUNCHED();
   return PXRHIP_OK;
 }

@@ -189,6 +189,7 @@ class Simulation:
         self.species: List[Species] = []
         self.antennas = []
         self.diag_fields = {}  # rho, dive, divb: allocated on first use
+        self._w64 = {}  # FP64 copies of mixed-precision weights for the charge density
 
     # ------------------------------------------------------------------
     # particles
@@ -489,7 +490,8 @@ class Simulation:
         """rho = 0, then the charge density of every species that deposits
         (frozen ones included) and of the laser antennas, which the
         reference holds as species that deposit (tiling.F90:1684-1697), at
-        the simulation's shape order."""
+        the simulation's shape order.  The FP32 weights of a mixed-precision
+        species are widened to FP64 first."""
         rho = self.diag_grid("rho")
         rho.fill(0.0)
         g = self.ng
@@ -497,10 +499,14 @@ class Simulation:
         for sp in self.species:
             if not sp.ldodepos or sp.npart == 0:
                 continue
-            if sp.precision != "fp64":
-                raise NotImplementedError("the charge density is deposited from FP64 species")
             a = sp.active.arrays
-            parts.append((sp.npart, a["x"], a["y"], a["z"], a["w"], sp.q))
+            w = a["w"]
+            if sp.precision != "fp64":  # FP32 weights, converted on the device
+                w = self._w64.get(sp.name)
+                if w is None or w.n < sp.npart:
+                    w = self._w64[sp.name] = DeviceArray(sp.npart)
+                _lib.call("pxrhip_convert_f32_f64", a["w"].ptr, w.ptr, sp.npart)
+            parts.append((sp.npart, a["x"], a["y"], a["z"], w, sp.q))
         for ant in self.antennas:
             if ant.npart:  # pid column 1 holds the weights
                 parts.append((ant.npart, ant.x, ant.y, ant.z, ant.pid, ant.charge))
