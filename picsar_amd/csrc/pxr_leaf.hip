@@ -409,76 +409,123 @@ struct NorderArgs {
 
 PXR_DEV int imin3(int a, int b, int c) { return min(min(a, b), c); }
 
-// yee.F90:46-133: E update with stencils cut at the guard region
-__global__ __launch_bounds__(256) void k_norder_e(NorderArgs a) {
+// Arbitrary-order FDTD, yee.F90:46-133 (E) and 752-833 (B), with stencils
+// cut at the guard region.
+// v +/- sum_i cf[i-1] * (f[c + (i - a) s] - f[c + (b - i) s]), term by term
+// in the reference's order.  NH > 0: the full stencil of NH terms, unrolled;
+// NH = 0: the n terms of a stencil cut at the guards.
+template <int NH, bool PLUS>
+PXR_DEV double fd_sum(double v, const double *cf, const double *f, i64 c, i64 s, int a, int b,
+                      int n) {
+  if (NH > 0) {
+#pragma unroll
+    for (int i = 1; i <= NH; i++) {
+      const double t = cf[i - 1] * (f[c + (i - a) * s] - f[c + (b - i) * s]);
+      v = PLUS ? v + t : v - t;
+    }
+  } else {
+    for (int i = 1; i <= n; i++) {
+      const double t = cf[i - 1] * (f[c + (i - a) * s] - f[c + (b - i) * s]);
+      v = PLUS ? v + t : v - t;
+    }
+  }
+  return v;
+}
+
+// node (j, k, l) of the loop box -s..n+s, or false past its end
+PXR_DEV bool norder_node(const NorderArgs &a, int &j, int &k, int &l) {
   const GridGeom &g = a.g;
   const int nxu = g.nx + 2 * a.nxs + 1, nyu = g.ny + 2 * a.nys + 1;
   const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (i64)nxu * nyu * (g.nz + 2 * a.nzs + 1)) return;
-  const int j = -a.nxs + (int)(t % nxu);
-  const int k = -a.nys + (int)((t / nxu) % nyu);
-  const int l = -a.nzs + (int)(t / ((i64)nxu * nyu));
+  if (t >= (i64)nxu * nyu * (g.nz + 2 * a.nzs + 1)) return false;
+  j = -a.nxs + (int)(t % nxu);
+  k = -a.nys + (int)((t / nxu) % nyu);
+  l = -a.nzs + (int)(t / ((i64)nxu * nyu));
+  return true;
+}
+
+// yee.F90:46-133 at one node; nxh..nzh terms per axis
+template <int NH>
+PXR_DEV void norder_e_at(const NorderArgs &a, i64 c, int nxh, int nyh, int nzh) {
   double *ex = a.f[0], *ey = a.f[1], *ez = a.f[2];
   const double *bx = a.f[3], *by = a.f[4], *bz = a.f[5], *jx = a.f[6], *jy = a.f[7], *jz = a.f[8];
-  const i64 c = g.idx(j, k, l);
+  const i64 sy = a.g.NX, sz = a.g.NX * a.g.NY;
   const int ist = a.ist;
-  const int nxh = imin3(a.co.nh[0], (g.nx - j) + g.gx, j + g.gx);
-  const int nyh = imin3(a.co.nh[1], (g.ny - k) + g.gy, k + g.gy);
-  const int nzh = imin3(a.co.nh[2], (g.nz - l) + g.gz, l + g.gz);
   double v = ex[c] - a.mudt * jx[c];
-  for (int i = 1; i <= nyh; i++)
-    v = v + a.co.c[1][i - 1] * (bz[g.idx(j, k + i - ist, l)] - bz[g.idx(j, k - i, l)]);
-  for (int i = 1; i <= nzh; i++)
-    v = v - a.co.c[2][i - 1] * (by[g.idx(j, k, l + i - ist)] - by[g.idx(j, k, l - i)]);
+  v = fd_sum<NH, true>(v, a.co.c[1], bz, c, sy, ist, 0, nyh);
+  v = fd_sum<NH, false>(v, a.co.c[2], by, c, sz, ist, 0, nzh);
   ex[c] = v;
   v = ey[c] - a.mudt * jy[c];
-  for (int i = 1; i <= nxh; i++)
-    v = v - a.co.c[0][i - 1] * (bz[g.idx(j + i - ist, k, l)] - bz[g.idx(j - i, k, l)]);
-  for (int i = 1; i <= nzh; i++)
-    v = v + a.co.c[2][i - 1] * (bx[g.idx(j, k, l + i - ist)] - bx[g.idx(j, k, l - i)]);
+  v = fd_sum<NH, false>(v, a.co.c[0], bz, c, 1, ist, 0, nxh);
+  v = fd_sum<NH, true>(v, a.co.c[2], bx, c, sz, ist, 0, nzh);
   ey[c] = v;
   v = ez[c] - a.mudt * jz[c];
-  for (int i = 1; i <= nxh; i++)
-    v = v + a.co.c[0][i - 1] * (by[g.idx(j + i - ist, k, l)] - by[g.idx(j - i, k, l)]);
-  for (int i = 1; i <= nyh; i++)
-    v = v - a.co.c[1][i - 1] * (bx[g.idx(j, k + i - ist, l)] - bx[g.idx(j, k - i, l)]);
+  v = fd_sum<NH, true>(v, a.co.c[0], by, c, 1, ist, 0, nxh);
+  v = fd_sum<NH, false>(v, a.co.c[1], bx, c, sy, ist, 0, nyh);
   ez[c] = v;
 }
 
-// yee.F90:752-833
-__global__ __launch_bounds__(256) void k_norder_b(NorderArgs a) {
-  const GridGeom &g = a.g;
-  const int nxu = g.nx + 2 * a.nxs + 1, nyu = g.ny + 2 * a.nys + 1;
-  const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (i64)nxu * nyu * (g.nz + 2 * a.nzs + 1)) return;
-  const int j = -a.nxs + (int)(t % nxu);
-  const int k = -a.nys + (int)((t / nxu) % nyu);
-  const int l = -a.nzs + (int)(t / ((i64)nxu * nyu));
+// yee.F90:752-833 at one node
+template <int NH>
+PXR_DEV void norder_b_at(const NorderArgs &a, i64 c, int nxh, int nyh, int nzh) {
   const double *ex = a.f[0], *ey = a.f[1], *ez = a.f[2];
   double *bx = a.f[3], *by = a.f[4], *bz = a.f[5];
-  const i64 c = g.idx(j, k, l);
+  const i64 sy = a.g.NX, sz = a.g.NX * a.g.NY;
   const int ist = a.ist;
+  double v = bx[c];
+  v = fd_sum<NH, false>(v, a.co.c[1], ez, c, sy, 0, ist, nyh);
+  v = fd_sum<NH, true>(v, a.co.c[2], ey, c, sz, 0, ist, nzh);
+  bx[c] = v;
+  v = by[c];
+  v = fd_sum<NH, true>(v, a.co.c[0], ez, c, 1, 0, ist, nxh);
+  v = fd_sum<NH, false>(v, a.co.c[2], ex, c, sz, 0, ist, nzh);
+  by[c] = v;
+  v = bz[c];
+  v = fd_sum<NH, false>(v, a.co.c[0], ey, c, 1, 0, ist, nxh);
+  v = fd_sum<NH, true>(v, a.co.c[1], ex, c, sy, 0, ist, nyh);
+  bz[c] = v;
+}
+
+// One thread per node.  Where every axis has its full stencil of NH terms
+// (NH > 0: all nodes but those within NH of the array ends) the unrolled
+// form runs, else the stencil is cut at the guards.
+template <int NH, bool E>
+__global__ __launch_bounds__(256) void k_norder(NorderArgs a) {
+  int j, k, l;
+  if (!norder_node(a, j, k, l)) return;
+  const GridGeom &g = a.g;
+  const i64 c = g.idx(j, k, l);
   const int nxh = imin3(a.co.nh[0], (g.nx - j) + g.gx, j + g.gx);
   const int nyh = imin3(a.co.nh[1], (g.ny - k) + g.gy, k + g.gy);
   const int nzh = imin3(a.co.nh[2], (g.nz - l) + g.gz, l + g.gz);
-  double v = bx[c];
-  for (int i = 1; i <= nyh; i++)
-    v = v - a.co.c[1][i - 1] * (ez[g.idx(j, k + i, l)] - ez[g.idx(j, k - i + ist, l)]);
-  for (int i = 1; i <= nzh; i++)
-    v = v + a.co.c[2][i - 1] * (ey[g.idx(j, k, l + i)] - ey[g.idx(j, k, l - i + ist)]);
-  bx[c] = v;
-  v = by[c];
-  for (int i = 1; i <= nxh; i++)
-    v = v + a.co.c[0][i - 1] * (ez[g.idx(j + i, k, l)] - ez[g.idx(j - i + ist, k, l)]);
-  for (int i = 1; i <= nzh; i++)
-    v = v - a.co.c[2][i - 1] * (ex[g.idx(j, k, l + i)] - ex[g.idx(j, k, l - i + ist)]);
-  by[c] = v;
-  v = bz[c];
-  for (int i = 1; i <= nxh; i++)
-    v = v - a.co.c[0][i - 1] * (ey[g.idx(j + i, k, l)] - ey[g.idx(j - i + ist, k, l)]);
-  for (int i = 1; i <= nyh; i++)
-    v = v + a.co.c[1][i - 1] * (ex[g.idx(j, k + i, l)] - ex[g.idx(j, k - i + ist, l)]);
-  bz[c] = v;
+  if (NH > 0 && nxh == NH && nyh == NH && nzh == NH) {
+    if (E) norder_e_at<NH>(a, c, nxh, nyh, nzh);
+    else norder_b_at<NH>(a, c, nxh, nyh, nzh);
+  } else {
+    if (E) norder_e_at<0>(a, c, nxh, nyh, nzh);
+    else norder_b_at<0>(a, c, nxh, nyh, nzh);
+  }
+}
+
+// the kernel for a stencil of nh terms on every axis (unrolled up to 4)
+static void launch_norder(const NorderArgs &a, bool e, i64 tot) {
+  const int nh = a.co.nh[0];
+  const int u = (nh == a.co.nh[1] && nh == a.co.nh[2] && nh <= 4) ? nh : 0;
+  const dim3 gr(grid1d(tot, 256)), bl(256);
+  hipStream_t st = rt().stream;
+  switch (u * 2 + (e ? 1 : 0)) {
+    case 2: hipLaunchKernelGGL((k_norder<1, false>), gr, bl, 0, st, a); break;
+    case 3: hipLaunchKernelGGL((k_norder<1, true>), gr, bl, 0, st, a); break;
+    case 4: hipLaunchKernelGGL((k_norder<2, false>), gr, bl, 0, st, a); break;
+    case 5: hipLaunchKernelGGL((k_norder<2, true>), gr, bl, 0, st, a); break;
+    case 6: hipLaunchKernelGGL((k_norder<3, false>), gr, bl, 0, st, a); break;
+    case 7: hipLaunchKernelGGL((k_norder<3, true>), gr, bl, 0, st, a); break;
+    case 8: hipLaunchKernelGGL((k_norder<4, false>), gr, bl, 0, st, a); break;
+    case 9: hipLaunchKernelGGL((k_norder<4, true>), gr, bl, 0, st, a); break;
+    default:
+      if (e) hipLaunchKernelGGL((k_norder<0, true>), gr, bl, 0, st, a);
+      else hipLaunchKernelGGL((k_norder<0, false>), gr, bl, 0, st, a);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1099,7 +1146,7 @@ int pxrhip_push_em3d_evec_norder(double *ex, double *ey, double *ez, const doubl
   }
   a.mudt = mudt;
   i64 tot = (i64)(nx + 2 * nxs + 1) * (ny + 2 * nys + 1) * (nz + 2 * nzs + 1);
-  hipLaunchKernelGGL(k_norder_e, dim3(grid1d(tot, 256)), dim3(256), 0, rt().stream, a);
+  launch_norder(a, true, tot);
   PXR_LAUNCHED();
   return PXRHIP_OK;
 }
@@ -1124,7 +1171,7 @@ int pxrhip_push_em3d_bvec_norder(const double *ex, const double *ey, const doubl
   for (int c = 6; c < 9; c++) a.f[c] = nullptr;
   a.mudt = 0.0;
   i64 tot = (i64)(nx + 2 * nxs + 1) * (ny + 2 * nys + 1) * (nz + 2 * nzs + 1);
-  hipLaunchKernelGGL(k_norder_b, dim3(grid1d(tot, 256)), dim3(256), 0, rt().stream, a);
+  launch_norder(a, false, tot);
   PXR_LAUNCHED();
   return PXRHIP_OK;
 }
