@@ -444,52 +444,49 @@ PXR_DEV bool norder_node(const NorderArgs &a, int &j, int &k, int &l) {
   return true;
 }
 
-// yee.F90:46-133 at one node; nxh..nzh terms per axis
-template <int NH>
+// yee.F90:46-133 at one node, with nxh..nzh terms per axis
 PXR_DEV void norder_e_at(const NorderArgs &a, i64 c, int nxh, int nyh, int nzh) {
   double *ex = a.f[0], *ey = a.f[1], *ez = a.f[2];
   const double *bx = a.f[3], *by = a.f[4], *bz = a.f[5], *jx = a.f[6], *jy = a.f[7], *jz = a.f[8];
   const i64 sy = a.g.NX, sz = a.g.NX * a.g.NY;
   const int ist = a.ist;
   double v = ex[c] - a.mudt * jx[c];
-  v = fd_sum<NH, true>(v, a.co.c[1], bz, c, sy, ist, 0, nyh);
-  v = fd_sum<NH, false>(v, a.co.c[2], by, c, sz, ist, 0, nzh);
+  v = fd_sum<0, true>(v, a.co.c[1], bz, c, sy, ist, 0, nyh);
+  v = fd_sum<0, false>(v, a.co.c[2], by, c, sz, ist, 0, nzh);
   ex[c] = v;
   v = ey[c] - a.mudt * jy[c];
-  v = fd_sum<NH, false>(v, a.co.c[0], bz, c, 1, ist, 0, nxh);
-  v = fd_sum<NH, true>(v, a.co.c[2], bx, c, sz, ist, 0, nzh);
+  v = fd_sum<0, false>(v, a.co.c[0], bz, c, 1, ist, 0, nxh);
+  v = fd_sum<0, true>(v, a.co.c[2], bx, c, sz, ist, 0, nzh);
   ey[c] = v;
   v = ez[c] - a.mudt * jz[c];
-  v = fd_sum<NH, true>(v, a.co.c[0], by, c, 1, ist, 0, nxh);
-  v = fd_sum<NH, false>(v, a.co.c[1], bx, c, sy, ist, 0, nyh);
+  v = fd_sum<0, true>(v, a.co.c[0], by, c, 1, ist, 0, nxh);
+  v = fd_sum<0, false>(v, a.co.c[1], bx, c, sy, ist, 0, nyh);
   ez[c] = v;
 }
 
 // yee.F90:752-833 at one node
-template <int NH>
 PXR_DEV void norder_b_at(const NorderArgs &a, i64 c, int nxh, int nyh, int nzh) {
   const double *ex = a.f[0], *ey = a.f[1], *ez = a.f[2];
   double *bx = a.f[3], *by = a.f[4], *bz = a.f[5];
   const i64 sy = a.g.NX, sz = a.g.NX * a.g.NY;
   const int ist = a.ist;
   double v = bx[c];
-  v = fd_sum<NH, false>(v, a.co.c[1], ez, c, sy, 0, ist, nyh);
-  v = fd_sum<NH, true>(v, a.co.c[2], ey, c, sz, 0, ist, nzh);
+  v = fd_sum<0, false>(v, a.co.c[1], ez, c, sy, 0, ist, nyh);
+  v = fd_sum<0, true>(v, a.co.c[2], ey, c, sz, 0, ist, nzh);
   bx[c] = v;
   v = by[c];
-  v = fd_sum<NH, true>(v, a.co.c[0], ez, c, 1, 0, ist, nxh);
-  v = fd_sum<NH, false>(v, a.co.c[2], ex, c, sz, 0, ist, nzh);
+  v = fd_sum<0, true>(v, a.co.c[0], ez, c, 1, 0, ist, nxh);
+  v = fd_sum<0, false>(v, a.co.c[2], ex, c, sz, 0, ist, nzh);
   by[c] = v;
   v = bz[c];
-  v = fd_sum<NH, false>(v, a.co.c[0], ey, c, 1, 0, ist, nxh);
-  v = fd_sum<NH, true>(v, a.co.c[1], ex, c, sy, 0, ist, nyh);
+  v = fd_sum<0, false>(v, a.co.c[0], ey, c, 1, 0, ist, nxh);
+  v = fd_sum<0, true>(v, a.co.c[1], ex, c, sy, 0, ist, nyh);
   bz[c] = v;
 }
 
-// One thread per node.  Where every axis has its full stencil of NH terms
-// (NH > 0: all nodes but those within NH of the array ends) the unrolled
-// form runs, else the stencil is cut at the guards.
-template <int NH, bool E>
+// One thread per node, for stencils of different orders per axis or of
+// more than 4 terms.
+template <bool E>
 __global__ __launch_bounds__(256) void k_norder(NorderArgs a) {
   int j, k, l;
   if (!norder_node(a, j, k, l)) return;
@@ -498,33 +495,138 @@ __global__ __launch_bounds__(256) void k_norder(NorderArgs a) {
   const int nxh = imin3(a.co.nh[0], (g.nx - j) + g.gx, j + g.gx);
   const int nyh = imin3(a.co.nh[1], (g.ny - k) + g.gy, k + g.gy);
   const int nzh = imin3(a.co.nh[2], (g.nz - l) + g.gz, l + g.gz);
-  if (NH > 0 && nxh == NH && nyh == NH && nzh == NH) {
-    if (E) norder_e_at<NH>(a, c, nxh, nyh, nzh);
-    else norder_b_at<NH>(a, c, nxh, nyh, nzh);
-  } else {
-    if (E) norder_e_at<0>(a, c, nxh, nyh, nzh);
-    else norder_b_at<0>(a, c, nxh, nyh, nzh);
-  }
+  if (E) norder_e_at(a, c, nxh, nyh, nzh);
+  else norder_b_at(a, c, nxh, nyh, nzh);
 }
 
-// the kernel for a stencil of nh terms on every axis (unrolled up to 4)
+// Register-blocked form for a full stencil of NH terms on every axis and a
+// staggering shift IST known at compile time.  A thread owns one (x, y)
+// column of the loop box and marches up ZCHUNK planes of it.  The two fields
+// differenced along z (B_y and B_x for E; E_y and E_x for B) are kept as a
+// window of 2 NH + 1 planes in registers, so each plane of them is loaded
+// once; the x and y differences read the plane from memory (neighbouring
+// lanes and the block's rows share those lines in L1/L2).  Nodes whose
+// stencil is cut at the guards take the per-node form.  The terms are
+// summed in the reference's order, so the results equal k_norder's.
+#ifndef ZCHUNK
+#define ZCHUNK 64  // planes per thread
+#endif
+#ifndef ZBX
+#define ZBX 64     // columns of a block along x (256 / ZBX along y)
+#endif
+template <int NH, bool E, int IST>
+__global__ __launch_bounds__(256) void k_norder_z(NorderArgs a) {
+  const GridGeom &g = a.g;
+  const int nxu = g.nx + 2 * a.nxs + 1, nyu = g.ny + 2 * a.nys + 1, nzu = g.nz + 2 * a.nzs + 1;
+  const int ju = (int)blockIdx.x * ZBX + (int)(threadIdx.x % ZBX);
+  const int ku = (int)blockIdx.y * (256 / ZBX) + (int)(threadIdx.x / ZBX);
+  if (ju >= nxu || ku >= nyu) return;
+  const int j = -a.nxs + ju, k = -a.nys + ku;
+  const int l0 = -a.nzs + (int)blockIdx.z * ZCHUNK;
+  const int l1 = min(l0 + ZCHUNK, -a.nzs + nzu);  // exclusive
+  const int nxh = imin3(a.co.nh[0], (g.nx - j) + g.gx, j + g.gx);
+  const int nyh = imin3(a.co.nh[1], (g.ny - k) + g.gy, k + g.gy);
+  // planes whose z stencil is whole: -gz + NH .. nz + gz - NH
+  int m0 = max(l0, -g.gz + NH), m1 = min(l1, g.nz + g.gz - NH + 1);
+  if (nxh != NH || nyh != NH || m0 >= m1) m0 = m1 = l1;  // no marching part
+  auto node = [&](int l) {
+    const int nzh = imin3(a.co.nh[2], (g.nz - l) + g.gz, l + g.gz);
+    if (E) norder_e_at(a, g.idx(j, k, l), nxh, nyh, nzh);
+    else norder_b_at(a, g.idx(j, k, l), nxh, nyh, nzh);
+  };
+  for (int l = l0; l < m0; l++) node(l);
+  if (m0 < m1) {
+    const i64 sy = g.NX, sz = g.NX * g.NY;
+    const double *c0 = a.co.c[0], *c1 = a.co.c[1], *c2 = a.co.c[2];
+    // u, w: the two fields differenced along z, planes l - NH .. l + NH
+    const double *fu = E ? a.f[4] : a.f[1];  // B_y | E_y
+    const double *fw = E ? a.f[3] : a.f[0];  // B_x | E_x
+    double u[2 * NH + 1], w[2 * NH + 1];
+    i64 c = g.idx(j, k, m0);
+#pragma unroll
+    for (int o = 0; o < 2 * NH; o++) {
+      u[o] = fu[c + (o - NH) * sz];
+      w[o] = fw[c + (o - NH) * sz];
+    }
+    for (int l = m0; l < m1; l++, c += sz) {
+      u[2 * NH] = fu[c + NH * sz];
+      w[2 * NH] = fw[c + NH * sz];
+      if (E) {
+        double *ex = a.f[0], *ey = a.f[1], *ez = a.f[2];
+        const double *bx = a.f[3], *by = a.f[4], *bz = a.f[5];
+        const double *jx = a.f[6], *jy = a.f[7], *jz = a.f[8];
+        double v = ex[c] - a.mudt * jx[c];
+        v = fd_sum<NH, true>(v, c1, bz, c, sy, IST, 0, NH);
+#pragma unroll
+        for (int i = 1; i <= NH; i++) v = v - c2[i - 1] * (u[NH + i - IST] - u[NH - i]);
+        ex[c] = v;
+        v = ey[c] - a.mudt * jy[c];
+        v = fd_sum<NH, false>(v, c0, bz, c, 1, IST, 0, NH);
+#pragma unroll
+        for (int i = 1; i <= NH; i++) v = v + c2[i - 1] * (w[NH + i - IST] - w[NH - i]);
+        ey[c] = v;
+        v = ez[c] - a.mudt * jz[c];
+        v = fd_sum<NH, true>(v, c0, by, c, 1, IST, 0, NH);
+        v = fd_sum<NH, false>(v, c1, bx, c, sy, IST, 0, NH);
+        ez[c] = v;
+      } else {
+        const double *ex = a.f[0], *ey = a.f[1], *ez = a.f[2];
+        double *bx = a.f[3], *by = a.f[4], *bz = a.f[5];
+        double v = bx[c];
+        v = fd_sum<NH, false>(v, c1, ez, c, sy, 0, IST, NH);
+#pragma unroll
+        for (int i = 1; i <= NH; i++) v = v + c2[i - 1] * (u[NH + i] - u[NH - i + IST]);
+        bx[c] = v;
+        v = by[c];
+        v = fd_sum<NH, true>(v, c0, ez, c, 1, 0, IST, NH);
+#pragma unroll
+        for (int i = 1; i <= NH; i++) v = v - c2[i - 1] * (w[NH + i] - w[NH - i + IST]);
+        by[c] = v;
+        v = bz[c];
+        v = fd_sum<NH, false>(v, c0, ey, c, 1, 0, IST, NH);
+        v = fd_sum<NH, true>(v, c1, ex, c, sy, 0, IST, NH);
+        bz[c] = v;
+      }
+#pragma unroll
+      for (int o = 0; o < 2 * NH; o++) {
+        u[o] = u[o + 1];
+        w[o] = w[o + 1];
+      }
+    }
+  }
+  for (int l = max(m1, l0); l < l1; l++) node(l);
+}
+
+template <int NH, bool E>
+static void launch_norder_z(const NorderArgs &a, hipStream_t st) {
+  const int nxu = a.g.nx + 2 * a.nxs + 1, nyu = a.g.ny + 2 * a.nys + 1;
+  const int nzu = a.g.nz + 2 * a.nzs + 1;
+  const dim3 gr((nxu + ZBX - 1) / ZBX, (nyu + 256 / ZBX - 1) / (256 / ZBX),
+               (nzu + ZCHUNK - 1) / ZCHUNK), bl(256);
+  if (a.ist) hipLaunchKernelGGL((k_norder_z<NH, E, 1>), gr, bl, 0, st, a);
+  else hipLaunchKernelGGL((k_norder_z<NH, E, 0>), gr, bl, 0, st, a);
+}
+
+// the register-blocked kernel for a stencil of nh <= 4 terms on every axis,
+// the per-node one otherwise
 static void launch_norder(const NorderArgs &a, bool e, i64 tot) {
   const int nh = a.co.nh[0];
   const int u = (nh == a.co.nh[1] && nh == a.co.nh[2] && nh <= 4) ? nh : 0;
-  const dim3 gr(grid1d(tot, 256)), bl(256);
   hipStream_t st = rt().stream;
   switch (u * 2 + (e ? 1 : 0)) {
-    case 2: hipLaunchKernelGGL((k_norder<1, false>), gr, bl, 0, st, a); break;
-    case 3: hipLaunchKernelGGL((k_norder<1, true>), gr, bl, 0, st, a); break;
-    case 4: hipLaunchKernelGGL((k_norder<2, false>), gr, bl, 0, st, a); break;
-    case 5: hipLaunchKernelGGL((k_norder<2, true>), gr, bl, 0, st, a); break;
-    case 6: hipLaunchKernelGGL((k_norder<3, false>), gr, bl, 0, st, a); break;
-    case 7: hipLaunchKernelGGL((k_norder<3, true>), gr, bl, 0, st, a); break;
-    case 8: hipLaunchKernelGGL((k_norder<4, false>), gr, bl, 0, st, a); break;
-    case 9: hipLaunchKernelGGL((k_norder<4, true>), gr, bl, 0, st, a); break;
-    default:
-      if (e) hipLaunchKernelGGL((k_norder<0, true>), gr, bl, 0, st, a);
-      else hipLaunchKernelGGL((k_norder<0, false>), gr, bl, 0, st, a);
+    case 2: launch_norder_z<1, false>(a, st); break;
+    case 3: launch_norder_z<1, true>(a, st); break;
+    case 4: launch_norder_z<2, false>(a, st); break;
+    case 5: launch_norder_z<2, true>(a, st); break;
+    case 6: launch_norder_z<3, false>(a, st); break;
+    case 7: launch_norder_z<3, true>(a, st); break;
+    case 8: launch_norder_z<4, false>(a, st); break;
+    case 9: launch_norder_z<4, true>(a, st); break;
+    default: {
+      const dim3 gr(grid1d(tot, 256)), bl(256);
+      if (e) hipLaunchKernelGGL((k_norder<true>), gr, bl, 0, st, a);
+      else hipLaunchKernelGGL((k_norder<false>), gr, bl, 0, st, a);
+    }
   }
 }
 

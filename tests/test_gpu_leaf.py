@@ -336,6 +336,41 @@ def test_fdtd_golden_bitexact(dev, pxr, golden, ggrid):
         np.testing.assert_array_equal(np.stack([HG(f) for f in F[3:6]]), d[f"no{no}_b"])
 
 
+@pytest.mark.parametrize("nodal", [False, True])
+@pytest.mark.parametrize("orders,ng,nxs", [((4, 4, 4), 3, 1), ((2, 2, 2), 2, 0),
+                                           ((6, 6, 6), 4, 2), ((8, 8, 8), 4, 1),
+                                           ((4, 2, 6), 3, 1)])
+def test_fdtd_norder_bitexact(dev, pxr, orc, nodal, orders, ng, nxs):
+    """pxrpush_em3d_{e,b}vec_norder against the oracle, bit for bit, on a grid
+    deeper than one z chunk of the register-blocked kernel (full stencils up
+    to order 8 on both staggerings) and with unequal orders per axis (the
+    per-node kernel)."""
+    from picsar_amd.sim import _fd_weights
+    rng = np.random.default_rng(21 + sum(orders))
+    nx, ny, nz = 37, 9, 140
+    shp = (nx + 1 + 2 * ng, ny + 1 + 2 * ng, nz + 1 + 2 * ng)
+    A = [np.asfortranarray(rng.standard_normal(shp)) for _ in range(9)]
+    d = (0.9, 1.1, 1.3)
+    dt = 0.3
+    coef = [dt / d[a] * _fd_weights(orders[a]) for a in range(3)]
+    want = [x.copy(order="F") for x in A]
+    orc.push_em3d_evec_norder(*want, dt, *coef, nx, ny, nz, *orders, ng, ng, ng, nxs, nxs, nxs,
+                              l_nodal=nodal)
+    got = [x.copy(order="F") for x in A]
+    pxr.pxrpush_em3d_evec_norder(*got, dt, *coef, nx, ny, nz, *orders, ng, ng, ng, nxs, nxs, nxs,
+                                 nodal)
+    for a, b in zip(got[:3], want[:3]):
+        np.testing.assert_array_equal(a, b)
+    want = [x.copy(order="F") for x in A]
+    orc.push_em3d_bvec_norder(*want[:6], *coef, nx, ny, nz, *orders, ng, ng, ng, nxs, nxs, nxs,
+                              l_nodal=nodal)
+    got = [x.copy(order="F") for x in A]
+    pxr.pxrpush_em3d_bvec_norder(*got[:6], *coef, nx, ny, nz, *orders, ng, ng, ng, nxs, nxs, nxs,
+                                 nodal)
+    for a, b in zip(got[3:6], want[3:6]):
+        np.testing.assert_array_equal(a, b)
+
+
 def test_guards_bitexact(dev, pxr, orc):
     rng = np.random.default_rng(9)
     a = np.asfortranarray(rng.standard_normal(SHAPE))
