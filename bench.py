@@ -414,9 +414,8 @@ def main():
                          "kernel (the reference point of HBM traffic counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(this build runs the path on one MI355X; see DESIGN.md for how it would shard)",
-              file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: the path runs "
+              "on one MI355X and is not sharded over several GPUs", file=sys.stderr)
         sys.exit(2)
     from picsar_amd import _lib
     _lib.call("pxrhip_init", 0)
