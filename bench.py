@@ -376,6 +376,31 @@ def cpu_baseline(args, port=False, seconds=None):
                       f"tiles (FP64)"}
 
 
+def precision_comparison(args):
+    """The same step at 256^3 cells, 8 ppc electrons + ions, 16^3-node tiles,
+    order-3 Esirkepov, in FP64 and in mixed precision (20 timed steps each):
+    the mixed-precision step against the FP64 one."""
+    import copy
+    out = {}
+    for prec in ("fp64", "mixed"):
+        a = copy.copy(args)
+        a.n, a.ppc, a.tiles, a.precision = 256, 8, 16, prec
+        a.steps, a.warmup, a.drift_steps, a.autotune, a.graph = 20, 2, 0, False, False
+        log(f"comparison: {workload_name(a)}")
+        r = run_gpu(a)
+        del r["sim"]
+        out[prec] = {"value": r["N"] * a.steps / (r["gpu_ms"] * 1e-3),
+                     "ms_per_step": r["gpu_ms"] / a.steps,
+                     "gather_push_deposit_ms_per_step": r["stage_ms"]["gather_push_deposit"],
+                     "launch_ms": r["per_launch_ms"],
+                     "roofline_frac": r["achieved"] / HBM_PEAK_GBS}
+        out["workload"] = workload_name(a).replace(", mixed precision", "").replace(", FP64", "")
+        out["particles"] = r["N"]
+    out["unit"] = "particle-steps/s"
+    out["mixed_over_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -409,6 +434,9 @@ def main():
                          "report that as the value")
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
                     help="launch parameter of the tiled kernels (pxrhip_set_tuning)")
+    ap.add_argument("--no-comparison", action="store_true",
+                    help="skip the FP64 / mixed-precision comparison at 256^3 cells, 8 ppc, "
+                         "16^3-node tiles that follows the default workload")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed region, copy 1 GiB with one plain device copy "
                          "kernel (the reference point of HBM traffic counters)")
@@ -476,6 +504,10 @@ def main():
                    (r["ke0"] + r["fe0"])},
         "gauss_law": r["gauss"],
     }
+    del r["sim"]
+    if (not args.no_comparison and args.workload == "thermal" and args.n == 128
+            and args.precision == "fp64" and args.order == 3 and not args.graph):
+        out["precision_comparison"] = precision_comparison(args)
     if args.calibrate_copy:
         from picsar_amd.devmem import DeviceArray, synchronize
         a = DeviceArray(1 << 27)
