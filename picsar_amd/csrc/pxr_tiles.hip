@@ -399,37 +399,55 @@ __global__ __launch_bounds__(1024) void k_scan_i64_1wg(i64 *v, i64 nb, i64 *tota
 
 // block-local exclusive scan plus the block's offset; writes out[0..n] with
 // out[n] = total
+// Block-local exclusive scan of SCAN_B counts plus the block's offset.  The
+// counts are loaded and the starts stored with consecutive lanes on
+// consecutive elements; in between they sit in LDS, where each thread scans
+// SCAN_B / 256 contiguous ones (one padding slot per 16 keeps those reads off
+// a single pair of banks).
+__device__ __forceinline__ int scan_slot(int e) { return e + (e >> 4); }
+
 __global__ __launch_bounds__(256) void k_scan_apply(const int *in, i64 n, const i64 *boff,
                                                     i64 *out, i64 *cursor) {
+  constexpr int PER = SCAN_B / 256;
+  __shared__ i64 sv[SCAN_B + SCAN_B / 16];
   __shared__ i64 s[256];
   const i64 b0 = (i64)blockIdx.x * SCAN_B;
-  constexpr int PER = SCAN_B / 256;
-  i64 loc[PER];
+  const int t = (int)threadIdx.x;
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const i64 i = b0 + k * 256 + t;
+    sv[scan_slot(k * 256 + t)] = i < n ? in[i] : 0;
+  }
+  __syncthreads();
   i64 acc = 0;
 #pragma unroll
-  for (int k = 0; k < PER; k++) {
-    const i64 i = b0 + (i64)threadIdx.x * PER + k;
-    loc[k] = i < n ? in[i] : 0;
-    acc += loc[k];
-  }
-  s[threadIdx.x] = acc;
+  for (int k = 0; k < PER; k++) acc += sv[scan_slot(t * PER + k)];
+  s[t] = acc;
   __syncthreads();
   for (int off = 1; off < 256; off <<= 1) {
-    const i64 y = threadIdx.x >= (unsigned)off ? s[threadIdx.x - off] : 0;
+    const i64 y = t >= off ? s[t - off] : 0;
     __syncthreads();
-    s[threadIdx.x] += y;
+    s[t] += y;
     __syncthreads();
   }
-  i64 run = boff[blockIdx.x] + s[threadIdx.x] - acc;
+  i64 run = boff[blockIdx.x] + s[t] - acc;
 #pragma unroll
   for (int k = 0; k < PER; k++) {
-    const i64 i = b0 + (i64)threadIdx.x * PER + k;
+    const int e = scan_slot(t * PER + k);
+    const i64 v = sv[e];
+    sv[e] = run;
+    run += v;
+    if (b0 + t * PER + k == n - 1) out[n] = run;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const i64 i = b0 + k * 256 + t;
     if (i < n) {
-      out[i] = run;
-      if (cursor) cursor[i] = run;
+      const i64 v = sv[scan_slot(k * 256 + t)];
+      out[i] = v;
+      if (cursor) cursor[i] = v;
     }
-    run += loc[k];
-    if (i == n - 1) out[n] = run;
   }
 }
 
