@@ -94,6 +94,33 @@ def test_one_step_against_fp64_oracle(torch, orc, order, lo, pusher, depo, jacc)
             assert v <= 1e-7, (k, v)
 
 
+@pytest.mark.parametrize("order,depo", [(1, "direct"), (3, "esirkepov")])
+def test_relativistic_step_against_fp64_oracle(torch, orc, order, depo):
+    """Electrons at u ~ 3 (v up to 0.99 c) cross cells on most steps: the
+    FP64 path of the shape differences and the wide Esirkepov box carry the
+    mixed-precision step, which stays within the same bounds on J, E, B and
+    the momenta.  Positions move by up to 0.55 cells per step here, and the
+    FP32 gamma of the push puts a few FP32 ulps of that displacement into
+    them: 4e-7 cells (1e-7 for the slow plasma of the other tests)."""
+    sim, ref, _ = mixed_build(torch, orc, order, True, "vay", depo, ppc=2, uth=(3.0, 0.3),
+                              seed=11)
+    unique_float32_weights(ref)
+    for sp, rs in zip(sim.species, ref.species):
+        for k in ("ux", "uy", "uz", "w"):
+            rs[k][...] = rs[k].astype(np.float32).astype(np.float64)
+        rs["gaminv"][...] = 1.0 / np.sqrt(1 + rs["ux"] ** 2 + rs["uy"] ** 2 + rs["uz"] ** 2)
+    sim.step(1)
+    ref.step()
+    err = errors(sim, ref)
+    for k in ("jx", "jy", "jz", "ex", "ey", "ez", "bx", "by", "bz"):
+        assert err[k] <= MIXED_TOL, (k, err[k])
+    for k, v in err.items():
+        if k.endswith(("ux", "uy", "uz")):
+            assert v <= MIXED_TOL, (k, v)
+        if k.endswith(".abs"):
+            assert v <= 4e-7, (k, v)
+
+
 def test_ten_steps_against_fp64_oracle(torch, orc):
     sim, ref, _ = mixed_build(torch, orc, 3, True, "vay", "esirkepov", ppc=2)
     for sp, rs in zip(sim.species, ref.species):

@@ -117,6 +117,21 @@ def test_one_step(torch, orc, order, lo, pusher, depo, norder):
     compare_particles(sim, ref, 1e-12 * L, 1e-12)
 
 
+@pytest.mark.parametrize("order,depo", [(1, "esirkepov"), (2, "direct"), (3, "esirkepov"),
+                                         (3, "direct")])
+def test_relativistic_particles(torch, orc, order, depo):
+    """Electrons at u ~ 3 (v up to 0.99 c) cross a cell boundary on most
+    steps, as C4's gamma = 10 bunch does, so the wide Esirkepov box and the
+    re-binning across tiles are the common case: one step at 1e-12."""
+    sim, ref, tng = build(torch, orc, order, True, "vay", depo, uth=(3.0, 0.3), seed=11)
+    sim.step(1)
+    ref.step()
+    check_tiles(sim)
+    compare_fields(sim, ref, tng, 1e-12)
+    L = max(sim.nx * sim.dx, sim.ny * sim.dy, sim.nz * sim.dz)
+    compare_particles(sim, ref, 1e-12 * L, 1e-12)
+
+
 def test_rebin_at_tile_boundaries(torch):
     """Particles at rest exactly on, and a few ulps either side of, every
     tile boundary (the real-number boundary xmin + (k*n0 - 1/2)*dx of
