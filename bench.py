@@ -281,8 +281,13 @@ def run_gpu(args):
         ke, fe = sim.kinetic_energy(), sim.field_energy()
     # untimed steps up to the energy-drift horizon (counted from the start)
     extra = max(0, args.drift_steps - done)
-    for _ in range(extra):
+    t_log = time.perf_counter()
+    for i in range(extra):
         sim.step(1, check=False)
+        if time.perf_counter() - t_log > 30.0:  # a progress line every half minute
+            synchronize()
+            log(f"untimed steps: {i + 1} of {extra}")
+            t_log = time.perf_counter()
     sim.check()
     synchronize()
     if extra:

@@ -123,8 +123,13 @@ class Simulation:
     """One periodic rank: Yee fields with guards and tiled particle species.
 
     Units are the reference's -DLIBRARY normalised units (clight = mu0 =
-    eps0 = 1); momenta are gamma*v.  ``dt`` defaults to cfl/sqrt(1/dx^2 +
-    1/dy^2 + 1/dz^2) (submain.F90:581-590 with c = 1).
+    eps0 = 1); momenta are gamma*v.  ``dt`` defaults to cfl times the
+    stability limit of the FDTD stencil: 1/sqrt(1/dx^2 + 1/dy^2 + 1/dz^2)
+    for Yee (submain.F90:581-590 with c = 1), divided by the sum of the
+    absolute stencil weights for norder > 2 (7/6 at order 4), since the
+    reference's formula, stable for Yee, is not for the longer stencils at
+    the same coefficient (its default dtcoef of 0.7, control_file.F90:179,
+    stays below the order-4 limit of 6/7).
     """
 
     def __init__(self, nx, ny, nz, dx=1.0, dy=1.0, dz=1.0, xmin=0.0, ymin=0.0, zmin=0.0,
@@ -152,7 +157,8 @@ class Simulation:
         ng = max(self.order, 2, self.norder // 2) if nguard is None else int(nguard)
         self.ng = ng
         if dt is None:
-            dt = cfl / math.sqrt(1 / self.dx ** 2 + 1 / self.dy ** 2 + 1 / self.dz ** 2)
+            wsum = float(np.abs(_fd_weights(self.norder)).sum()) if self.norder > 2 else 1.0
+            dt = cfl / (wsum * math.sqrt(1 / self.dx ** 2 + 1 / self.dy ** 2 + 1 / self.dz ** 2))
         self.dt = float(dt)
         self.chunk = int(chunk)
         self.it = 0
