@@ -575,3 +575,36 @@ def test_relativistic_bunch_on_uneven_tiles(torch, orc, precision):
         ia, ib = np.argsort(a["w"]), np.argsort(rs["w"])
         for k in ("ux", "uy", "uz"):
             assert rel(a[k][ia].astype(np.float64), rs[k][ib]) <= tol, k
+
+
+@pytest.mark.parametrize("norder", [2, 4, 6])
+def test_default_time_step_is_stable(torch, norder):
+    """The default dt is cfl (0.95) times the stability limit of the stencil:
+    random fields in vacuum keep their energy over 300 steps.  At the Yee
+    limit an order-4 stencil is unstable (checked by dt = 0.95 of the Yee
+    limit: the grid-scale modes grow by orders of magnitude)."""
+    import math
+    from picsar_amd.sim import Simulation, _fd_weights
+
+    def run(dt):
+        sim = Simulation(24, 20, 16, 1.0, 1.0, 1.0, 0.0, 0.0, 0.0, 1, 1, 1, order=1,
+                         norder=norder, dt=dt)
+        rng = np.random.default_rng(4)
+        for k in ("ex", "ey", "ez", "bx", "by", "bz"):
+            a = np.asfortranarray(rng.standard_normal(sim.fields[k].shape))
+            sim.fields[k].upload(a)
+        sim.efield_bcs()
+        sim.bfield_bcs()
+        e0 = sim.field_energy()
+        for _ in range(300):
+            sim.field_step()
+        return sim, sim.field_energy() / e0
+
+    sim, growth = run(None)
+    yee = 1.0 / math.sqrt(3.0)
+    wsum = 1.0 if norder == 2 else float(np.abs(_fd_weights(norder)).sum())
+    assert sim.dt == pytest.approx(0.95 * yee / wsum, rel=1e-14)
+    assert growth < 2.0, growth
+    if norder == 4:
+        _, growth = run(0.95 * yee)
+        assert growth > 1e3, growth
