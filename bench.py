@@ -44,15 +44,25 @@ BYTES_PER_CELL = 72            # read E, B (48 B); write J (24 B)
 OMEGA_PE = 0.02                # electron plasma frequency (c = dx = 1)
 
 
+def traffic_profile(args):
+    """The committed counter summary that holds this run's fused launch:
+    the default workload and launch parameters, in FP64 or mixed precision."""
+    if (args.n, args.ppc, args.tiles, args.chunk) != (128, 16, 8, 0) or args.tune or \
+            args.workload != "thermal" or args.autotune or \
+            (args.order, args.pusher, args.depo) != (3, "vay", "esirkepov"):
+        return None
+    return "fused_c2_hbm_traffic.md" if args.precision == "fp64" else \
+        "fused_c2_mixed_hbm_traffic.md"
+
+
 def profiled_traffic(args):
     """HBM bytes per fused launch from the committed counter summary
     (FETCH_SIZE and WRITE_SIZE passes scaled by a copy-kernel calibration),
     valid for the default workload and launch parameters only."""
-    if (args.n, args.ppc, args.tiles, args.chunk) != (128, 16, 8, 0) or args.tune or \
-            args.precision != "fp64" or args.workload != "thermal" or args.autotune or \
-            (args.order, args.pusher, args.depo) != (3, "vay", "esirkepov"):
+    name = traffic_profile(args)
+    if name is None:
         return None
-    path = os.path.join(ROOT, "profiles", "fused_c2_hbm_traffic.md")
+    path = os.path.join(ROOT, "profiles", name)
     try:
         for line in open(path):
             if line.startswith("traffic_bytes_per_launch ="):
@@ -495,8 +505,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": r["achieved"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": r["achieved"] / HBM_PEAK_GBS,
                      "traffic": profiled_traffic(args),
-                     "traffic_source": "profiles/fused_c2_hbm_traffic.md (rocprofv3 FETCH_SIZE, "
-                                       "WRITE_SIZE, copy-calibrated)",
+                     "traffic_source": (f"profiles/{traffic_profile(args)} (rocprofv3 "
+                                        "FETCH_SIZE, WRITE_SIZE, copy-calibrated)")
+                     if traffic_profile(args) else None,
                      "kernel": "k_fused (gather+push+deposit, one species per launch)",
                      "launch_ms": r["per_launch_ms"],
                      "bytes_per_launch": r["bytes_launch"]},
