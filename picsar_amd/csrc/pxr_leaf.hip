@@ -236,10 +236,19 @@ struct GlobalAcc {
   PXR_DEV void add_row(double *r, int o, double v) const { unsafeAtomicAdd(r + o, v); }
 };
 
+// Particles of the deposition kernels: a block takes 256 * DEP_S of them
+// and the lanes of a wave take particles 4 * DEP_S apart (each lane DEP_S
+// consecutive ones in turn).  In cell-sorted input, as the tiled species
+// hold it, the lanes of one atomic instruction then sit in different cells
+// instead of queueing on the same nodes.
+#define DEP_S 4
+PXR_DEV i64 dep_particle(int k) {
+  const int t = (int)threadIdx.x;
+  return (i64)blockIdx.x * (256 * DEP_S) + (t & 63) * (4 * DEP_S) + (t >> 6) * DEP_S + k;
+}
+
 template <int N>
-__global__ __launch_bounds__(256) void k_esirkepov(DepoArgs a) {
-  const i64 ip = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-  if (ip >= a.np) return;
+PXR_DEV void esirkepov_one(const DepoArgs &a, i64 ip) {
   EsirkepovIn<double> in;
   esirkepov_prepare<N, double>(in, a.xp[ip], a.yp[ip], a.zp[ip], a.ux[ip], a.uy[ip], a.uz[ip],
                                a.gi[ip], a.w[ip], a.ec);
@@ -255,9 +264,15 @@ __global__ __launch_bounds__(256) void k_esirkepov(DepoArgs a) {
 }
 
 template <int N>
-__global__ __launch_bounds__(256) void k_direct(DepoArgs a) {
-  const i64 ip = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-  if (ip >= a.np) return;
+__global__ __launch_bounds__(256) void k_esirkepov(DepoArgs a) {
+  for (int k = 0; k < DEP_S; k++) {
+    const i64 ip = dep_particle(k);
+    if (ip < a.np) esirkepov_one<N>(a, ip);
+  }
+}
+
+template <int N>
+PXR_DEV void direct_one(const DepoArgs &a, i64 ip) {
   DirectIn<double> in;
   direct_prepare<N, double>(in, a.xp[ip], a.yp[ip], a.zp[ip], a.ux[ip], a.uy[ip], a.uz[ip],
                             a.gi[ip], a.w[ip], a.q, a.xmin, a.ymin, a.zmin, a.dt, a.dx, a.dy,
@@ -271,6 +286,14 @@ __global__ __launch_bounds__(256) void k_direct(DepoArgs a) {
   const i64 o = a.g.idx(0, 0, 0);
   GlobalAcc acc{{a.jx + o, a.jy + o, a.jz + o}, a.g.NX, a.g.NX * a.g.NY};
   direct_deposit<N, double>(in, acc);
+}
+
+template <int N>
+__global__ __launch_bounds__(256) void k_direct(DepoArgs a) {
+  for (int k = 0; k < DEP_S; k++) {
+    const i64 ip = dep_particle(k);
+    if (ip < a.np) direct_one<N>(a, ip);
+  }
 }
 
 // charge density, pxr_depose_rho_n (charge_deposition_3d.F90:2081-2228):
@@ -287,9 +310,7 @@ struct RhoArgs {
 };
 
 template <int N>
-__global__ __launch_bounds__(256) void k_rho(RhoArgs a) {
-  const i64 ip = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-  if (ip >= a.np) return;
+PXR_DEV void rho_one(const RhoArgs &a, i64 ip) {
   double x = (a.xp[ip] - a.xmin) * a.dxi;
   double y = (a.yp[ip] - a.ymin) * a.dyi;
   const double z = (a.zp[ip] - a.zmin) * a.dzi;
@@ -314,6 +335,14 @@ __global__ __launch_bounds__(256) void k_rho(RhoArgs a) {
 #pragma unroll
       for (int e = 0; e <= N; e++)
         unsafeAtomicAdd(r + e + a.g.NX * (b + (i64)a.g.NY * c), sx[e] * sy[b] * sz[c] * wq);
+}
+
+template <int N>
+__global__ __launch_bounds__(256) void k_rho(RhoArgs a) {
+  for (int k = 0; k < DEP_S; k++) {
+    const i64 ip = dep_particle(k);
+    if (ip < a.np) rho_one<N>(a, ip);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1029,7 +1058,7 @@ int pxrhip_depose_jxjyjz(double *jx, double *jy, double *jz, int64_t np, const d
   a.g = mkgeom(nx, ny, nz, nxguard, nyguard, nzguard);
   a.jx = jx; a.jy = jy; a.jz = jz;
   a.bad = rt().d_bad;
-  dim3 grd(grid1d(np, 256)), blk(256);
+  dim3 grd(grid1d(np, 256 * DEP_S)), blk(256);
   hipStream_t st = rt().stream;
   const bool direct = current_depo_algo >= 2;  // depose_jxjyjz_generic CASE(2), CASE(3)
   if (direct) {
@@ -1071,7 +1100,7 @@ int pxrhip_depose_rho_n(double *rho, int64_t np, const double *xp, const double 
   a.g = mkgeom(nx, ny, nz, nxguard, nyguard, nzguard);
   a.rho = rho;
   a.bad = rt().d_bad;
-  dim3 grd(grid1d(np, 256)), blk(256);
+  dim3 grd(grid1d(np, 256 * DEP_S)), blk(256);
   hipStream_t st = rt().stream;
   if (nox == 1) hipLaunchKernelGGL(k_rho<1>, grd, blk, 0, st, a);
   else if (nox == 2) hipLaunchKernelGGL(k_rho<2>, grd, blk, 0, st, a);
