@@ -356,6 +356,17 @@ int pxrhip_tiles_gather_push_deposit(
 int pxrhip_tiles_rebin_sort(const pxrhip_grid *grid, pxrhip_species *src,
                             pxrhip_species *dst);
 
+/* Charge density of a tiled species, ADDED into rho (guards included, not
+ * folded): the per-tile blocks of pxrdepose_rho_on_grid_sub_openmp_3d
+ * (charge_deposition_manager.F90:516-743) with the shapes of
+ * pxr_depose_rho_n (charge_deposition_3d.F90:2081-2228) of order 1..3.  The
+ * species must be binned (as pxrhip_tiles_rebin_sort leaves it); particles
+ * that moved since are still deposited, from global memory.  Particles
+ * outside the arrays are skipped and counted for pxrhip_tiles_check.  A
+ * mixed-precision species' float32 weights are read as such. */
+int pxrhip_tiles_depose_rho(const pxrhip_grid *grid, const pxrhip_species *sp,
+                            double *rho, int64_t order);
+
 /* Launch parameters of the tiled kernels.  No counterpart in the reference
  * (its CPU blocking factors lvec_* play this role).  Keys:
  * "fused.lds_bytes" (dynamic LDS per workgroup), "fused.wg_per_cu",

@@ -726,6 +726,163 @@ static Tuning g_tune;
 static int g_last_wg_per_cu = 0, g_last_lds_bytes = 0;
 }  // namespace pxr
 
+// Charge density of a binned species, per work item: the tile blocks of
+// pxrdepose_rho_on_grid_sub_openmp_3d (charge_deposition_manager.F90:516-743)
+// with the nodal shapes of pxr_depose_rho_n (charge_deposition_3d.F90:
+// 2081-2228).  A work item's particles add q w S_x S_y S_z / vol into an LDS
+// block of rho, which is then added to rho; a particle whose stencil leaves
+// the block (moved since it was binned) adds into rho directly.
+struct RhoTileArgs {
+  const Chunk *chunks;
+  const int *nchunks;
+  int *queue;
+  const double *x, *y, *z;
+  const void *w;  // FP64, or FP32 for a mixed-precision species
+  int mixed;
+  double *rho;
+  GridGeom g;
+  BoxGeom bg;
+  double q, xmin, ymin, zmin, dxi, dyi, dzi, invvol;
+  int lds_bytes;
+  unsigned long long *bad;
+};
+
+template <int N>
+__global__ __launch_bounds__(256) void k_rho_tiles(RhoTileArgs a) {
+  extern __shared__ double srho[];
+  __shared__ int s_chunk;
+  const GridGeom &g = a.g;
+  const int glo[3] = {-g.gx, -g.gy, -g.gz};
+  const int ghi[3] = {g.nx + g.gx, g.ny + g.gy, g.nz + g.gz};
+  const int nch = *a.nchunks;
+  for (;;) {
+    if (threadIdx.x == 0) s_chunk = atomicAdd(a.queue, 1);
+    __syncthreads();
+    const int c = s_chunk;
+    __syncthreads();  // read by all before the next fetch overwrites it
+    if (c >= nch) break;
+    const Chunk ch = a.chunks[c];
+    int R0[3] = {0, 0, 0}, e[3] = {0, 0, 0};
+    bool blk = ch.z >= 0;
+    if (blk) {
+      double ulo[3], uhi[3];
+      rows_range(a.bg, ch.tile, ch.z, ch.z1, ch.y0, ch.y1, ulo, uhi);
+      for (int d = 0; d < 3; d++) {
+        int lo, hi;
+        deposit_range(a.bg, ulo[d], uhi[d], 0.0, lo, hi);
+        R0[d] = max(lo, glo[d]);
+        e[d] = max(min(hi, ghi[d]) - R0[d] + 1, 0);
+      }
+      const i64 nb64 = (i64)e[0] * e[1] * e[2];
+      blk = nb64 > 0 && nb64 * (i64)sizeof(double) <= a.lds_bytes;
+    }
+    const int nb = blk ? e[0] * e[1] * e[2] : 0;
+    for (int i = threadIdx.x; i < nb; i += 256) srho[i] = 0.0;
+    __syncthreads();
+    for (int k = threadIdx.x; k < ch.n; k += 256) {
+      const i64 p = ch.begin + k;
+      double sx[N + 1], sy[N + 1], sz[N + 1];
+      const int j0 = shape_nodal<N, double>((a.x[p] - a.xmin) * a.dxi, sx);
+      const int k0 = shape_nodal<N, double>((a.y[p] - a.ymin) * a.dyi, sy);
+      const int l0 = shape_nodal<N, double>((a.z[p] - a.zmin) * a.dzi, sz);
+      const double w = a.mixed ? (double)((const float *)a.w)[p] : ((const double *)a.w)[p];
+      const double wq = a.q * w * a.invvol;
+      if (blk && j0 >= R0[0] && j0 + N < R0[0] + e[0] && k0 >= R0[1] && k0 + N < R0[1] + e[1] &&
+          l0 >= R0[2] && l0 + N < R0[2] + e[2]) {
+        double *r = srho + (j0 - R0[0]) + e[0] * ((k0 - R0[1]) + e[1] * (l0 - R0[2]));
+#pragma unroll
+        for (int cc = 0; cc <= N; cc++)
+#pragma unroll
+          for (int bb = 0; bb <= N; bb++)
+#pragma unroll
+            for (int aa = 0; aa <= N; aa++)
+              atomicAdd(r + aa + e[0] * (bb + e[1] * cc), sx[aa] * sy[bb] * sz[cc] * wq);
+      } else if (g.inside(j0, k0, l0) && g.inside(j0 + N, k0 + N, l0 + N)) {
+        double *r = a.rho + g.idx(j0, k0, l0);
+#pragma unroll
+        for (int cc = 0; cc <= N; cc++)
+#pragma unroll
+          for (int bb = 0; bb <= N; bb++)
+#pragma unroll
+            for (int aa = 0; aa <= N; aa++)
+              unsafeAtomicAdd(r + aa + g.NX * (bb + g.NY * (i64)cc), sx[aa] * sy[bb] * sz[cc] * wq);
+      } else {
+        count_bad(a.bad);
+      }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nb; i += 256) {
+      const double v = srho[i];
+      if (v != 0.0) {
+        const int ii = i % e[0], jj = (i / e[0]) % e[1], kk = i / (e[0] * e[1]);
+        unsafeAtomicAdd(a.rho + g.idx(R0[0] + ii, R0[1] + jj, R0[2] + kk), v);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Work items of a binned species (k_chunk_groups, k_chunk_count, the scan,
+// k_chunk_write): runs of at most ch particles whose blocks, estimated from
+// bg with fnode / jnode bytes per node, fit lds_bytes / j_bytes.  The table
+// and its counters live in the library's scratch slots 8-10 and 12-15.
+static int build_chunks(const TileTable *tt, const pxrhip_species *sp, const BoxGeom &bg, int ch,
+                        i64 lds_bytes, i64 j_bytes, int fnode, int jnode, hipStream_t st,
+                        Chunk **chunks_out, int **ctl_out, int2 **ovf_out, i64 *maxchunks_out) {
+  const i64 cap = sp->capacity;
+  const int nzmax = tt->nzmax;
+  const i64 nunits = (i64)tt->ntiles * nzmax;
+  // A plane gives one chunk if it heads a group of whole planes, none if it
+  // lies inside one, else its row walk: a chunk per row that opens one
+  // (<= its rows), per cut of a row denser than ch (<= cap/ch over all
+  // planes) and a last one.  Hence at most nrows + cap/ch + nunits.
+  const i64 maxchunks = tt->nrows + (cap / ch + 1) + nunits + 16;
+  const i64 nblk = (nunits + SCAN_B - 1) / SCAN_B;
+  int *cnt = (int *)scratch(8, sizeof(int) * (size_t)nunits);
+  int *grp = (int *)scratch(15, sizeof(int) * (size_t)nunits);
+  i64 *cstart = (i64 *)scratch(12, sizeof(i64) * (size_t)(nunits + 1));
+  i64 *bsum = (i64 *)scratch(13, sizeof(i64) * (size_t)(nblk + 1));
+  Chunk *chunks = (Chunk *)scratch(9, sizeof(Chunk) * (size_t)maxchunks);
+  int *ctl = (int *)scratch(10, 4 * sizeof(int));
+  int2 *ovf = (int2 *)scratch(14, sizeof(int2) * (size_t)maxchunks);
+  if (!cnt || !grp || !cstart || !bsum || !chunks || !ctl || !ovf) return set_error(PXRHIP_ERR_HIP, "scratch allocation failed");
+  ChunkArgs ca;
+  ca.ntiles = tt->ntiles;
+  ca.ch = ch;
+  ca.nzmax = nzmax;
+  ca.bg = bg;
+  ca.lds_bytes = lds_bytes;
+  ca.j_bytes = j_bytes;
+  ca.fnode_bytes = fnode;
+  ca.jnode_bytes = jnode;
+  ca.tile_off = sp->tile_off;
+  ca.tile_np = sp->tile_np;
+  ca.bin_start = sp->bin_start;
+  ca.n3 = tt->n3;
+  ca.bin_base = tt->bin_base;
+  ca.count = cnt;
+  ca.group = grp;
+  ca.start = cstart;
+  ca.out = chunks;
+  ca.nchunks = ctl;
+  ca.queue = ctl + 1;
+  ca.maxchunks = maxchunks;
+  ca.bad = rt().d_bad;
+  hipLaunchKernelGGL(k_chunk_groups, dim3(grid1d(tt->ntiles, 64)), dim3(64), 0, st, ca);
+  hipLaunchKernelGGL(k_chunk_count, dim3(grid1d(nunits, 256)), dim3(256), 0, st, ca);
+  hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nblk), dim3(256), 0, st, cnt, nunits, bsum);
+  hipLaunchKernelGGL(k_scan_i64_1wg, dim3(1), dim3(1024), 0, st, bsum, nblk, (i64 *)nullptr);
+  hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nblk), dim3(256), 0, st, cnt, nunits, bsum,
+                     cstart, (i64 *)nullptr);
+  hipLaunchKernelGGL(k_chunk_write, dim3(grid1d(nunits, 256)), dim3(256), 0, st, ca);
+  PXR_LAUNCHED();
+  *chunks_out = chunks;
+  *ctl_out = ctl;
+  *ovf_out = ovf;
+  *maxchunks_out = maxchunks;
+  return PXRHIP_OK;
+}
+
 extern "C" {
 
 int pxrhip_tiles_nbins(const pxrhip_grid *grid, int64_t *nbins) {
@@ -856,6 +1013,73 @@ static RebinGeom make_rebin_geom(const pxrhip_grid *grid, const TileTable *tt) {
   return rg;
 }
 
+int pxrhip_tiles_depose_rho(const pxrhip_grid *grid, const pxrhip_species *sp, double *rho,
+                            int64_t order) {
+  PXR_INIT();
+  TileTable *tt;
+  int r = get_table(grid, &tt);
+  if (r) return r;
+  if ((r = check_species(sp, true))) return r;
+  if (order < 1 || order > 3) return set_error(PXRHIP_ERR_UNSUPPORTED, "shape order must be 1..3");
+  if (!rho) return set_error(PXRHIP_ERR_ARG, "null rho");
+  hipStream_t st = rt().stream;
+  BoxGeom bg;
+  const double x0[3] = {grid->xmin, grid->ymin, grid->zmin};
+  const double dd[3] = {grid->dx, grid->dy, grid->dz};
+  const double sd[3] = {grid->sorting_dx, grid->sorting_dy, grid->sorting_dz};
+  const int64_t nn[3] = {grid->nx, grid->ny, grid->nz}, ng[3] = {grid->ngx, grid->ngy, grid->ngz};
+  for (int d = 0; d < 3; d++) {
+    bg.x0[d] = x0[d];
+    bg.di[d] = 1.0 / dd[d];
+    bg.sd[d] = sd[d];
+    bg.mv[d] = 0.0;
+    bg.glo[d] = (int)-ng[d];
+    bg.ghi[d] = (int)(nn[d] + ng[d]);
+  }
+  bg.s = 0.5;
+  bg.N = bg.M = (int)order;
+  bg.depo = 3;  // the nodal index ranges of the direct deposition cover those of rho
+  bg.sorg = tt->sorg;
+  bg.tbox = tt->tbox;
+  bg.n3 = tt->n3;
+  // four workgroups per CU, each with a quarter of the LDS less its static part
+  const i64 lds_bytes = 38912;
+  Chunk *chunks;
+  int *ctl;
+  int2 *ovf;
+  i64 maxchunks;
+  if ((r = build_chunks(tt, sp, bg, g_tune.chunk, lds_bytes, lds_bytes, 0, 8, st, &chunks, &ctl,
+                        &ovf, &maxchunks)))
+    return r;
+  RhoTileArgs a;
+  a.chunks = chunks;
+  a.nchunks = ctl;
+  a.queue = ctl + 1;
+  a.x = sp->x; a.y = sp->y; a.z = sp->z;
+  a.w = sp->w;
+  a.mixed = sp->precision == 1;
+  a.rho = rho;
+  GridGeom g;
+  g.nx = (int)grid->nx; g.ny = (int)grid->ny; g.nz = (int)grid->nz;
+  g.gx = (int)grid->ngx; g.gy = (int)grid->ngy; g.gz = (int)grid->ngz;
+  g.NX = grid->nx + 1 + 2 * grid->ngx;
+  g.NY = grid->ny + 1 + 2 * grid->ngy;
+  a.g = g;
+  a.bg = bg;
+  a.q = sp->q;
+  a.xmin = grid->xmin; a.ymin = grid->ymin; a.zmin = grid->zmin;
+  a.dxi = 1.0 / grid->dx; a.dyi = 1.0 / grid->dy; a.dzi = 1.0 / grid->dz;
+  a.invvol = 1.0 / (grid->dx * grid->dy * grid->dz);
+  a.lds_bytes = (int)lds_bytes;
+  a.bad = rt().d_bad;
+  const dim3 gr((unsigned)(rt().num_cus * 4)), bl(256);
+  if (order == 1) hipLaunchKernelGGL(k_rho_tiles<1>, gr, bl, (size_t)lds_bytes, st, a);
+  else if (order == 2) hipLaunchKernelGGL(k_rho_tiles<2>, gr, bl, (size_t)lds_bytes, st, a);
+  else hipLaunchKernelGGL(k_rho_tiles<3>, gr, bl, (size_t)lds_bytes, st, a);
+  PXR_LAUNCHED();
+  return PXRHIP_OK;
+}
+
 int pxrhip_tiles_rebin_sort(const pxrhip_grid *grid, pxrhip_species *src, pxrhip_species *dst) {
   PXR_INIT();
   TileTable *tt;
@@ -938,9 +1162,6 @@ int pxrhip_tiles_gather_push_deposit(const pxrhip_grid *grid, const pxrhip_speci
     return set_error(PXRHIP_ERR_ARG, "null field array");
   const int ch = chunk > 0 ? (int)(chunk < (1 << 20) ? chunk : (1 << 20)) : g_tune.chunk;
   hipStream_t st = rt().stream;
-  // chunk table
-  const i64 cap = sp->capacity;
-  const int nzmax = tt->nzmax;
   // block geometry of the work items
   BoxGeom bg;
   {
@@ -987,51 +1208,13 @@ int pxrhip_tiles_gather_push_deposit(const pxrhip_grid *grid, const pxrhip_speci
   }
   const i64 j_bytes = g_tune.j_lds_bytes < 0 || g_tune.j_lds_bytes > lds_bytes ? lds_bytes
                                                                               : g_tune.j_lds_bytes;
-  const i64 nunits = (i64)tt->ntiles * nzmax;
-  // A plane gives one chunk if it heads a group of whole planes, none if it
-  // lies inside one, else its row walk: a chunk per row that opens one
-  // (<= its rows), per cut of a row denser than ch (<= cap/ch over all
-  // planes) and a last one.  Hence at most nrows + cap/ch + nunits.
-  const i64 maxchunks = tt->nrows + (cap / ch + 1) + nunits + 16;
-  const i64 nblk = (nunits + SCAN_B - 1) / SCAN_B;
-  int *cnt = (int *)scratch(8, sizeof(int) * (size_t)nunits);
-  int *grp = (int *)scratch(15, sizeof(int) * (size_t)nunits);
-  i64 *cstart = (i64 *)scratch(12, sizeof(i64) * (size_t)(nunits + 1));
-  i64 *bsum = (i64 *)scratch(13, sizeof(i64) * (size_t)(nblk + 1));
-  Chunk *chunks = (Chunk *)scratch(9, sizeof(Chunk) * (size_t)maxchunks);
-  int *ctl = (int *)scratch(10, 4 * sizeof(int));
-  int2 *ovf = (int2 *)scratch(14, sizeof(int2) * (size_t)maxchunks);
-  if (!cnt || !grp || !cstart || !bsum || !chunks || !ctl || !ovf) return set_error(PXRHIP_ERR_HIP, "scratch allocation failed");
-  ChunkArgs ca;
-  ca.ntiles = tt->ntiles;
-  ca.ch = ch;
-  ca.nzmax = nzmax;
-  ca.bg = bg;
-  ca.lds_bytes = lds_bytes;
-  ca.j_bytes = j_bytes;
-  ca.fnode_bytes = 6 * esize;
-  ca.jnode_bytes = jnode;
-  ca.tile_off = sp->tile_off;
-  ca.tile_np = sp->tile_np;
-  ca.bin_start = sp->bin_start;
-  ca.n3 = tt->n3;
-  ca.bin_base = tt->bin_base;
-  ca.count = cnt;
-  ca.group = grp;
-  ca.start = cstart;
-  ca.out = chunks;
-  ca.nchunks = ctl;
-  ca.queue = ctl + 1;
-  ca.maxchunks = maxchunks;
-  ca.bad = rt().d_bad;
-  hipLaunchKernelGGL(k_chunk_groups, dim3(grid1d(tt->ntiles, 64)), dim3(64), 0, st, ca);
-  hipLaunchKernelGGL(k_chunk_count, dim3(grid1d(nunits, 256)), dim3(256), 0, st, ca);
-  hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nblk), dim3(256), 0, st, cnt, nunits, bsum);
-  hipLaunchKernelGGL(k_scan_i64_1wg, dim3(1), dim3(1024), 0, st, bsum, nblk, (i64 *)nullptr);
-  hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nblk), dim3(256), 0, st, cnt, nunits, bsum,
-                     cstart, (i64 *)nullptr);
-  hipLaunchKernelGGL(k_chunk_write, dim3(grid1d(nunits, 256)), dim3(256), 0, st, ca);
-  PXR_LAUNCHED();
+  Chunk *chunks;
+  int *ctl;
+  int2 *ovf;
+  i64 maxchunks;
+  if ((r = build_chunks(tt, sp, bg, ch, lds_bytes, j_bytes, 6 * esize, jnode, st, &chunks, &ctl,
+                        &ovf, &maxchunks)))
+    return r;
   // fused kernel
   FusedArgs a;
   a.chunks = chunks;

@@ -195,7 +195,6 @@ class Simulation:
         self.species: List[Species] = []
         self.antennas = []
         self.diag_fields = {}  # rho, dive, divb: allocated on first use
-        self._w64 = {}  # FP64 copies of mixed-precision weights for the charge density
 
     # ------------------------------------------------------------------
     # particles
@@ -496,30 +495,22 @@ class Simulation:
         """rho = 0, then the charge density of every species that deposits
         (frozen ones included) and of the laser antennas, which the
         reference holds as species that deposit (tiling.F90:1684-1697), at
-        the simulation's shape order.  The FP32 weights of a mixed-precision
-        species are widened to FP64 first."""
+        the simulation's shape order: the species with the tiled kernel
+        (LDS blocks per work item), the antennas with the leaf one."""
         rho = self.diag_grid("rho")
         rho.fill(0.0)
         g = self.ng
-        parts = []
         for sp in self.species:
-            if not sp.ldodepos or sp.npart == 0:
-                continue
-            a = sp.active.arrays
-            w = a["w"]
-            if sp.precision != "fp64":  # FP32 weights, converted on the device
-                w = self._w64.get(sp.name)
-                if w is None or w.n < sp.npart:
-                    w = self._w64[sp.name] = DeviceArray(sp.npart)
-                _lib.call("pxrhip_convert_f32_f64", a["w"].ptr, w.ptr, sp.npart)
-            parts.append((sp.npart, a["x"], a["y"], a["z"], w, sp.q))
+            if sp.ldodepos and sp.npart:
+                s = sp.struct(sp.cur)
+                _lib.call("pxrhip_tiles_depose_rho", C.byref(self.grid), C.byref(s), rho.ptr,
+                          self.order)
         for ant in self.antennas:
             if ant.npart:  # pid column 1 holds the weights
-                parts.append((ant.npart, ant.x, ant.y, ant.z, ant.pid, ant.charge))
-        for n, x, y, z, w, q in parts:
-            _lib.call("pxrhip_depose_rho_n", rho.ptr, n, x.ptr, y.ptr, z.ptr, w.ptr, q, self.xmin,
-                      self.ymin, self.zmin, self.dx, self.dy, self.dz, self.nx, self.ny, self.nz,
-                      g, g, g, self.order, self.order, self.order, 1, 0)
+                _lib.call("pxrhip_depose_rho_n", rho.ptr, ant.npart, ant.x.ptr, ant.y.ptr,
+                          ant.z.ptr, ant.pid.ptr, ant.charge, self.xmin, self.ymin, self.zmin,
+                          self.dx, self.dy, self.dz, self.nx, self.ny, self.nz, g, g, g,
+                          self.order, self.order, self.order, 1, 0)
 
     def charge_bcs(self):
         """Periodic fold of the rho guards (summation_bcs on rho)."""

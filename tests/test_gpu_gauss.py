@@ -128,3 +128,46 @@ def test_gauss_law_in_mixed_precision(dev, order):
     r1, dv, _ = residual(sim)
     assert np.linalg.norm(dv) > 1e-4 * SCALE
     assert np.linalg.norm(r1 - r0) / SCALE < 1e-5
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_tiled_rho_matches_the_leaf_routine(dev, order, precision):
+    """pxrhip_tiles_depose_rho (LDS blocks per work item) against
+    pxr_depose_rho_n over the same particles, binned and then after a push
+    that moved them out of their bins (deposited from global memory)."""
+    import ctypes as C
+    from picsar_amd import _lib
+    from picsar_amd.devmem import DeviceArray, DeviceGrid
+    sim = neutral_plasma(order, "esirkepov", precision=precision)
+    g = sim.ng
+
+    def leaf():
+        r = DeviceGrid(*N, g)
+        for sp in sim.species:
+            a = sp.arrays()
+            dev_arr = [DeviceArray.from_numpy(np.ascontiguousarray(a[k], dtype=np.float64))
+                       for k in ("x", "y", "z", "w")]
+            _lib.call("pxrhip_depose_rho_n", r.ptr, sp.npart, *(d.ptr for d in dev_arr), sp.q,
+                      sim.xmin, sim.ymin, sim.zmin, sim.dx, sim.dy, sim.dz, *N, g, g, g, order,
+                      order, order, 1, 0)
+        return r.numpy()
+
+    def tiled():
+        r = DeviceGrid(*N, g)
+        for sp in sim.species:
+            s = sp.struct(sp.cur)
+            _lib.call("pxrhip_tiles_depose_rho", C.byref(sim.grid), C.byref(s), r.ptr, order)
+        _lib.call("pxrhip_tiles_check")
+        return r.numpy()
+
+    # electrons only, so that rho does not cancel
+    sim.species[1].ldodepos = False
+    sim.species = sim.species[:1]
+    for stage in ("binned", "moved"):
+        want, got = leaf(), tiled()
+        scale = np.abs(want).max()
+        assert scale > 0
+        assert np.abs(got - want).max() <= 1e-13 * scale, (stage, np.abs(got - want).max() / scale)
+        sim.gather_push_deposit()  # positions move; the bins are not rebuilt
+        sim.check()

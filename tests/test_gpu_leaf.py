@@ -508,3 +508,15 @@ def test_torch_tensors_interoperate(dev, pxr, orc):
                                   *tF, False, True, 8, 0)
     for a, b in zip(te, e_ref):
         assert rel(a.cpu().numpy(), b) < REL
+
+
+def test_convert_f32_f64(dev):
+    """pxrhip_convert_f32_f64 widens float32 values exactly."""
+    from picsar_amd.devmem import DeviceArray
+    rng = np.random.default_rng(31)
+    h = (rng.standard_normal(10007) * 1e3).astype(np.float32)
+    src = DeviceArray.from_numpy(h)
+    dst = DeviceArray(len(h))
+    dev.call("pxrhip_convert_f32_f64", src.ptr, dst.ptr, len(h))
+    np.testing.assert_array_equal(dst.numpy(), h.astype(np.float64))
+    dev.call("pxrhip_convert_f32_f64", src.ptr, dst.ptr, 0)  # n = 0: nothing to do
