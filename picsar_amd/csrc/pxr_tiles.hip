@@ -355,46 +355,47 @@ __global__ __launch_bounds__(256) void k_rb_keys(RebinArgs a, i64 cap) {
 // of SCAN_B elements
 #define SCAN_B 2048
 __global__ __launch_bounds__(256) void k_scan_blocks(const int *in, i64 n, i64 *bsum) {
-  __shared__ i64 s[256];
+  __shared__ i64 s[4];
   const i64 b0 = (i64)blockIdx.x * SCAN_B;
   i64 acc = 0;
   for (int k = threadIdx.x; k < SCAN_B; k += 256) {
     const i64 i = b0 + k;
     if (i < n) acc += in[i];
   }
-  s[threadIdx.x] = acc;
+  // the block's sum: within each wave by shuffles, then the four wave sums
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o);
+  if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = acc;
   __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) bsum[blockIdx.x] = s[0];
+  if (threadIdx.x == 0) bsum[blockIdx.x] = s[0] + s[1] + s[2] + s[3];
 }
 
 // in place exclusive scan of nb 64-bit values, one workgroup
 __global__ __launch_bounds__(1024) void k_scan_i64_1wg(i64 *v, i64 nb, i64 *total) {
-  __shared__ i64 s[1024];
+  __shared__ i64 s[16];  // inclusive sums of the sixteen waves
   __shared__ i64 carry;
-  if (threadIdx.x == 0) carry = 0;
+  const int t = (int)threadIdx.x, lane = t & 63, wv = t >> 6;
+  if (t == 0) carry = 0;
   __syncthreads();
   for (i64 base = 0; base < nb; base += 1024) {
-    const i64 i = base + threadIdx.x;
+    const i64 i = base + t;
     const i64 x = i < nb ? v[i] : 0;
-    s[threadIdx.x] = x;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-      const i64 y = threadIdx.x >= (unsigned)off ? s[threadIdx.x - off] : 0;
-      __syncthreads();
-      s[threadIdx.x] += y;
-      __syncthreads();
+    i64 inc = x;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const i64 y = __shfl_up(inc, off);
+      if (lane >= off) inc += y;
     }
-    const i64 incl = s[threadIdx.x];
-    if (i < nb) v[i] = carry + incl - x;
+    if (lane == 63) s[wv] = inc;
     __syncthreads();
-    if (threadIdx.x == 1023) carry += incl;
+    i64 wbase = carry;
+    for (int w = 0; w < wv; w++) wbase += s[w];
+    if (i < nb) v[i] = wbase + inc - x;
+    __syncthreads();  // s and carry read by all
+    if (t == 1023) carry = wbase + inc;
     __syncthreads();
   }
-  if (threadIdx.x == 0 && total) *total = carry;
+  if (t == 0 && total) *total = carry;
 }
 
 // block-local exclusive scan plus the block's offset; writes out[0..n] with
@@ -410,7 +411,7 @@ __global__ __launch_bounds__(256) void k_scan_apply(const int *in, i64 n, const 
                                                     i64 *out, i64 *cursor) {
   constexpr int PER = SCAN_B / 256;
   __shared__ i64 sv[SCAN_B + SCAN_B / 16];
-  __shared__ i64 s[256];
+  __shared__ i64 s[4];
   const i64 b0 = (i64)blockIdx.x * SCAN_B;
   const int t = (int)threadIdx.x;
 #pragma unroll
@@ -422,15 +423,20 @@ __global__ __launch_bounds__(256) void k_scan_apply(const int *in, i64 n, const 
   i64 acc = 0;
 #pragma unroll
   for (int k = 0; k < PER; k++) acc += sv[scan_slot(t * PER + k)];
-  s[t] = acc;
-  __syncthreads();
-  for (int off = 1; off < 256; off <<= 1) {
-    const i64 y = t >= off ? s[t - off] : 0;
-    __syncthreads();
-    s[t] += y;
-    __syncthreads();
+  // inclusive scan of the 256 thread sums: within each wave by shuffles,
+  // then across the four waves through LDS
+  const int lane = t & 63, wv = t >> 6;
+  i64 inc = acc;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const i64 y = __shfl_up(inc, off);
+    if (lane >= off) inc += y;
   }
-  i64 run = boff[blockIdx.x] + s[t] - acc;
+  if (lane == 63) s[wv] = inc;
+  __syncthreads();
+  i64 wbase = 0;
+  for (int w = 0; w < wv; w++) wbase += s[w];
+  i64 run = boff[blockIdx.x] + wbase + inc - acc;
 #pragma unroll
   for (int k = 0; k < PER; k++) {
     const int e = scan_slot(t * PER + k);
