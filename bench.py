@@ -85,7 +85,8 @@ def workload_name(args):
         return (f"laser-driven underdense plasma {nx}x{ny}x{nz}, electrons ramped over 100 cells "
                 f"to 4 ppc (immobile ions neither move nor deposit and are not carried), "
                 f"Gaussian laser a0=2, lambda 32 cells, waist 64 cells, antenna at z=0, "
-                f"order-4 FDTD, order-3 gather, Vay, order-3 Esirkepov, 16^3-node tiles, {prec}")
+                f"order-4 FDTD, order-3 gather, Vay, order-3 Esirkepov, "
+                f"{args.tile_nodes or 8}^3-node tiles, {prec}")
     meth = (f"order-{args.order} gather, {args.pusher.capitalize()}, order-{args.order} "
             f"{'Esirkepov' if args.depo == 'esirkepov' else 'direct'}")
     if args.workload == "bunch":
@@ -104,8 +105,9 @@ def build_laser_workload(args):
     from picsar_amd.laser import emax_from_a0
     from picsar_amd.sim import Simulation
     nx, ny, nz = laser_dims(args)
-    sim = Simulation(nx, ny, nz, 1.0, 1.0, 1.0, 0.0, 0.0, 0.0, (nx + 1) // 16, (ny + 1) // 16,
-                     (nz + 1) // 16, order=3, l_lower_order_in_v=True, pusher="vay",
+    tn = args.tile_nodes or 8
+    sim = Simulation(nx, ny, nz, 1.0, 1.0, 1.0, 0.0, 0.0, 0.0, (nx + 1) // tn, (ny + 1) // tn,
+                     (nz + 1) // tn, order=3, l_lower_order_in_v=True, pusher="vay",
                      depo="esirkepov", norder=4, cfl=0.95, chunk=args.chunk,
                      precision=args.precision)
     lam, ppc, z0, ramp = 32.0, 4, 16, 100
@@ -426,6 +428,8 @@ def main():
     ap.add_argument("--ppc", type=int, default=16)
     ap.add_argument("--tiles", type=int, default=8, help="tiles per axis")
     ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--tile-nodes", type=int, default=0,
+                    help="nodes per tile along each axis of the laser workload (default 8)")
     ap.add_argument("--workload", choices=("thermal", "bunch", "laser"), default="thermal",
                     help="thermal: periodic thermal plasma (electrons + ions); bunch: Gaussian "
                          "gamma=10 electron bunch in a 1 ppc background (uneven tiles); laser: "
