@@ -22,6 +22,7 @@ from __future__ import annotations
 
 import argparse
 import concurrent.futures as cf
+import copy
 import json
 import os
 import sys
@@ -193,6 +194,45 @@ def gauss_state(sim):
     return dv - rho, float(np.linalg.norm(rho))
 
 
+def choose_tiles(args):
+    """Tile size by measurement (--autotune-tiles): the workload is built with
+    8^3- and 16^3-node tiles (and the size given on the command line), run
+    for one step and then timed over three whole steps; the fastest is kept
+    in args.  One candidate is resident at a time."""
+    import gc
+    from picsar_amd.devmem import synchronize
+    laser = args.workload == "laser"
+    if laser:
+        cands = sorted({8, 16, args.tile_nodes or 8})
+    else:
+        cands = sorted({(args.n + 1) // 8, (args.n + 1) // 16, args.tiles})
+    times = {}
+    for c in cands:
+        a = copy.copy(args)
+        if laser:
+            a.tile_nodes = c
+        else:
+            a.tiles = c
+        sim = build_workload(a)
+        sim.step(1, check=False)
+        synchronize()
+        t0 = time.perf_counter()
+        sim.step(3, check=False)
+        synchronize()
+        sim.check()
+        times[c] = (time.perf_counter() - t0) / 3 * 1e3
+        log(f"autotune tiles: {workload_name(a)}: {times[c]:.2f} ms/step")
+        del sim
+        gc.collect()
+    best = min(times, key=times.get)
+    if laser:
+        args.tile_nodes = best
+    else:
+        args.tiles = best
+    key = "tile_nodes" if laser else "tiles_per_axis"
+    return {key: best, "ms_per_step": {str(k): round(v, 3) for k, v in times.items()}}
+
+
 def run_gpu(args):
     from picsar_amd import _lib
     from picsar_amd.devmem import synchronize
@@ -203,6 +243,7 @@ def run_gpu(args):
         _lib.call("pxrhip_timer_elapsed", a, b, ctypes.byref(v))
         return v.value
 
+    tiles_tuned = choose_tiles(args) if args.autotune_tiles else None
     sim = build_workload(args)
     N = sim.total_particles()
     log(f"workload ready: {N} particles, {sim.nx}x{sim.ny}x{sim.nz} cells")
@@ -316,7 +357,8 @@ def run_gpu(args):
     return dict(gauss=gauss, sim=sim, N=N, wall=wall, gpu_ms=total_ms, stage_ms={k: v / K for k, v in stage.items()},
                 per_launch_ms=per_launch_ms, achieved=achieved, bytes_launch=bytes_launch,
                 ke0=ke0, fe0=fe0, ke=ke, fe=fe, ke_init=ke_init, fe_init=fe_init,
-                drift_steps=done + extra, ke_d=ke_d, fe_d=fe_d, tuned=tuned, graph=graph)
+                drift_steps=done + extra, ke_d=ke_d, fe_d=fe_d, tuned=tuned, graph=graph,
+                tiles_tuned=tiles_tuned)
 
 
 # ---------------------------------------------------------------------------
@@ -438,6 +480,8 @@ def main():
                     help="shape order of the gather and the deposition (thermal and bunch)")
     ap.add_argument("--pusher", choices=("vay", "boris"), default="vay")
     ap.add_argument("--depo", choices=("esirkepov", "direct"), default="esirkepov")
+    ap.add_argument("--autotune-tiles", action="store_true",
+                    help="choose the tile size by timing 8^3- and 16^3-node tiles first")
     ap.add_argument("--autotune", action="store_true",
                     help="pick the launch parameters with picsar_amd.autotune before timing")
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
@@ -502,6 +546,7 @@ def main():
         "data": "synthetic (numpy seed 12345, resident on the GPU before timing)",
         "config": {"workload": workload_name(args), "particles": r["N"], "cells": args.n ** 3,
                    "tuning": tuning, "autotuned": r["tuned"] is not None,
+                   "tiles_autotuned": r["tiles_tuned"],
                    "step": "HIP graph of two steps, replayed" if r["graph"] else "eager"},
         "graph": r["graph"],
         "stages": {"ms_per_step": r["stage_ms"],
