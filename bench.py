@@ -1,22 +1,33 @@
 #!/usr/bin/env python3
 """Throughput of the PIC hot path on one MI355X.
 
-Workload (the plain FP64 GPU configuration): a periodic thermal plasma on a
-128^3 grid, dx = dy = dz = 1, dt = 0.95 CFL, electrons and ions at 16
-particles per cell each (67,108,864 macro-particles), Maxwellian momenta
-(u_th = 0.01 c and 0.001 c), weights giving an electron plasma frequency
-0.02 (Debye length 0.5 cell, w_pe*dt = 0.011), order-3 energy-conserving
-gather, Vay pusher,
-order-3 Esirkepov deposition, 2nd-order Yee FDTD, 16^3-node tiles, FP64.
-Synthetic data generated on the device from a fixed seed.
+Default workload (the precision configuration, the one the metric is quoted
+on): a periodic thermal plasma on a 256^3 grid, dx = dy = dz = 1, dt = 0.95
+CFL, electrons and ions at 8 particles per cell each (268,435,456
+macro-particles), Maxwellian momenta (u_th = 0.01 c and 0.001 c), weights
+giving an electron plasma frequency 0.02, order-3 energy-conserving gather,
+Vay pusher, order-3 Esirkepov deposition, 2nd-order Yee FDTD, 8^3-node tiles.
+It is built twice from the same seed and the same arrays: in mixed precision
+(FP64 positions; FP32 momenta, gamma, weights, gather, push and per-particle
+current; FP64 LDS and global current and FDTD) -- the headline `value` -- and
+in FP64, timed beside it.  Synthetic data generated from a fixed seed and
+resident on the GPU before timing.
 
 A timed step is the whole PIC step: fused gather+push+deposit for both
-species, current guard fold, FDTD (B half, E, B half with guard
-exchanges), periodic re-binning and per-tile sort.  `value` is
-particle-steps/s over that whole step; the gather+push+deposit stage alone
-is reported in `stages`.
+species, current guard fold, FDTD (B half, E, B half with guard exchanges),
+periodic re-binning and per-tile sort.  `value` is particle-steps/s over that
+whole step; the gather+push+deposit stage alone is reported in `stages`.
+
+The error of the mixed-precision step against the FP64 one is reported
+under `precision_error`: J and momenta after the first step (the particles
+of both runs are still in the same order then), J and the kinetic energy
+after the timed steps, and the total-energy drift of each run over 1000
+steps.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
+    python bench.py --n 128 --ppc 16 --tiles 8 --precision fp64 --no-compare   # 128^3 FP64
+    python bench.py --workload bunch --n 256 --tiles 32 --precision mixed       # uneven tiles
+    python bench.py --workload laser --n 512 --precision mixed                  # laser plasma
 """
 from __future__ import annotations
 
@@ -34,39 +45,42 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+CLOCK_GHZ = 2.4   # MI355X peak engine clock
+NUM_CU = 256
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
 
 BYTES_PER_PARTICLE_FP64 = 120  # read x,y,z,ux,uy,uz,gaminv,w; write x..gaminv
 BYTES_PER_PARTICLE_MIXED = 84  # positions FP64 (24 + 24), u, gaminv, w FP32 (20 + 16)
 BYTES_PER_CELL = 72            # read E, B (48 B); write J (24 B)
 OMEGA_PE = 0.02                # electron plasma frequency (c = dx = 1)
 
+# committed counter summaries of the fused launch, by workload and precision
+PROFILES = {
+    ("thermal", 256, 8, 32, "mixed"): "fused_c3_mixed",
+    ("thermal", 256, 8, 32, "fp64"): "fused_c3_fp64",
+    ("thermal", 128, 16, 8, "fp64"): "fused_c2",
+    ("thermal", 128, 16, 8, "mixed"): "fused_c2_mixed",
+}
 
-def traffic_profile(args):
-    """The committed counter summary that holds this run's fused launch:
-    the default workload and launch parameters, in FP64 or mixed precision."""
-    if (args.n, args.ppc, args.tiles, args.chunk) != (128, 16, 8, 0) or args.tune or \
-            args.workload != "thermal" or args.autotune or \
+
+def profile_name(args):
+    """Stem of the committed counter summaries that hold this run's fused
+    launch, for the default launch parameters only."""
+    if args.chunk or args.tune or args.autotune or \
             (args.order, args.pusher, args.depo) != (3, "vay", "esirkepov"):
         return None
-    return "fused_c2_hbm_traffic.md" if args.precision == "fp64" else \
-        "fused_c2_mixed_hbm_traffic.md"
+    return PROFILES.get((args.workload, args.n, args.ppc, args.tiles, args.precision))
 
 
-def profiled_traffic(args):
-    """HBM bytes per fused launch from the committed counter summary
-    (FETCH_SIZE and WRITE_SIZE passes scaled by a copy-kernel calibration),
-    valid for the default workload and launch parameters only."""
-    name = traffic_profile(args)
-    if name is None:
-        return None
-    path = os.path.join(ROOT, "profiles", name)
+def profile_value(name, key):
+    """`key = value` line of profiles/<name>.md, or None."""
     try:
-        for line in open(path):
-            if line.startswith("traffic_bytes_per_launch ="):
+        for line in open(os.path.join(ROOT, "profiles", name + ".md")):
+            if line.startswith(key + " ="):
                 return float(line.split("=")[1])
     except OSError:
         pass
@@ -84,7 +98,7 @@ def workload_name(args):
     if args.workload == "laser":
         nx, ny, nz = laser_dims(args)
         return (f"laser-driven underdense plasma {nx}x{ny}x{nz}, electrons ramped over 100 cells "
-                f"to 4 ppc (immobile ions neither move nor deposit and are not carried), "
+                f"to 4 ppc, immobile ions at 1 ppc (frozen, not depositing), "
                 f"Gaussian laser a0=2, lambda 32 cells, waist 64 cells, antenna at z=0, "
                 f"order-4 FDTD, order-3 gather, Vay, order-3 Esirkepov, "
                 f"{args.tile_nodes or 8}^3-node tiles, {prec}")
@@ -100,9 +114,11 @@ def workload_name(args):
 def build_laser_workload(args):
     """A laser-driven underdense plasma: order-4 FDTD; electrons at 4 ppc
     with a weight that ramps linearly over 100 cells (from z = 16) to a
-    plateau at 0.1 critical density; a Gaussian laser (a0 = 2, lambda = 32
-    cells, waist 64 cells) launched by an antenna in the plane z = 0.  The
-    electrons are generated and uploaded slab by slab."""
+    plateau at 0.1 critical density; immobile ions at 1 ppc with the
+    electrons' charge density (frozen and not depositing: the reference's
+    lfreeze and ldodepos); a Gaussian laser (a0 = 2, lambda = 32 cells,
+    waist 64 cells) launched by an antenna in the plane z = 0.  The
+    particles are generated and uploaded slab by slab."""
     from picsar_amd.laser import emax_from_a0
     from picsar_amd.sim import Simulation
     nx, ny, nz = laser_dims(args)
@@ -115,42 +131,49 @@ def build_laser_workload(args):
     k0 = 2 * np.pi / lam
     n_plateau = 0.1 * k0 ** 2       # 0.1 n_c with omega^2 = n_c (m = q = eps0 = 1)
     rng = np.random.default_rng(12345)
-    nper = nx * ny * ppc
 
-    def slabs(step=8):
+    def slabs(ppc_, mom, step=8):
         for k0_ in range(z0, nz, step):
             if (k0_ - z0) % (16 * step) == 0:
-                log(f"generating electrons: z planes {k0_} of {nz}")
+                log(f"generating particles: z planes {k0_} of {nz}")
             k1 = min(k0_ + step, nz)
             cells = np.arange(nx * ny * (k1 - k0_))
             kk = k0_ + cells // (nx * ny)
             jj = (cells // nx) % ny
             ii = cells % nx
-            c = np.repeat(np.stack([ii, jj, kk]), ppc, axis=1).astype(np.float64)
+            c = np.repeat(np.stack([ii, jj, kk]), ppc_, axis=1).astype(np.float64)
             m = c.shape[1]
             x = c[0] + rng.random(m)
             y = c[1] + rng.random(m)
             z = c[2] + rng.random(m)
             dens = n_plateau * np.clip((z - z0) / ramp, 0.0, 1.0)
-            u = [rng.standard_normal(m) * 0.001 for _ in range(3)]
-            yield dict(x=x, y=y, z=z, ux=u[0], uy=u[1], uz=u[2], w=dens / ppc)
+            u = [rng.standard_normal(m) * mom for _ in range(3)]
+            yield dict(x=x, y=y, z=z, ux=u[0], uy=u[1], uz=u[2], w=dens / ppc_)
 
-    sim.add_species_chunks("electrons", -1.0, 1.0, nper * (nz - z0), slabs())
+    sim.add_species_chunks("electrons", -1.0, 1.0, nx * ny * ppc * (nz - z0), slabs(ppc, 0.001))
+    sim.add_species_chunks("ions", 1.0, 1836.0, nx * ny * (nz - z0), slabs(1, 0.0),
+                           lfreeze=True, ldodepos=False)
     sim.add_laser_antenna(spot=(nx / 2.0, ny / 2.0, 0.0), vector=(0, 0, 1), polvector1=(1, 0, 0),
                           lambda_laser=lam, laser_w0=64.0, laser_ctau=2 * lam, t_peak=4 * lam,
                           emax1=emax_from_a0(2.0, lam), charge=-1.0)
     return sim
 
 
-def build_workload(args):
+def make_sim(args, precision):
     from picsar_amd.sim import Simulation
-    if args.workload == "laser":
-        return build_laser_workload(args)
     n = args.n
-    sim = Simulation(n, n, n, 1.0, 1.0, 1.0, 0.0, 0.0, 0.0, args.tiles, args.tiles, args.tiles,
-                     order=args.order, l_lower_order_in_v=True, pusher=args.pusher,
-                     depo=args.depo, norder=2,
-                     cfl=0.95, chunk=args.chunk, precision=args.precision)
+    return Simulation(n, n, n, 1.0, 1.0, 1.0, 0.0, 0.0, 0.0, args.tiles, args.tiles, args.tiles,
+                      order=args.order, l_lower_order_in_v=True, pusher=args.pusher,
+                      depo=args.depo, norder=2, cfl=0.95, chunk=args.chunk, precision=precision)
+
+
+def build_workloads(args, precisions):
+    """The workload in each of `precisions`, every one from the same arrays."""
+    if args.workload == "laser":
+        assert len(precisions) == 1
+        return {precisions[0]: build_laser_workload(args)}
+    sims = {p: make_sim(args, p) for p in precisions}
+    n = args.n
     rng = np.random.default_rng(12345)
     if args.workload == "bunch":
         # a Gaussian electron bunch (sigma 20 cells, peak 64 ppc, gamma = 10
@@ -163,21 +186,28 @@ def build_workload(args):
         uz0 = np.sqrt(gamma ** 2 - 1.0)
         mom = [rng.standard_normal(nb) * 0.01, rng.standard_normal(nb) * 0.01,
                uz0 + rng.standard_normal(nb) * 0.01 * uz0]
-        sim.add_species("bunch", -1.0, 1.0, *pos, *mom, np.full(nb, OMEGA_PE ** 2 / peak))
+        for s in sims.values():
+            s.add_species("bunch", -1.0, 1.0, *pos, *mom, np.full(nb, OMEGA_PE ** 2 / peak))
         del pos, mom
         npart = n ** 3
         pos = [rng.random(npart) * n for _ in range(3)]
         mom = [rng.standard_normal(npart) * 0.01 for _ in range(3)]
-        sim.add_species("background", -1.0, 1.0, *pos, *mom, np.full(npart, OMEGA_PE ** 2))
-        return sim
+        for s in sims.values():
+            s.add_species("background", -1.0, 1.0, *pos, *mom, np.full(npart, OMEGA_PE ** 2))
+        return sims
     npart = n ** 3 * args.ppc
     for name, q, m, uth in (("electrons", -1.0, 1.0, 0.01), ("ions", 1.0, 1836.0, 0.001)):
         pos = [rng.random(npart) * n for _ in range(3)]
         mom = [rng.standard_normal(npart) * uth for _ in range(3)]
         w = np.full(npart, OMEGA_PE ** 2 / args.ppc)
-        sim.add_species(name, q, m, *pos, *mom, w)
+        for s in sims.values():
+            s.add_species(name, q, m, *pos, *mom, w)
         del pos, mom, w
-    return sim
+    return sims
+
+
+def build_workload(args):
+    return build_workloads(args, [args.precision])[args.precision]
 
 
 def gauss_state(sim):
@@ -233,34 +263,25 @@ def choose_tiles(args):
     return {key: best, "ms_per_step": {str(k): round(v, 3) for k, v in times.items()}}
 
 
-def run_gpu(args):
+def elapsed(a, b):
+    import ctypes
+    from picsar_amd import _lib
+    v = ctypes.c_double(0.0)
+    _lib.call("pxrhip_timer_elapsed", a, b, ctypes.byref(v))
+    return v.value
+
+
+def timed_steps(sim, args, label):
+    """W warmup steps, then K steps with HIP events around every stage and
+    every fused launch (recorded on the library's stream)."""
     from picsar_amd import _lib
     from picsar_amd.devmem import synchronize
     import ctypes
-
-    def elapsed(a, b):
-        v = ctypes.c_double(0.0)
-        _lib.call("pxrhip_timer_elapsed", a, b, ctypes.byref(v))
-        return v.value
-
-    tiles_tuned = choose_tiles(args) if args.autotune_tiles else None
-    sim = build_workload(args)
-    N = sim.total_particles()
-    log(f"workload ready: {N} particles, {sim.nx}x{sim.ny}x{sim.nz} cells")
-    tuned = None
-    if args.autotune:
-        from picsar_amd.autotune import autotune
-        tuned = autotune(sim)
-        log(f"autotune: {tuned['best']} ({tuned['ms_per_step']:.2f} ms/step gather+push+deposit)")
-    ke_init, fe_init = sim.kinetic_energy(), sim.field_energy()
-    # the Gauss-law check goes with the drift leg (rho of every particle by
-    # the leaf deposition, outside the timed region)
-    gauss0 = gauss_state(sim) if args.drift_steps > 0 else None
     for _ in range(args.warmup):
         sim.step(1, check=False)
     sim.check()
     synchronize()
-    log(f"{args.warmup} warmup steps done")
+    log(f"{label}: {args.warmup} warmup steps done")
     ke0, fe0 = sim.kinetic_energy(), sim.field_energy()
     K = args.steps
     nsp = len(sim.species)
@@ -288,9 +309,9 @@ def run_gpu(args):
     synchronize()
     wall = time.perf_counter() - t0
     sim.check()
-    log(f"{K} timed steps done in {wall:.2f} s")
+    log(f"{label}: {K} timed steps done in {wall:.2f} s")
     stage = {"gather_push_deposit": 0.0, "current_bcs": 0.0, "fdtd": 0.0, "rebin_sort": 0.0}
-    launches = []
+    launches = {i: [] for i in range(nsp)}
     total_ms = 0.0
     for s in range(K):
         b = s * per
@@ -300,91 +321,152 @@ def run_gpu(args):
         stage["rebin_sort"] += elapsed(b + 2 * nsp + 3, b + 2 * nsp + 4)
         total_ms += elapsed(b, b + 2 * nsp + 4)
         for i in range(nsp):
-            launches.append(elapsed(b + 1 + 2 * i, b + 2 + 2 * i))
-    per_launch_ms = float(np.mean(launches))
+            launches[i].append(elapsed(b + 1 + 2 * i, b + 2 + 2 * i))
     ni, no = ctypes.c_int64(0), ctypes.c_int64(0)
     _lib.call("pxrhip_tiles_stats", ctypes.byref(ni), ctypes.byref(no))
-    log(f"last launch: {ni.value} work items, {no.value} finished from global memory")
-    npart_launch = N / nsp
+    log(f"{label}: last launch: {ni.value} work items, {no.value} finished from global memory")
+    # the dominant kernel: the fused launches of the species that are pushed
+    # and deposit, each over its own particles
+    pushed = [i for i, sp in enumerate(sim.species) if not sp.lfreeze and sp.ldodepos]
+    lms = [float(np.mean(launches[i])) for i in pushed]
+    bpp = BYTES_PER_PARTICLE_MIXED if sim.precision == "mixed" else BYTES_PER_PARTICLE_FP64
     ncells = sim.nx * sim.ny * sim.nz
-    bpp = BYTES_PER_PARTICLE_MIXED if args.precision == "mixed" else BYTES_PER_PARTICLE_FP64
-    bytes_launch = npart_launch * bpp + ncells * BYTES_PER_CELL
-    achieved = bytes_launch / (per_launch_ms * 1e-3) / 1e9
+    bytes_launch = float(np.mean([sim.species[i].npart * bpp + ncells * BYTES_PER_CELL
+                                  for i in pushed]))
+    per_launch_ms = float(np.mean(lms))
+    N = sim.total_particles()
     ke, fe = sim.kinetic_energy(), sim.field_energy()
-    done = args.warmup + K
-    graph = None
-    if args.graph:
-        # the same steps replayed from a HIP graph of two whole steps
-        # (particle counts do not change in a periodic box)
-        g = sim.capture_steps(2)
-        sim.step_graph(g, 1, check=False)
-        nrep = max(1, K // 2)
-        t0g = per * K + 10
-        _lib.call("pxrhip_timer_record", t0g)
-        sim.step_graph(g, nrep, check=False)
-        _lib.call("pxrhip_timer_record", t0g + 1)
-        synchronize()
-        sim.check()
-        gms = elapsed(t0g, t0g + 1) / (2 * nrep)
-        sim.it += 2 * (nrep + 1)
-        done += 2 * (nrep + 1)
-        graph = {"steps": 2 * nrep, "ms_per_step": gms}
-        del g
-        log(f"graph: {2 * nrep} steps replayed at {gms:.3f} ms/step")
-        ke, fe = sim.kinetic_energy(), sim.field_energy()
-    # untimed steps up to the energy-drift horizon (counted from the start)
-    extra = max(0, args.drift_steps - done)
+    return dict(N=N, wall=wall, gpu_ms=total_ms, K=K, stage_ms={k: v / K for k, v in stage.items()},
+                per_launch_ms=per_launch_ms, bytes_launch=bytes_launch,
+                achieved=bytes_launch / (per_launch_ms * 1e-3) / 1e9,
+                value=N * K / (total_ms * 1e-3), ke0=ke0, fe0=fe0, ke=ke, fe=fe,
+                items=ni.value, overflow=no.value)
+
+
+def graph_steps(sim, K):
+    """The same steps replayed from a HIP graph of two whole steps."""
+    from picsar_amd import _lib
+    from picsar_amd.devmem import synchronize
+    g = sim.capture_steps(2)
+    sim.step_graph(g, 1, check=False)
+    nrep = max(1, K // 2)
+    t0g = 60000
+    _lib.call("pxrhip_timer_record", t0g)
+    sim.step_graph(g, nrep, check=False)
+    _lib.call("pxrhip_timer_record", t0g + 1)
+    synchronize()
+    sim.check()
+    gms = elapsed(t0g, t0g + 1) / (2 * nrep)
+    del g
+    log(f"graph: {2 * nrep} steps replayed at {gms:.3f} ms/step")
+    return {"steps": 2 * nrep, "ms_per_step": gms}
+
+
+def run_to(sim, total, label):
+    """Untimed steps until sim.it == total (a progress line every half minute)."""
+    from picsar_amd.devmem import synchronize
+    extra = max(0, total - sim.it)
     t_log = time.perf_counter()
     for i in range(extra):
         sim.step(1, check=False)
-        if time.perf_counter() - t_log > 30.0:  # a progress line every half minute
+        if time.perf_counter() - t_log > 30.0:
             synchronize()
-            log(f"untimed steps: {i + 1} of {extra}")
+            log(f"{label}: untimed steps: {i + 1} of {extra}")
             t_log = time.perf_counter()
     sim.check()
     synchronize()
     if extra:
-        log(f"{extra} more untimed steps for the energy drift over {done + extra} steps")
-    ke_d, fe_d = sim.kinetic_energy(), sim.field_energy()
-    gauss = None
-    if gauss0 is not None:
-        r1, rho1 = gauss_state(sim)
-        gauss = {"steps": done + extra, "rho_norm_initial": gauss0[1], "rho_norm_final": rho1,
-                 "residual_change_rel": float(np.linalg.norm(r1 - gauss0[0])) / gauss0[1],
-                 "what": "|(div E - rho)_final - (div E - rho)_initial| / |rho_initial| on the "
-                         "nodes 0..n-1 of each axis"}
-        del r1
-    return dict(gauss=gauss, sim=sim, N=N, wall=wall, gpu_ms=total_ms, stage_ms={k: v / K for k, v in stage.items()},
-                per_launch_ms=per_launch_ms, achieved=achieved, bytes_launch=bytes_launch,
-                ke0=ke0, fe0=fe0, ke=ke, fe=fe, ke_init=ke_init, fe_init=fe_init,
-                drift_steps=done + extra, ke_d=ke_d, fe_d=fe_d, tuned=tuned, graph=graph,
-                tiles_tuned=tiles_tuned)
+        log(f"{label}: {extra} more untimed steps, {sim.it} in all")
+
+
+def energy_record(sim, e_init, label):
+    ke, fe = sim.kinetic_energy(), sim.field_energy()
+    return {"steps": sim.it, "kinetic_initial": e_init[0], "field_initial": e_init[1],
+            "kinetic_final": ke, "field_final": fe,
+            "relative_drift": (ke + fe - e_init[0] - e_init[1]) / (e_init[0] + e_init[1])}
+
+
+def rel(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+def current_error(sa, sb):
+    """Relative L2 difference of J of sa against sb, all three components."""
+    num = den = 0.0
+    for k in ("jx", "jy", "jz"):
+        a, b = sa.fields[k].numpy(), sb.fields[k].numpy()
+        num += float(np.sum((a - b) ** 2))
+        den += float(np.sum(b ** 2))
+    return (num / max(den, 1e-300)) ** 0.5
+
+
+def first_step_errors(smx, s64, sample):
+    """Step 1 of both runs, stage by stage: after the fused gather + push +
+    deposit the particles of both runs are still in the same order (same
+    initial arrays, same stable binning), so momenta are compared particle
+    by particle on the first `sample` of each species; J after the guard
+    fold."""
+    out = {}
+    for s in (smx, s64):
+        s.gather_push_deposit()
+        s.current_bcs()
+    num = den = 0.0
+    for spm, sp6 in zip(smx.species, s64.species):
+        n = min(sample, spm.npart)
+        for k in ("ux", "uy", "uz"):
+            a = spm.active.arrays[k].numpy(n).astype(np.float64)
+            b = sp6.active.arrays[k].numpy(n)
+            num += float(np.sum((a - b) ** 2))
+            den += float(np.sum(b ** 2))
+    out["momentum_rel_step1"] = (num / den) ** 0.5
+    out["momentum_sample"] = (f"the first {sample} particles of each species (cell-sorted order) "
+                              f"after the first push")
+    out["j_rel_step1"] = current_error(smx, s64)
+    for s in (smx, s64):
+        s.field_step()
+        s.particle_bcs_and_sorting()
+        s.it += 1
+        s.check()
+    return out
 
 
 # ---------------------------------------------------------------------------
 # CPU baseline: the reference's FP64 kernels on the host cores, tile by tile
 # ---------------------------------------------------------------------------
+def cpu_threads():
+    """Host threads for the CPU baseline: the CPUs this process may run on
+    (its affinity; one GPU's share of the host on the GPU boxes)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, n)
+
+
 def cpu_baseline(args, port=False, seconds=None):
-    """The reference's compiled kernels on the host cores ("reference"), or
-    the C restatement of the oracle ("port") when they are absent or when
-    ``port`` asks for it."""
+    """The reference's compiled kernels, built with its production flags
+    (-O3 -fopenmp), on the host cores ("reference"), or the C restatement of
+    the oracle ("port") when they are absent or when ``port`` asks for it.
+    The sample is whole tiles of the timed workload: the tile size and
+    particles per cell of the run, electrons and ions."""
     from oracle import load_oracle, load_reference
-    ref = None if port else load_reference()
+    ref = None if port else load_reference(production=True)
     kind = "reference" if ref is not None else "port"
     lib = ref if ref is not None else load_oracle()
-    nc, ng, ppc = 15, 3, args.ppc        # one 16^3-node tile of the workload
+    tnodes = (args.n + 1) // args.tiles     # nodes per tile of the workload
+    nc, ng, ppc = tnodes - 1, 3, args.ppc
     shape = (nc + 1 + 2 * ng,) * 3
     dt = 0.95 / np.sqrt(3.0)
-    threads = min(16, os.cpu_count() or 1)
+    threads = cpu_threads()
     rng = np.random.default_rng(7)
     fields = [np.asfortranarray(rng.standard_normal(shape) * 1e-3) for _ in range(6)]
-    ntile = 16 ** 3 * ppc
+    ntile = tnodes ** 3 * ppc
     pool = []
     for i in range(threads):
         r = np.random.default_rng(1000 + i)
         sp = []
         for q, m, uth in ((-1.0, 1.0, 0.01), (1.0, 1836.0, 0.001)):
-            x, y, z = (r.uniform(-0.5, 15.5, ntile) for _ in range(3))
+            x, y, z = (r.uniform(-0.5, nc + 0.5, ntile) for _ in range(3))
             ux, uy, uz = (r.standard_normal(ntile) * uth for _ in range(3))
             g = 1.0 / np.sqrt(1 + ux * ux + uy * uy + uz * uz)
             sp.append((q, m, (x, y, z, ux, uy, uz, g, np.ones(ntile))))
@@ -428,47 +510,49 @@ def cpu_baseline(args, port=False, seconds=None):
             for f in futs:
                 total += f.result()
     wall = time.perf_counter() - t0
+    build = ("the reference's Fortran kernels built with amdflang -O3 -fopenmp -DLIBRARY "
+             "(its Makefile's optimisation flags)") if kind == "reference" else \
+        "the oracle's C restatement, gcc -O2"
     return {"value": total / wall, "unit": "particle-steps/s", "cores": threads, "kind": kind,
-            "sample": f"{total} particle-steps in {wall:.1f} s: whole 16^3-node tiles of the "
-                      f"workload ({ppc} ppc electrons + ions), order-3 gather + Vay + pushxyz + "
-                      f"order-3 Esirkepov into the tile J block, {threads} host threads over "
-                      f"tiles (FP64)"}
+            "nproc": os.cpu_count(), "build": build,
+            "sample": f"{total} particle-steps in {wall:.1f} s: whole {tnodes}^3-node tiles of "
+                      f"the workload ({ppc} ppc electrons + ions), order-3 gather + Vay + "
+                      f"pushxyz + order-3 Esirkepov into the tile J block, {threads} host "
+                      f"threads over tiles (FP64)"}
 
 
-def precision_comparison(args):
-    """The same step at 256^3 cells, 8 ppc electrons + ions, 8^3-node tiles
-    (the fastest of the tile sizes measured at this size, in both precisions),
-    order-3 Esirkepov, in FP64 and in mixed precision (20 timed steps each):
-    the mixed-precision step against the FP64 one."""
-    import copy
-    out = {}
-    for prec in ("fp64", "mixed"):
-        a = copy.copy(args)
-        a.n, a.ppc, a.tiles, a.precision = 256, 8, 32, prec
-        a.steps, a.warmup, a.drift_steps, a.autotune, a.graph = 20, 2, 0, False, False
-        log(f"comparison: {workload_name(a)}")
-        r = run_gpu(a)
-        del r["sim"]
-        out[prec] = {"value": r["N"] * a.steps / (r["gpu_ms"] * 1e-3),
-                     "ms_per_step": r["gpu_ms"] / a.steps,
-                     "gather_push_deposit_ms_per_step": r["stage_ms"]["gather_push_deposit"],
-                     "launch_ms": r["per_launch_ms"],
-                     "roofline_frac": r["achieved"] / HBM_PEAK_GBS}
-        out["workload"] = workload_name(a).replace(", mixed precision", "").replace(", FP64", "")
-        out["particles"] = r["N"]
-    out["unit"] = "particle-steps/s"
-    out["mixed_over_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
-    return out
+def roofline_record(r, args, prec):
+    a = copy.copy(args)
+    a.precision = prec
+    name = profile_name(a)
+    rec = {"bound": "hbm", "achieved": r["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": r["achieved"] / HBM_PEAK_GBS,
+           "traffic": profile_value(name + "_hbm_traffic", "traffic_bytes_per_launch")
+           if name else None,
+           "traffic_source": f"profiles/{name}_hbm_traffic.md (rocprofv3 FETCH_SIZE, "
+                             f"WRITE_SIZE, copy-calibrated)" if name else None,
+           "kernel": "k_fused (gather+push+deposit, one species per launch)",
+           "launch_ms": r["per_launch_ms"], "bytes_per_launch": r["bytes_launch"]}
+    if name:
+        # compute-side fractions of the same kernel from its counter summary:
+        # VALU instructions issued and LDS instructions' active cycles per CU
+        # cycle of the launch (one wave instruction per cycle per CU is the
+        # VALU issue peak; FP64 FMA at that rate is the 78.6 TFLOP/s peak)
+        for key in ("valu_frac", "lds_frac", "valu_insts_per_particle",
+                    "lds_insts_per_particle"):
+            rec[key] = profile_value(name + "_counters", key)
+        rec["counters_source"] = f"profiles/{name}_counters.md"
+    return rec
 
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--n", type=int, default=128, help="cells per axis")
-    ap.add_argument("--ppc", type=int, default=16)
-    ap.add_argument("--tiles", type=int, default=8, help="tiles per axis")
+    ap.add_argument("--n", type=int, default=256, help="cells per axis")
+    ap.add_argument("--ppc", type=int, default=8)
+    ap.add_argument("--tiles", type=int, default=32, help="tiles per axis")
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--tile-nodes", type=int, default=0,
                     help="nodes per tile along each axis of the laser workload (default 8)")
@@ -484,9 +568,13 @@ def main():
                     help="choose the tile size by timing 8^3- and 16^3-node tiles first")
     ap.add_argument("--autotune", action="store_true",
                     help="pick the launch parameters with picsar_amd.autotune before timing")
-    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
+    ap.add_argument("--precision", choices=("fp64", "mixed"), default="mixed",
                     help="fp64: every particle array FP64; mixed: FP64 positions, FP32 momenta, "
-                         "gamma and weights, FP32 gather/push/LDS current, FP64 J and FDTD")
+                         "gamma and weights, FP32 gather/push/per-particle current, FP64 J and "
+                         "FDTD")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="time only --precision (by default a mixed-precision run is timed "
+                         "beside an FP64 run of the same arrays, and their difference reported)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--drift-steps", type=int, default=1000,
                     help="total steps (warmup + timed + untimed) over which the energy drift "
@@ -498,85 +586,129 @@ def main():
                          "report that as the value")
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
                     help="launch parameter of the tiled kernels (pxrhip_set_tuning)")
-    ap.add_argument("--no-comparison", action="store_true",
-                    help="skip the FP64 / mixed-precision comparison at 256^3 cells, 8 ppc, "
-                         "8^3-node tiles that follows the default workload")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed region, copy 1 GiB with one plain device copy "
                          "kernel (the reference point of HBM traffic counters)")
+    ap.add_argument("--sample", type=int, default=1 << 24,
+                    help="particles per species whose momenta are compared after step 1")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: the path runs "
               "on one MI355X and is not sharded over several GPUs", file=sys.stderr)
         sys.exit(2)
     from picsar_amd import _lib
+    import ctypes
     _lib.call("pxrhip_init", 0)
-    tuning = {}
     for kv in args.tune:
         k, v = kv.split("=")
         _lib.call("pxrhip_set_tuning", k.encode(), int(v))
-    r = run_gpu(args)
+    tiles_tuned = choose_tiles(args) if args.autotune_tiles else None
+    compare = (not args.no_compare and args.workload != "laser" and not args.graph)
+    precs = [args.precision] + ([p for p in ("fp64", "mixed") if p != args.precision]
+                                if compare else [])
+    sims = build_workloads(args, precs)
+    main_sim = sims[args.precision]
+    log(f"workload ready: {main_sim.total_particles()} particles, "
+        f"{main_sim.nx}x{main_sim.ny}x{main_sim.nz} cells, precisions {precs}")
+    tuned = None
+    if args.autotune:
+        from picsar_amd.autotune import autotune
+        tuned = autotune(main_sim)
+        log(f"autotune: {tuned['best']} ({tuned['ms_per_step']:.2f} ms/step gather+push+deposit)")
+    e_init = {p: (s.kinetic_energy(), s.field_energy()) for p, s in sims.items()}
+    gauss0 = {p: gauss_state(s) for p, s in sims.items()} if args.drift_steps > 0 else {}
+    perr = None
+    if compare:
+        perr = first_step_errors(sims["mixed"], sims["fp64"], args.sample)
+        log(f"step 1: J error {perr['j_rel_step1']:.2e}, momentum error "
+            f"{perr['momentum_rel_step1']:.2e} (mixed against FP64)")
+    runs = {}
+    for p in precs:
+        runs[p] = timed_steps(sims[p], args, p)
+    graph = graph_steps(main_sim, args.steps) if args.graph else None
+    if compare:
+        perr["steps"] = sims["mixed"].it
+        perr["j_rel"] = current_error(sims["mixed"], sims["fp64"])
+        k6 = runs["fp64"]["ke"]
+        perr["kinetic_energy_rel"] = abs(runs["mixed"]["ke"] - k6) / k6
+    drift = {}
+    gauss = {}
+    for p in precs:
+        run_to(sims[p], args.drift_steps, p)
+        drift[p] = energy_record(sims[p], e_init[p], p)
+        if gauss0.get(p) is not None:
+            r1, rho1 = gauss_state(sims[p])
+            gauss[p] = {"steps": sims[p].it, "rho_norm_initial": gauss0[p][1],
+                        "rho_norm_final": rho1,
+                        "residual_change_rel": float(np.linalg.norm(r1 - gauss0[p][0])) /
+                        gauss0[p][1]}
+    if compare and args.drift_steps > 0:
+        perr["j_rel_final"] = current_error(sims["mixed"], sims["fp64"])
+        perr["final_steps"] = sims["mixed"].it
+        perr["energy_drift_fp64"] = drift["fp64"]["relative_drift"]
+        perr["energy_drift_mixed"] = drift["mixed"]["relative_drift"]
+        perr["note"] = ("J and momentum differences of the mixed-precision run against the FP64 "
+                        "run from the same arrays; after many steps the two runs separate as any "
+                        "two plasma runs with different round-off do, so the later differences "
+                        "bound the error rather than measure it")
+    tuning = {}
     for k in ("fused.lds_bytes", "fused.wg_per_cu", "fused.spread", "fused.chunk",
               "fused.j_lds_bytes", "fused.jpad", "fused.margin_pct", "fused.sstride",
-              "mixed.jacc",
-              "mixed.lds_bytes", "mixed.wg_per_cu", "last.wg_per_cu", "last.lds_bytes"):
-        import ctypes
+              "mixed.jacc", "mixed.lds_bytes", "mixed.wg_per_cu", "last.wg_per_cu",
+              "last.lds_bytes", "rebin.stable"):
         v = ctypes.c_int64(0)
         _lib.call("pxrhip_get_tuning", k.encode(), ctypes.byref(v))
         tuning[k] = v.value
-    K = args.steps
-    value = r["N"] * K / (r["gpu_ms"] * 1e-3)
-    if r["graph"] is not None:  # --graph: the replayed steps are the measurement
-        r["graph"]["eager_ms_per_step"] = r["gpu_ms"] / K
-        value = r["N"] / (r["graph"]["ms_per_step"] * 1e-3)
+    r = runs[args.precision]
+    value = r["value"]
+    if graph is not None:  # --graph: the replayed steps are the measurement
+        graph["eager_ms_per_step"] = r["gpu_ms"] / r["K"]
+        value = r["N"] / (graph["ms_per_step"] * 1e-3)
     gpd = r["N"] / (r["stage_ms"]["gather_push_deposit"] * 1e-3)
+    cells = main_sim.nx * main_sim.ny * main_sim.nz
     out = {
         "metric": "particle-steps/s (gather+push+deposit), FP32-mixed vs FP64, % of HBM peak",
         "value": value,
         "unit": "particle-steps/s",
         "n_gpus": 1,
-        "steps": K,
+        "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": r["gpu_ms"] / K if r["graph"] is None else r["graph"]["ms_per_step"],
+        "ms_per_step": r["gpu_ms"] / r["K"] if graph is None else graph["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32 (gather, push, LDS current) + f64 "
-                                                          "(positions, J, FDTD)",
+        "dtype": "f64" if args.precision == "fp64" else "f32 (gather, push, per-particle "
+                                                          "current) + f64 (positions, J, FDTD)",
         "data": "synthetic (numpy seed 12345, resident on the GPU before timing)",
-        "config": {"workload": workload_name(args), "particles": r["N"], "cells": args.n ** 3,
-                   "tuning": tuning, "autotuned": r["tuned"] is not None,
-                   "tiles_autotuned": r["tiles_tuned"],
-                   "step": "HIP graph of two steps, replayed" if r["graph"] else "eager"},
-        "graph": r["graph"],
+        "config": {"workload": workload_name(args), "particles": r["N"], "cells": cells,
+                   "tuning": tuning, "autotuned": tuned is not None,
+                   "tiles_autotuned": tiles_tuned,
+                   "step": "HIP graph of two steps, replayed" if graph else "eager"},
+        "graph": graph,
         "stages": {"ms_per_step": r["stage_ms"],
                    "gather_push_deposit_particle_steps_per_s": gpd,
-                   "wall_s_timed_region": r["wall"]},
-        "roofline": {"bound": "hbm", "achieved": r["achieved"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": r["achieved"] / HBM_PEAK_GBS,
-                     "traffic": profiled_traffic(args),
-                     "traffic_source": (f"profiles/{traffic_profile(args)} (rocprofv3 "
-                                        "FETCH_SIZE, WRITE_SIZE, copy-calibrated)")
-                     if traffic_profile(args) else None,
-                     "kernel": "k_fused (gather+push+deposit, one species per launch)",
-                     "launch_ms": r["per_launch_ms"],
-                     "bytes_per_launch": r["bytes_launch"]},
-        "energy": {"kinetic_initial": r["ke_init"], "field_initial": r["fe_init"],
-                   "steps": r["drift_steps"], "kinetic_final": r["ke_d"],
-                   "field_final": r["fe_d"],
-                   "relative_drift": (r["ke_d"] + r["fe_d"] - r["ke_init"] - r["fe_init"]) /
-                   (r["ke_init"] + r["fe_init"]),
-                   "relative_drift_timed_region": (r["ke"] + r["fe"] - r["ke0"] - r["fe0"]) /
-                   (r["ke0"] + r["fe0"]),
-                   "note": ("the laser antenna injects energy: the change is the injected "
-                            "energy, not an error of the step") if args.workload == "laser"
-                   else "periodic, no sources: the change is the step's error"},
-        "gauss_law": r["gauss"],
+                   "wall_s_timed_region": r["wall"],
+                   "work_items_last_launch": r["items"],
+                   "work_items_from_global_memory": r["overflow"]},
+        "roofline": roofline_record(r, args, args.precision),
+        "energy": dict(drift[args.precision],
+                       relative_drift_timed_region=(r["ke"] + r["fe"] - r["ke0"] - r["fe0"]) /
+                       (r["ke0"] + r["fe0"]),
+                       note=("the laser antenna injects energy: the change is the injected "
+                             "energy, not an error of the step") if args.workload == "laser"
+                       else "periodic, no sources: the change is the step's error"),
+        "gauss_law": gauss.get(args.precision),
     }
-    del r["sim"]
-    if (not args.no_comparison and args.workload == "thermal" and args.n == 128
-            and args.precision == "fp64" and args.order == 3 and not args.graph):
-        out["precision_comparison"] = precision_comparison(args)
+    if compare:
+        other = [p for p in precs if p != args.precision][0]
+        ro = runs[other]
+        out[other] = {"value": ro["value"], "ms_per_step": ro["gpu_ms"] / ro["K"],
+                      "stages_ms_per_step": ro["stage_ms"],
+                      "roofline": roofline_record(ro, args, other),
+                      "energy": drift[other], "gauss_law": gauss.get(other)}
+        out["mixed_over_fp64"] = runs["mixed"]["value"] / runs["fp64"]["value"]
+        out["precision_error"] = perr
+    del sims, main_sim
     if args.calibrate_copy:
         from picsar_amd.devmem import DeviceArray, synchronize
         a = DeviceArray(1 << 27)
@@ -587,7 +719,7 @@ def main():
         synchronize()
         out["calibration_copy_bytes"] = 2 * a.nbytes
         del a, b
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and args.workload == "thermal":
         log(f"timing the CPU baseline for about {args.cpu_seconds:.0f} s")
         out["cpu_baseline"] = cpu_baseline(args)
         if out["cpu_baseline"]["kind"] == "reference":

@@ -59,7 +59,12 @@ int pxrhip_bad_count(long long *count, int reset);
  * recorded instead of run (only calls that do not synchronise may be
  * recorded: the tiled step, the field solvers and guard exchanges; not
  * copies to the host or checked leaf calls).  The recorded work keeps the
- * pointers it was given.  pxrhip_graph_launch replays it on the stream. */
+ * pointers it was given.  pxrhip_graph_launch replays it on the stream.
+ * It also keeps the library's own scratch buffers and tile table: when a
+ * later call grows a scratch buffer or switches to another grid
+ * description, those are reallocated, pxrhip_graph_end fails if that
+ * happened during the capture, and pxrhip_graph_launch of a graph captured
+ * before fails with PXRHIP_ERR_ARG instead of replaying on freed memory. */
 int pxrhip_graph_begin(void);
 int pxrhip_graph_end(void **graph);
 int pxrhip_graph_launch(void *graph);
@@ -273,14 +278,23 @@ typedef struct {
   double sorting_dx, sorting_dy, sorting_dz;
   double sorting_shiftx, sorting_shifty, sorting_shiftz;
   /* particle boundary at x_min, x_max, y_min, y_max, z_min, z_max
-   * (pbound_x_min ... of particle_bcs_mpi_blocking,
-   * particle_boundaries.F90:1106-1300): PXRHIP_PBOUND_* */
+   * (pbound_x_min ..., modules.F90): PXRHIP_PBOUND_* */
   int64_t pbound[6];
+  /* the particle exchange of particle_bcs (particle_boundaries.F90:83-217,
+   * the reference's partcom): 0, its default (control_file.F90:139), is
+   * particle_bcs_tiles_and_mpi_3d (:2107-3031), whose domain boundaries
+   * know absorbing and reinjecting faces and treat every other value,
+   * reflecting included, as periodic (:2308-2333); 1 and 2 are
+   * particle_bcs_tiles followed by particle_bcs_mpi_blocking (:1040-1371,
+   * mpicom_curr = 1, its default), where a reflecting face moves the
+   * particle one cell back in and reverses its normal momentum. */
+  int64_t partcom;
 } pxrhip_grid;
 
 #define PXRHIP_PBOUND_PERIODIC 0
 #define PXRHIP_PBOUND_ABSORBING 1   /* the particle is removed */
-#define PXRHIP_PBOUND_REFLECTING 2  /* one cell back in, normal momentum reversed */
+#define PXRHIP_PBOUND_REFLECTING 2  /* partcom 1, 2: one cell back in, normal momentum
+                                       reversed; partcom 0: periodic */
 #define PXRHIP_PBOUND_REINJECTING 3 /* mirrored in, at rest, kept in its tile */
 
 typedef struct {
@@ -342,8 +356,29 @@ int pxrhip_tiles_gather_push_deposit(
     int64_t order, int64_t l_lower_order_in_v, int64_t pusher,
     int64_t depo_algo, int64_t chunk);
 
+/* field_gathering_plus_particle_pusher (particle_pusher_manager_3d.F90:
+ * 431-672) on a tiled species: the gather, push and position push of
+ * pxrhip_tiles_gather_push_deposit without its deposition (pusher 0 Boris,
+ * 1 Vay).  The species should be binned; particles that moved since still
+ * gather, from global memory. */
+int pxrhip_tiles_gather_push(
+    const pxrhip_grid *grid, const pxrhip_species *sp, const double *exg,
+    const double *eyg, const double *ezg, const double *bxg, const double *byg,
+    const double *bzg, double dt, int64_t order, int64_t l_lower_order_in_v,
+    int64_t pusher, int64_t chunk);
+
+/* The current of a binned species, ADDED into jx..jz: the per-species work
+ * of pxrdepose_currents_on_grid_jxjyjz (current_deposition_manager_3d.F90:
+ * 224-1830) with depose_jxjyjz's shapes (depo_algo 0 Esirkepov, 3 direct,
+ * order 1..3), from the positions and momenta as they stand -- after
+ * particle_bcs, as submain.F90:125-140 orders the step.  The current
+ * blocks of the work items are accumulated in LDS. */
+int pxrhip_tiles_depose_currents(
+    const pxrhip_grid *grid, const pxrhip_species *sp, double *jx, double *jy,
+    double *jz, double dt, int64_t order, int64_t depo_algo, int64_t chunk);
+
 /* particle_bcs on one rank (particle_boundaries.F90:83-426 and the domain
- * boundaries of particle_bcs_mpi_blocking, :1040-1300, per grid->pbound;
+ * boundaries of the exchange that grid->partcom selects, per grid->pbound;
  * absorbed particles are dropped, so dst may hold fewer) followed by
  * particle_sorting_sub (sorting.F90:106-215): every particle that left the
  * box is wrapped, reflected, reinjected or removed (in src, in place), then
@@ -352,7 +387,10 @@ int pxrhip_tiles_gather_push_deposit(
  * cell (pxr_particle_bin_sorting's key).  The result is written to dst,
  * which must have the same capacity; dst's tile_off, tile_np and bin_start
  * are filled in (tile_off[ntiles] is the number of particles kept).
- * Particles of one cell keep no particular order among themselves. */
+ * The sort is stable: the particles of one cell keep their order in src,
+ * as pxr_particle_bin_sorting's counting sort keeps it (sorting.F90:
+ * 333-343), so the result does not depend on the order in which GPU waves
+ * ran (tuning key "rebin.stable" 0 drops that order for a cheaper scatter). */
 int pxrhip_tiles_rebin_sort(const pxrhip_grid *grid, pxrhip_species *src,
                             pxrhip_species *dst);
 
@@ -381,7 +419,9 @@ int pxrhip_tiles_depose_rho(const pxrhip_grid *grid, const pxrhip_species *sp,
  * (LDS current of a mixed-precision species: 0 FP32 atomics, 1 FP32
  * atomics compensated by TwoSum into a second array, 2 FP64 atomics),
  * "mixed.lds_bytes" and "mixed.wg_per_cu" (the launch of a mixed-precision
- * species; fused.lds_bytes and fused.wg_per_cu are those of an FP64 one).
+ * species; fused.lds_bytes and fused.wg_per_cu are those of an FP64 one),
+ * "rebin.stable" (1, the default: pxrhip_tiles_rebin_sort keeps the source
+ * order inside a cell; 0: the order in which the GPU's waves arrive).
  * wg_per_cu 0 (the default) is automatic: three workgroups per CU with
  * 52224 bytes each when a work item of four sorting rows of the widest tile
  * fits that, else two with 79616 bytes; a wg_per_cu above 0 takes the

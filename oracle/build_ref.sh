@@ -20,12 +20,22 @@
 #                            constants; the MPI module it also uses is not
 #                            needed by these two)
 # -DLIBRARY selects normalised units (clight = mu0 = eps0 = 1).
+#
+# Two libraries are built from the same sources:
+#   libpxrref.so     -O2 -ffp-contract=off: the golden vectors (no contraction,
+#                    so that the oracle's operation order can match it);
+#   libpxrref_o3.so  -O3 -fopenmp, the reference's production flags
+#                    (Makefile:56, without -g -fbounds-check): the CPU baseline.
+#                    The leaf kernels hold !$OMP SIMD directives only, which
+#                    -fopenmp turns on; threads run over tiles, as the
+#                    reference's managers do.
 set -euo pipefail
 REF=${PXR_REF:-/root/reference}
 HERE=$(cd "$(dirname "$0")" && pwd)
 OUT="$HERE/_ref"
 FC=${FC:-amdflang}
-FFLAGS="-cpp -DLIBRARY -O2 -fPIC -ffp-contract=off"
+FFLAGS_GOLDEN="-cpp -DLIBRARY -O2 -fPIC -ffp-contract=off"
+FFLAGS_PROD="-cpp -DLIBRARY -O3 -fopenmp -fPIC"
 SRC="$REF/src"
 if [ ! -d "$SRC" ]; then
   echo "build_ref: reference tree not found at $REF" >&2
@@ -59,25 +69,36 @@ G="$SRC/diags/diags.F90"
   printf 'END MODULE pxr_ref_diags\n'; } > "$TMP/divdiags.F90"
 
 cd "$TMP"
-$FC $FFLAGS -c pxr_consts.F90 -o pxr_consts.o
-OBJS="pxr_consts.o"
-for f in field_gathering/energy_conserving/field_gathering_o1_3d.F90 \
-         field_gathering/energy_conserving/field_gathering_o2_3d.F90 \
-         field_gathering/energy_conserving/field_gathering_o3_3d.F90 \
-         particle_pushers/boris_pusher/boris_3d.F90 \
-         particle_pushers/vay_pusher/vay_3d.F90 \
-         particle_deposition/current_deposition/esirkepov/esirkepov_3d.F90 \
-         particle_deposition/charge_deposition/charge_deposition_3d.F90 \
-         field_solvers/Maxwell/yee_solver/yee.F90; do
-  o=$(basename "$f" .F90).o
-  $FC $FFLAGS -c "$SRC/$f" -o "$o"
-  OBJS="$OBJS $o"
-done
-for f in direct_scalar binsort fdweights laserprof divdiags; do
-  $FC $FFLAGS -c "$f.F90" -o "$f.o"
-  OBJS="$OBJS $f.o"
-done
-mkdir -p "$OUT"
-$FC -shared -o "$OUT/libpxrref.so.tmp" $OBJS
-mv "$OUT/libpxrref.so.tmp" "$OUT/libpxrref.so"
-echo "build_ref: wrote $OUT/libpxrref.so"
+# compile every unit with the flags $1 into the shared library $2
+build_lib() {
+  local FFLAGS="$1" LIB="$2" LDF="$3"
+  mkdir -p "$TMP/$LIB.d"
+  cd "$TMP/$LIB.d"
+  $FC $FFLAGS -c ../pxr_consts.F90 -o pxr_consts.o
+  local OBJS="pxr_consts.o"
+  for f in field_gathering/energy_conserving/field_gathering_o1_3d.F90 \
+           field_gathering/energy_conserving/field_gathering_o2_3d.F90 \
+           field_gathering/energy_conserving/field_gathering_o3_3d.F90 \
+           particle_pushers/boris_pusher/boris_3d.F90 \
+           particle_pushers/vay_pusher/vay_3d.F90 \
+           particle_deposition/current_deposition/esirkepov/esirkepov_3d.F90 \
+           particle_deposition/charge_deposition/charge_deposition_3d.F90 \
+           field_solvers/Maxwell/yee_solver/yee.F90; do
+    local o=$(basename "$f" .F90).o
+    $FC $FFLAGS -c "$SRC/$f" -o "$o"
+    OBJS="$OBJS $o"
+  done
+  for f in direct_scalar binsort fdweights laserprof divdiags; do
+    $FC $FFLAGS -c "../$f.F90" -o "$f.o"
+    OBJS="$OBJS $f.o"
+  done
+  mkdir -p "$OUT"
+  $FC -shared $LDF -o "$OUT/$LIB.tmp" $OBJS
+  mv "$OUT/$LIB.tmp" "$OUT/$LIB"
+  echo "build_ref: wrote $OUT/$LIB"
+  cd "$TMP"
+}
+build_lib "$FFLAGS_GOLDEN" libpxrref.so "" &
+build_lib "$FFLAGS_PROD" libpxrref_o3.so "-fopenmp" &
+wait %1
+wait %2

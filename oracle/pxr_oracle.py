@@ -19,6 +19,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libpxrref.so")
+# the same kernels built with the reference's production flags (-O3 -fopenmp)
+REF_O3_SO = os.path.join(HERE, "_ref", "libpxrref_o3.so")
 
 _i64 = C.c_int64
 _f64 = C.c_double
@@ -480,8 +482,12 @@ def load_oracle() -> Oracle:
     return Oracle()
 
 
-def load_reference():
-    """The compiled reference kernels, or None where they were not built."""
-    if not os.path.exists(REF_SO):
+def load_reference(production: bool = False):
+    """The compiled reference kernels, or None where they were not built:
+    the golden-vector build (-O2 -ffp-contract=off), or with production
+    the build with the reference's own flags (-O3 -fopenmp) that the CPU
+    baseline times."""
+    path = REF_O3_SO if production else REF_SO
+    if not os.path.exists(path):
         return None
-    return Reference()
+    return Reference(path)

@@ -78,6 +78,8 @@ SIGNATURES = {
     "pxrhip_set_tuning": [C.c_char_p, _I],
     "pxrhip_get_tuning": [C.c_char_p, C.POINTER(C.c_int64)],
     "pxrhip_tiles_gather_push_deposit": [_P, _P] + [_P] * 6 + [_P] * 3 + [_F] + [_I] * 5,
+    "pxrhip_tiles_gather_push": [_P, _P] + [_P] * 6 + [_F] + [_I] * 4,
+    "pxrhip_tiles_depose_currents": [_P, _P] + [_P] * 3 + [_F] + [_I] * 3,
     "pxrhip_tiles_rebin_sort": [_P, _P, _P],
     "pxrhip_tiles_depose_rho": [_P, _P, _P, _I],
 }

@@ -124,7 +124,11 @@ struct FusedArgs {
   int *ovf_count;        // work items whose blocks do not fit in LDS ...
   int2 *ovf;             // ... (work item, stage): 0 untouched, 1 pushed
   int ovf_cap;
+  int phase;             // PHASE_PUSH | PHASE_DEPOSIT
 };
+
+// phases of k_fused: gather + push + position push, current deposition
+constexpr int PHASE_PUSH = 1, PHASE_DEPOSIT = 2;
 
 struct Box6 {
   int lo[3], hi[3];
@@ -186,7 +190,7 @@ PXR_DEV bool in_block(const int *lo, const int *hi, const int *R0, const int *e)
 // memory (fg).  Extends the box of the current the particle will deposit.
 template <int N, bool LO, int PUSHER, int DEPO, typename VL, typename VG>
 PXR_DEV void gpd_push(const FusedArgs &a, i64 p, const VL *fl, const VG *fg, const int *FR0,
-                      const int *fe, Box6 &dep) {
+                      const int *fe, Box6 &dep, bool box) {
   constexpr int M = LO ? N - 1 : N;
   const GridGeom &g = a.g;
   const double x = a.x[p], y = a.y[p], z = a.z[p];
@@ -217,6 +221,7 @@ PXR_DEV void gpd_push(const FusedArgs &a, i64 p, const VL *fl, const VG *fg, con
   a.x[p] = xn;
   a.y[p] = yn;
   a.z[p] = zn;
+  if (!box) return;
   int b0[3], b1[3];
   if (DEPO == 0) {
     EsirkepovIn<double> in;
@@ -226,6 +231,28 @@ PXR_DEV void gpd_push(const FusedArgs &a, i64 p, const VL *fl, const VG *fg, con
     DirectIn<double> in;
     direct_prepare<N, double>(in, xn, yn, zn, u[0], u[1], u[2], gi, a.w[p], a.q, a.xmin, a.ymin,
                               a.zmin, a.dt, a.dx, a.dy, a.dz, a.s);
+    direct_box<N>(in, b0, b1);
+  }
+  for (int d = 0; d < 3; d++) {
+    dep.lo[d] = min(dep.lo[d], b0[d]);
+    dep.hi[d] = max(dep.hi[d], b1[d]);
+  }
+}
+
+// Extends dep by the box of the current that particle p (in memory as it
+// stands) deposits: the deposition-only phase.
+template <int N, int DEPO>
+PXR_DEV void gpd_box(const FusedArgs &a, i64 p, Box6 &dep) {
+  int b0[3], b1[3];
+  if (DEPO == 0) {
+    EsirkepovIn<double> in;
+    esirkepov_prepare<N, double>(in, a.x[p], a.y[p], a.z[p], a.ux[p], a.uy[p], a.uz[p], a.gi[p],
+                                 a.w[p], a.ec);
+    if (!esirkepov_box<N, double>(in, b0, b1)) return;
+  } else {
+    DirectIn<double> in;
+    direct_prepare<N, double>(in, a.x[p], a.y[p], a.z[p], a.ux[p], a.uy[p], a.uz[p], a.gi[p],
+                              a.w[p], a.q, a.xmin, a.ymin, a.zmin, a.dt, a.dx, a.dy, a.dz, a.s);
     direct_box<N>(in, b0, b1);
   }
   for (int d = 0; d < 3; d++) {
@@ -289,7 +316,7 @@ PXR_DEV bool mx_box(const FusedArgs &a, const double *pn, const double *v, int *
 
 template <int N, bool LO, int PUSHER, int DEPO, typename VL, typename VG>
 PXR_DEV void gpd_push_mx(const FusedArgs &a, i64 p, const VL *fl, const VG *fg, const int *FR0,
-                         const int *fe, Box6 &dep) {
+                         const int *fe, Box6 &dep, bool box) {
   constexpr int M = LO ? N - 1 : N;
   const GridGeom &g = a.g;
   const double x = a.x[p], y = a.y[p], z = a.z[p];
@@ -320,7 +347,24 @@ PXR_DEV void gpd_push_mx(const FusedArgs &a, i64 p, const VL *fl, const VG *fg, 
   a.x[p] = xn;
   a.y[p] = yn;
   a.z[p] = zn;
+  if (!box) return;
   const double pn[3] = {(xn - a.xmin) * a.dxi, (yn - a.ymin) * a.dyi, (zn - a.zmin) * a.dzi};
+  int b0[3], b1[3];
+  if (!mx_box<N, DEPO>(a, pn, v, b0, b1)) return;
+  for (int d = 0; d < 3; d++) {
+    dep.lo[d] = min(dep.lo[d], b0[d]);
+    dep.hi[d] = max(dep.hi[d], b1[d]);
+  }
+}
+
+// mixed-precision counterpart of gpd_box
+template <int N, int DEPO>
+PXR_DEV void gpd_box_mx(const FusedArgs &a, i64 p, Box6 &dep) {
+  const float gi = a.fgi[p];
+  const double v[3] = {(double)a.fux[p] * (double)gi, (double)a.fuy[p] * (double)gi,
+                       (double)a.fuz[p] * (double)gi};
+  const double pn[3] = {(a.x[p] - a.xmin) * a.dxi, (a.y[p] - a.ymin) * a.dyi,
+                        (a.z[p] - a.zmin) * a.dzi};
   int b0[3], b1[3];
   if (!mx_box<N, DEPO>(a, pn, v, b0, b1)) return;
   for (int d = 0; d < 3; d++) {
@@ -381,6 +425,10 @@ PXR_DEV bool gpd_deposit_mx(const FusedArgs &a, i64 p, const Acc &acc) {
 // A work item whose field block does not fit in LDS is left whole to
 // k_fused_overflow; one whose current block does not fit is left to it
 // after the push.
+// a.phase selects the phases: PHASE_PUSH alone is the gather and push of
+// field_gathering_plus_particle_pusher; PHASE_DEPOSIT alone deposits the
+// particles as they stand (pxrdepose_currents_on_grid_jxjyjz after
+// particle_bcs), the box of step 3 then coming from one pass over them.
 // Registers are capped for three waves per SIMD (three workgroups of 256
 // lanes per CU), except for the order-3 direct deposition of an FP64
 // species, which would spill and keeps two.
@@ -411,74 +459,75 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(
     if (c >= nch) break;
     const Chunk ch = a.chunks[c];
     const i64 p0 = ch.begin;
-    // ---- 1. field block -------------------------------------------------
-    double ulo[3], uhi[3];
-    if (ch.z >= 0) {
-      rows_range(a.bg, ch.tile, ch.z, ch.z1, ch.y0, ch.y1, ulo, uhi);
-    } else {
-      // no sorting information: one pass over the positions, in whole cells
-      Box6 b;
-      box_reset(b);
-      for (int k = threadIdx.x; k < ch.n; k += BS) {
-        const i64 p = p0 + k;
-        const double pp[3] = {a.x[p], a.y[p], a.z[p]};
-        for (int d = 0; d < 3; d++) {
-          const int cc = f_floor((pp[d] - x0[d]) * di[d]);
-          b.lo[d] = min(b.lo[d], cc);
-          b.hi[d] = max(b.hi[d], cc);
-        }
-      }
-      wave_minmax(b);
-      if ((threadIdx.x & 63) == 0)
-        for (int d = 0; d < 3; d++) {
-          atomicMin(&s_box[d], b.lo[d]);
-          atomicMax(&s_box[3 + d], b.hi[d]);
-        }
-      __syncthreads();
-      for (int d = 0; d < 3; d++) {
-        const int lo = max(s_box[d], glo[d] - 4), hi = min(s_box[3 + d], ghi[d] + 4);
-        ulo[d] = lo - 1e-7;
-        uhi[d] = hi + 1.0 + 1e-7;
-      }
-      __syncthreads();
-      if (threadIdx.x == 0)
-        for (int d = 0; d < 3; d++) {
-          s_box[d] = INT_MAX;
-          s_box[3 + d] = INT_MIN;
-        }
-    }
-    int FR0[3], fe[3];
-    for (int d = 0; d < 3; d++) {
-      int lo, hi;
-      gather_range(a.bg, ulo[d], uhi[d], lo, hi);
-      FR0[d] = max(lo, glo[d]);
-      fe[d] = max(min(hi, ghi[d]) - FR0[d] + 1, 0);
-    }
-    const i64 nb64 = (i64)fe[0] * fe[1] * fe[2];
-    if (nb64 <= 0 || 6 * nb64 * (i64)sizeof(FT) > a.lds_bytes) {
-      if (threadIdx.x == 0) {
-        const int slot = atomicAdd(a.ovf_count, 1);
-        if (slot < a.ovf_cap) a.ovf[slot] = make_int2(c, 0);
-      }
-      __syncthreads();
-      continue;
-    }
-    const int nb = (int)nb64;
-    FT *sf = reinterpret_cast<FT *>(smem);
-    for (int f = 0; f < 6; f++) {
-      const double *src = a.F[f];
-      for (int idx = threadIdx.x; idx < nb; idx += BS) {
-        const int i = FR0[0] + idx % fe[0];
-        const int j = FR0[1] + (idx / fe[0]) % fe[1];
-        const int k = FR0[2] + idx / (fe[0] * fe[1]);
-        sf[f * nb + idx] = (FT)src[g.idx(i, j, k)];
-      }
-    }
-    __syncthreads();
-    // ---- 2. gather, push, position push ---------------------------------
+    const bool push = a.phase & PHASE_PUSH, depo = a.phase & PHASE_DEPOSIT;
     Box6 dep;
     box_reset(dep);
-    {
+    if (push) {
+      // ---- 1. field block -----------------------------------------------
+      double ulo[3], uhi[3];
+      if (ch.z >= 0) {
+        rows_range(a.bg, ch.tile, ch.z, ch.z1, ch.y0, ch.y1, ulo, uhi);
+      } else {
+        // no sorting information: one pass over the positions, in whole cells
+        Box6 b;
+        box_reset(b);
+        for (int k = threadIdx.x; k < ch.n; k += BS) {
+          const i64 p = p0 + k;
+          const double pp[3] = {a.x[p], a.y[p], a.z[p]};
+          for (int d = 0; d < 3; d++) {
+            const int cc = f_floor((pp[d] - x0[d]) * di[d]);
+            b.lo[d] = min(b.lo[d], cc);
+            b.hi[d] = max(b.hi[d], cc);
+          }
+        }
+        wave_minmax(b);
+        if ((threadIdx.x & 63) == 0)
+          for (int d = 0; d < 3; d++) {
+            atomicMin(&s_box[d], b.lo[d]);
+            atomicMax(&s_box[3 + d], b.hi[d]);
+          }
+        __syncthreads();
+        for (int d = 0; d < 3; d++) {
+          const int lo = max(s_box[d], glo[d] - 4), hi = min(s_box[3 + d], ghi[d] + 4);
+          ulo[d] = lo - 1e-7;
+          uhi[d] = hi + 1.0 + 1e-7;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0)
+          for (int d = 0; d < 3; d++) {
+            s_box[d] = INT_MAX;
+            s_box[3 + d] = INT_MIN;
+          }
+      }
+      int FR0[3], fe[3];
+      for (int d = 0; d < 3; d++) {
+        int lo, hi;
+        gather_range(a.bg, ulo[d], uhi[d], lo, hi);
+        FR0[d] = max(lo, glo[d]);
+        fe[d] = max(min(hi, ghi[d]) - FR0[d] + 1, 0);
+      }
+      const i64 nb64 = (i64)fe[0] * fe[1] * fe[2];
+      if (nb64 <= 0 || 6 * nb64 * (i64)sizeof(FT) > a.lds_bytes) {
+        if (threadIdx.x == 0) {
+          const int slot = atomicAdd(a.ovf_count, 1);
+          if (slot < a.ovf_cap) a.ovf[slot] = make_int2(c, 0);
+        }
+        __syncthreads();
+        continue;
+      }
+      const int nb = (int)nb64;
+      FT *sf = reinterpret_cast<FT *>(smem);
+      for (int f = 0; f < 6; f++) {
+        const double *src = a.F[f];
+        for (int idx = threadIdx.x; idx < nb; idx += BS) {
+          const int i = FR0[0] + idx % fe[0];
+          const int j = FR0[1] + (idx / fe[0]) % fe[1];
+          const int k = FR0[2] + idx / (fe[0] * fe[1]);
+          sf[f * nb + idx] = (FT)src[g.idx(i, j, k)];
+        }
+      }
+      __syncthreads();
+      // ---- 2. gather, push, position push -------------------------------
       LdsView<FT> fl[6];
       for (int f = 0; f < 6; f++)
         fl[f] = LdsView<FT>{sf + f * nb, FR0[0], FR0[1], FR0[2], fe[0], fe[0] * fe[1]};
@@ -486,13 +535,23 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(
         FieldViewF fg[6];
         for (int f = 0; f < 6; f++) fg[f] = FieldViewF{a.F[f] + g.idx(0, 0, 0), g.NX, g.NX * g.NY};
         for (int k = threadIdx.x; k < ch.n; k += BS)
-          gpd_push_mx<N, LO, PUSHER, DEPO>(a, p0 + k, fl, fg, FR0, fe, dep);
+          gpd_push_mx<N, LO, PUSHER, DEPO>(a, p0 + k, fl, fg, FR0, fe, dep, depo);
       } else {
         FieldView<double> fg[6];
         for (int f = 0; f < 6; f++)
           fg[f] = FieldView<double>{a.F[f] + g.idx(0, 0, 0), g.NX, g.NX * g.NY};
         for (int k = threadIdx.x; k < ch.n; k += BS)
-          gpd_push<N, LO, PUSHER, DEPO>(a, p0 + k, fl, fg, FR0, fe, dep);
+          gpd_push<N, LO, PUSHER, DEPO>(a, p0 + k, fl, fg, FR0, fe, dep, depo);
+      }
+      if (!depo) {
+        __syncthreads();  // field block no longer read
+        continue;
+      }
+    } else {
+      // deposition only: the box of the currents of the particles as they are
+      for (int k = threadIdx.x; k < ch.n; k += BS) {
+        if constexpr (MX) gpd_box_mx<N, DEPO>(a, p0 + k, dep);
+        else gpd_box<N, DEPO>(a, p0 + k, dep);
       }
     }
     wave_minmax(dep);
@@ -586,23 +645,24 @@ __global__ __launch_bounds__(BS) void k_fused_overflow(FusedArgs a) {
   for (int o = blockIdx.x; o < n; o += gridDim.x) {
     const int2 ov = a.ovf[o];
     const Chunk ch = a.chunks[ov.x];
-    if (ov.y == 0) {
+    if (ov.y == 0 && (a.phase & PHASE_PUSH)) {
       Box6 dep;
       box_reset(dep);
       if constexpr (MX) {
         FieldViewF fg[6];
         for (int f = 0; f < 6; f++) fg[f] = FieldViewF{a.F[f] + g.idx(0, 0, 0), g.NX, g.NX * g.NY};
         for (int k = threadIdx.x; k < ch.n; k += BS)
-          gpd_push_mx<N, LO, PUSHER, DEPO>(a, ch.begin + k, fg, fg, none, none, dep);
+          gpd_push_mx<N, LO, PUSHER, DEPO>(a, ch.begin + k, fg, fg, none, none, dep, false);
       } else {
         FieldView<double> fg[6];
         for (int f = 0; f < 6; f++)
           fg[f] = FieldView<double>{a.F[f] + g.idx(0, 0, 0), g.NX, g.NX * g.NY};
         for (int k = threadIdx.x; k < ch.n; k += BS)
-          gpd_push<N, LO, PUSHER, DEPO>(a, ch.begin + k, fg, fg, none, none, dep);
+          gpd_push<N, LO, PUSHER, DEPO>(a, ch.begin + k, fg, fg, none, none, dep, false);
       }
     }
     __syncthreads();
+    if (!(a.phase & PHASE_DEPOSIT)) continue;
     const int S = (ch.n + 63) >> 6;
     for (int kk = threadIdx.x; kk < S * 64; kk += BS) {
       const int k = a.spread ? (kk & 63) * S + (kk >> 6) : kk;

@@ -17,6 +17,10 @@ struct Runtime {
   bool checks = true;
   unsigned long long *d_bad = nullptr;  // device counter of skipped particles
   int num_cus = 256;
+  // incremented whenever library-owned device memory that recorded work may
+  // point at is freed (a scratch slot that grows, the cached tile table):
+  // a HIP graph captured before that refers to freed memory
+  unsigned long long mem_gen = 0;
 };
 
 Runtime &rt();

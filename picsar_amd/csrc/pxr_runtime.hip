@@ -42,6 +42,7 @@ void *scratch(int slot, size_t bytes) {
     hipFree(s.p);
     s.p = nullptr;
     s.bytes = 0;
+    g_rt.mem_gen++;
   }
   size_t want = bytes < 4096 ? 4096 : bytes;
   if (hipMalloc(&s.p, want) != hipSuccess) {
@@ -122,7 +123,9 @@ int pxrhip_set_stream(void *s) {
 struct GraphHandle {
   hipGraph_t graph;
   hipGraphExec_t exec;
+  unsigned long long mem_gen;  // Runtime::mem_gen when it was captured
 };
+static unsigned long long g_capture_gen = 0;
 
 int pxrhip_graph_begin(void) {
   PXR_INIT();
@@ -130,6 +133,7 @@ int pxrhip_graph_begin(void) {
   if (g_rt.capturing) return set_error(PXRHIP_ERR_ARG, "a capture is already open");
   PXR_HIP(hipStreamBeginCapture(g_rt.stream, hipStreamCaptureModeRelaxed));
   g_rt.capturing = true;
+  g_capture_gen = g_rt.mem_gen;
   return PXRHIP_OK;
 }
 
@@ -141,20 +145,30 @@ int pxrhip_graph_end(void **handle) {
   g_rt.capturing = false;
   hipGraph_t g = nullptr;
   PXR_HIP(hipStreamEndCapture(g_rt.stream, &g));
+  if (g_rt.mem_gen != g_capture_gen) {
+    hipGraphDestroy(g);
+    return set_error(PXRHIP_ERR_ARG, "library memory was reallocated while the graph was "
+                     "recorded; run the work once before capturing it");
+  }
   hipGraphExec_t e = nullptr;
   hipError_t err = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
   if (err != hipSuccess) {
     hipGraphDestroy(g);
     return set_error(PXRHIP_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(err));
   }
-  *handle = new GraphHandle{g, e};
+  *handle = new GraphHandle{g, e, g_capture_gen};
   return PXRHIP_OK;
 }
 
 int pxrhip_graph_launch(void *handle) {
   PXR_INIT();
   if (!handle) return set_error(PXRHIP_ERR_ARG, "null graph");
-  PXR_HIP(hipGraphLaunch(((GraphHandle *)handle)->exec, g_rt.stream));
+  GraphHandle *h = (GraphHandle *)handle;
+  if (h->mem_gen != g_rt.mem_gen)
+    return set_error(PXRHIP_ERR_ARG, "the graph refers to library memory that has been "
+                     "reallocated since it was captured (a scratch buffer grew or the tile "
+                     "table changed); capture it again");
+  PXR_HIP(hipGraphLaunch(h->exec, g_rt.stream));
   return PXRHIP_OK;
 }
 

@@ -96,7 +96,10 @@ struct TileTable {
 static TileTable g_tt;
 
 static void free_table(TileTable &t) {
-  if (t.sorg || t.n3 || t.bin_base || t.tbox) (void)hipStreamSynchronize(rt().stream);
+  if (t.sorg || t.n3 || t.bin_base || t.tbox) {
+    (void)hipStreamSynchronize(rt().stream);
+    rt().mem_gen++;  // recorded graphs that read the table are stale
+  }
   if (t.sorg) (void)hipFree(t.sorg);
   if (t.tbox) (void)hipFree(t.tbox);
   if (t.n3) (void)hipFree(t.n3);
@@ -120,6 +123,8 @@ static int check_grid_desc(const pxrhip_grid *g) {
     if (g->pbound[k] < PXRHIP_PBOUND_PERIODIC || g->pbound[k] > PXRHIP_PBOUND_REINJECTING)
       return set_error(PXRHIP_ERR_ARG, "pbound[%d] = %lld is not a particle boundary (0..3)", k,
                        (long long)g->pbound[k]);
+  if (g->partcom < 0 || g->partcom > 2)
+    return set_error(PXRHIP_ERR_ARG, "partcom = %lld is not 0, 1 or 2", (long long)g->partcom);
   double tot = (double)(g->nx + 1 + 2 * g->ngx) * (double)(g->ny + 1 + 2 * g->ngy) *
                (double)(g->nz + 1 + 2 * g->ngz);
   if (tot > 4.0e9) return set_error(PXRHIP_ERR_ARG, "grid too large");
@@ -215,8 +220,10 @@ struct RebinGeom {
 };
 
 // Domain boundaries of one particle that left the box
-// (particle_bcs_mpi_blocking, particle_boundaries.F90:1106-1300), axis by
-// axis in x, y, z order as the reference examines them:
+// (particle_bcs_mpi_blocking, particle_boundaries.F90:1106-1300, and
+// particle_bcs_tiles_and_mpi_3d, :2280-2480, which lacks the reflecting
+// case: make_rebin_geom maps it to periodic there), axis by axis in x, y, z
+// order as the reference examines them:
 //   periodic:    x +/- L;
 //   absorbing:   the particle is removed (returns false), no further axis;
 //   reflecting:  x +/- dx (one cell back in) and u_d = -u_d;
@@ -499,6 +506,80 @@ __global__ __launch_bounds__(256) void k_rb_scatter(ScatterArgs a, i64 cap) {
   }
 }
 
+// Stable counting sort, in three passes after the scan.
+// 1. k_rb_slots: every run of equal bins on neighbouring lanes takes its
+//    slots with one atomic and writes its source indices there, in order;
+//    absorbed particles (key -1) take none.  Inside a bin the runs sit in
+//    the order their atomics arrived.
+__global__ __launch_bounds__(256) void k_rb_slots(const int *key, const int64_t *src_off,
+                                                  int ntiles, unsigned long long *cursor,
+                                                  int *perm, i64 cap) {
+  const i64 i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const i64 n = src_off[ntiles];
+  if (i >= n || i >= cap) return;
+  const int k = key[i];
+  if (k < 0) return;
+  int head, cnt;
+  wave_runs(k, head, cnt);
+  unsigned long long slot = 0;
+  if (cnt) slot = atomicAdd(cursor + k, (unsigned long long)cnt);
+  slot = __shfl(slot, head);
+  perm[(i64)slot + (__lane_id() - head)] = (int)i;
+}
+
+// 2. k_rb_order: one thread per bin sorts the source indices of its slots
+//    (insertion sort: the slots hold a few increasing runs, most bins one),
+//    so that a bin lists its particles in source order, as the reference's
+//    stable counting sort does (sorting.F90:333-343), whatever order the
+//    runs arrived in.
+__global__ __launch_bounds__(256) void k_rb_order(const i64 *bin_start, i64 nbins, int *perm) {
+  const i64 b = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbins) return;
+  const i64 s = bin_start[b], e = bin_start[b + 1];
+  if (e - s < 2) return;
+  int prev = perm[s];
+  for (i64 j = s + 1; j < e; j++) {
+    const int v = perm[j];
+    if (v >= prev) {
+      prev = v;
+      continue;
+    }
+    i64 k = j - 1;
+    while (k >= s && perm[k] > v) {
+      perm[k + 1] = perm[k];
+      k--;
+    }
+    perm[k + 1] = v;
+  }
+}
+
+struct GatherArgs {
+  const double *src[3];  // positions
+  double *dst[3];
+  const void *srcq[5];   // ux, uy, uz, gaminv, w: FP64, or FP32 when mixed
+  void *dstq[5];
+  int mixed;
+  const int *perm;       // source index of every destination slot
+  const i64 *total;      // number of particles kept
+};
+
+// 3. k_rb_gather: destination slot j takes source particle perm[j]; the
+//    writes are contiguous, the reads follow the previous order.
+__global__ __launch_bounds__(256) void k_rb_gather(GatherArgs a, i64 cap) {
+  const i64 j = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (j >= cap || j >= *a.total) return;
+  const i64 i = a.perm[j];
+#pragma unroll
+  for (int c = 0; c < 3; c++) a.dst[c][j] = a.src[c][i];
+  if (a.mixed) {
+#pragma unroll
+    for (int c = 0; c < 5; c++) ((float *)a.dstq[c])[j] = ((const float *)a.srcq[c])[i];
+  } else {
+#pragma unroll
+    for (int c = 0; c < 5; c++) ((double *)a.dstq[c])[j] = ((const double *)a.srcq[c])[i];
+  }
+}
+
 __global__ __launch_bounds__(256) void k_rb_tiles(int ntiles, const int64_t *bin_base,
                                                   const int64_t *bin_start, int64_t *off,
                                                   int64_t *np) {
@@ -722,6 +803,7 @@ struct Tuning {
   int jpad = 0;           // odd row/plane pitch of the LDS current block
   int sstride = 0;        // largest lane stride of the deposition (0: whole work item)
   int jacc = 2;           // mixed precision LDS current: 0 FP32, 1 compensated FP32, 2 FP64
+  int stable_rebin = 1;   // re-binning keeps the source order inside a bin (0: arrival order)
   // launch of a mixed-precision species: its kernels take fewer registers,
   // so three workgroups with smaller LDS blocks fit on a CU
   int mx_lds_bytes = LDS_THREE_WG;
@@ -930,6 +1012,8 @@ int pxrhip_set_tuning(const char *key, int64_t value) {
     g_tune.sstride = (int)value;
   } else if (!strcmp(key, "fused.jpad")) {
     g_tune.jpad = value != 0;
+  } else if (!strcmp(key, "rebin.stable")) {
+    g_tune.stable_rebin = value != 0;
   } else if (!strcmp(key, "fused.margin_pct")) {
     if (value < 0 || value > 200) return set_error(PXRHIP_ERR_ARG, "fused.margin_pct must be in [0, 200]");
     g_tune.margin_pct = (int)value;
@@ -951,6 +1035,7 @@ int pxrhip_get_tuning(const char *key, int64_t *value) {
   else if (!strcmp(key, "fused.j_lds_bytes")) *value = g_tune.j_lds_bytes;
   else if (!strcmp(key, "fused.margin_pct")) *value = g_tune.margin_pct;
   else if (!strcmp(key, "fused.jpad")) *value = g_tune.jpad;
+  else if (!strcmp(key, "rebin.stable")) *value = g_tune.stable_rebin;
   else if (!strcmp(key, "fused.sstride")) *value = g_tune.sstride;
   else if (!strcmp(key, "mixed.jacc")) *value = g_tune.jacc;
   else if (!strcmp(key, "mixed.lds_bytes")) *value = g_tune.mx_lds_bytes;
@@ -1011,7 +1096,13 @@ static RebinGeom make_rebin_geom(const pxrhip_grid *grid, const TileTable *tt) {
     rg.nt[k] = tt->nt[k];
     rg.n0[k] = tt->n0[k];
   }
-  for (int k = 0; k < 6; k++) rg.pbound[k] = (int)grid->pbound[k];
+  // particle_bcs_tiles_and_mpi_3d (partcom 0) has no reflecting case: its
+  // SELECT CASE falls through to the periodic wrap (particle_boundaries.F90:
+  // 2308-2333 and the five faces after it)
+  for (int k = 0; k < 6; k++) {
+    const int b = (int)grid->pbound[k];
+    rg.pbound[k] = (grid->partcom == 0 && b == PXRHIP_PBOUND_REFLECTING) ? PXRHIP_PBOUND_PERIODIC : b;
+  }
   rg.sorg = tt->sorg;
   rg.n3 = tt->n3;
   rg.bin_base = tt->bin_base;
@@ -1143,29 +1234,56 @@ int pxrhip_tiles_rebin_sort(const pxrhip_grid *grid, pxrhip_species *src, pxrhip
   sa.ntiles = tt->ntiles;
   sa.key = key;
   sa.cursor = cursor;
-  hipLaunchKernelGGL(k_rb_scatter, dim3(grid1d(cap, 256)), dim3(256), 0, st, sa, cap);
+  if (g_tune.stable_rebin) {
+    int *perm = (int *)scratch(16, sizeof(int) * (size_t)(cap > 1 ? cap : 1));
+    if (!perm) return set_error(PXRHIP_ERR_HIP, "scratch allocation failed");
+    hipLaunchKernelGGL(k_rb_slots, dim3(grid1d(cap, 256)), dim3(256), 0, st, (const int *)key,
+                       (const int64_t *)src->tile_off, tt->ntiles, cursor, perm, cap);
+    hipLaunchKernelGGL(k_rb_order, dim3(grid1d(nbins, 256)), dim3(256), 0, st,
+                       (const i64 *)dst->bin_start, nbins, perm);
+    GatherArgs ga;
+    for (int k = 0; k < 3; k++) {
+      ga.src[k] = S[k];
+      ga.dst[k] = D[k];
+    }
+    for (int k = 0; k < 5; k++) {
+      ga.srcq[k] = SQ[k];
+      ga.dstq[k] = DQ[k];
+    }
+    ga.mixed = sa.mixed;
+    ga.perm = perm;
+    ga.total = (const i64 *)dst->bin_start + nbins;
+    hipLaunchKernelGGL(k_rb_gather, dim3(grid1d(cap, 256)), dim3(256), 0, st, ga, cap);
+  } else {
+    hipLaunchKernelGGL(k_rb_scatter, dim3(grid1d(cap, 256)), dim3(256), 0, st, sa, cap);
+  }
   hipLaunchKernelGGL(k_rb_tiles, dim3(grid1d(tt->ntiles + 1, 256)), dim3(256), 0, st, tt->ntiles,
                      tt->bin_base, dst->bin_start, dst->tile_off, dst->tile_np);
   PXR_LAUNCHED();
   return PXRHIP_OK;
 }
 
-int pxrhip_tiles_gather_push_deposit(const pxrhip_grid *grid, const pxrhip_species *sp,
-                                     const double *exg, const double *eyg, const double *ezg,
-                                     const double *bxg, const double *byg, const double *bzg,
-                                     double *jx, double *jy, double *jz, double dt, int64_t order,
-                                     int64_t l_lower_order_in_v, int64_t pusher,
-                                     int64_t depo_algo, int64_t chunk) {
+}  // extern "C"
+
+// The tiled gather + push + deposit of one species (k_fused), restricted
+// to the phases in `phase` (PHASE_PUSH, PHASE_DEPOSIT or both).
+static int tiles_fused(const pxrhip_grid *grid, const pxrhip_species *sp, const double *exg,
+                       const double *eyg, const double *ezg, const double *bxg, const double *byg,
+                       const double *bzg, double *jx, double *jy, double *jz, double dt,
+                       int64_t order, int64_t l_lower_order_in_v, int64_t pusher,
+                       int64_t depo_algo, int64_t chunk, int phase) {
   PXR_INIT();
   TileTable *tt;
   int r = get_table(grid, &tt);
   if (r) return r;
-  if ((r = check_species(sp, false))) return r;
+  if ((r = check_species(sp, phase == PHASE_DEPOSIT))) return r;
   if (order < 1 || order > 3) return set_error(PXRHIP_ERR_UNSUPPORTED, "shape order must be 1..3");
   if (pusher != 0 && pusher != 1) return set_error(PXRHIP_ERR_ARG, "pusher must be 0 (Boris) or 1 (Vay)");
   if (depo_algo != 0 && depo_algo != 3) return set_error(PXRHIP_ERR_ARG, "depo_algo must be 0 or 3");
-  if (!exg || !eyg || !ezg || !bxg || !byg || !bzg || !jx || !jy || !jz)
+  if ((phase & PHASE_PUSH) && (!exg || !eyg || !ezg || !bxg || !byg || !bzg))
     return set_error(PXRHIP_ERR_ARG, "null field array");
+  if ((phase & PHASE_DEPOSIT) && (!jx || !jy || !jz)) return set_error(PXRHIP_ERR_ARG, "null current array");
+  if (!(dt > 0)) return set_error(PXRHIP_ERR_ARG, "dt must be > 0");
   const int ch = chunk > 0 ? (int)(chunk < (1 << 20) ? chunk : (1 << 20)) : g_tune.chunk;
   hipStream_t st = rt().stream;
   // block geometry of the work items
@@ -1218,7 +1336,10 @@ int pxrhip_tiles_gather_push_deposit(const pxrhip_grid *grid, const pxrhip_speci
   int *ctl;
   int2 *ovf;
   i64 maxchunks;
-  if ((r = build_chunks(tt, sp, bg, ch, lds_bytes, j_bytes, 6 * esize, jnode, st, &chunks, &ctl,
+  // blocks that the work items' phases stage: a phase left out needs no LDS
+  const int fnode = (phase & PHASE_PUSH) ? 6 * esize : 0;
+  const int jnode_b = (phase & PHASE_DEPOSIT) ? jnode : 0;
+  if ((r = build_chunks(tt, sp, bg, ch, lds_bytes, j_bytes, fnode, jnode_b, st, &chunks, &ctl,
                         &ovf, &maxchunks)))
     return r;
   // fused kernel
@@ -1265,6 +1386,7 @@ int pxrhip_tiles_gather_push_deposit(const pxrhip_grid *grid, const pxrhip_speci
   a.ovf_count = ctl + 2;
   a.ovf = ovf;
   a.ovf_cap = maxchunks < INT_MAX ? (int)maxchunks : INT_MAX;
+  a.phase = phase;
   const int nwg = rt().num_cus * wg_per_cu;
   g_last_wg_per_cu = wg_per_cu;
   g_last_lds_bytes = (int)lds_bytes;
@@ -1273,6 +1395,34 @@ int pxrhip_tiles_gather_push_deposit(const pxrhip_grid *grid, const pxrhip_speci
   if (!mx) return launch_fused_fp64(key, a, nwg, lds, st);
   return jacc == 2 ? launch_fused_mixed_f64acc(key, a, nwg, lds, st)
                    : launch_fused_mixed_f32(key, a, nwg, lds, st);
+}
+
+extern "C" {
+
+int pxrhip_tiles_gather_push_deposit(const pxrhip_grid *grid, const pxrhip_species *sp,
+                                     const double *exg, const double *eyg, const double *ezg,
+                                     const double *bxg, const double *byg, const double *bzg,
+                                     double *jx, double *jy, double *jz, double dt, int64_t order,
+                                     int64_t l_lower_order_in_v, int64_t pusher,
+                                     int64_t depo_algo, int64_t chunk) {
+  return tiles_fused(grid, sp, exg, eyg, ezg, bxg, byg, bzg, jx, jy, jz, dt, order,
+                     l_lower_order_in_v, pusher, depo_algo, chunk, PHASE_PUSH | PHASE_DEPOSIT);
+}
+
+int pxrhip_tiles_gather_push(const pxrhip_grid *grid, const pxrhip_species *sp,
+                             const double *exg, const double *eyg, const double *ezg,
+                             const double *bxg, const double *byg, const double *bzg, double dt,
+                             int64_t order, int64_t l_lower_order_in_v, int64_t pusher,
+                             int64_t chunk) {
+  return tiles_fused(grid, sp, exg, eyg, ezg, bxg, byg, bzg, nullptr, nullptr, nullptr, dt,
+                     order, l_lower_order_in_v, pusher, 0, chunk, PHASE_PUSH);
+}
+
+int pxrhip_tiles_depose_currents(const pxrhip_grid *grid, const pxrhip_species *sp, double *jx,
+                                 double *jy, double *jz, double dt, int64_t order,
+                                 int64_t depo_algo, int64_t chunk) {
+  return tiles_fused(grid, sp, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, jx, jy, jz,
+                     dt, order, 1, 0, depo_algo, chunk, PHASE_DEPOSIT);
 }
 
 }  // extern "C"

@@ -19,64 +19,24 @@ tiled species that ``sim`` keeps on the GPU, so a driver written against the
 managers runs unchanged, in the reference's order: the current is deposited
 after the particle boundaries, from the wrapped positions.
 
-The gather, push and deposition managers run the leaf kernels over each
-species' arrays (FP64 species); ``Simulation.step`` is the fast form of the
-same step, with one fused kernel per species.  The re-binning that
-``particle_bcs`` performs also sorts every tile, so ``pxr_particle_sorting``
-has nothing left to do.
+The gather-and-push and the deposition managers run the tiled kernel of the
+step in one phase each (FP64 or mixed-precision species); with periodic
+particle faces ``Simulation.step`` runs both phases in one launch per
+species.  The re-binning that ``particle_bcs`` performs also sorts every
+tile, so ``pxr_particle_sorting`` has nothing left to do.
 """
 from __future__ import annotations
-
-import numpy as np
-
-from . import _lib
-from .devmem import DeviceArray
 
 
 class Managers:
     def __init__(self, sim):
         self.sim = sim
-        self._e = None  # per-particle E and B of the largest species
-
-    def _fields_at_particles(self, n):
-        if self._e is None or self._e[0].n < n:
-            self._e = [DeviceArray(max(n, 1)) for _ in range(6)]
-        for a in self._e:
-            a.fill(0.0)  # the gather adds into them
-        return self._e
-
-    def _species(self):
-        for sp in self.sim.species:
-            if sp.precision != "fp64":
-                raise NotImplementedError("the manager-level step runs FP64 species; use "
-                                          "Simulation.step for mixed precision")
-            yield sp
 
     def field_gathering_plus_particle_pusher(self):
-        """Gather, push and position push of every species
-        (particle_pusher_manager_3d.F90:431-672)."""
-        s = self.sim
-        f = s.fields
-        g = s.ng
-        for sp in self._species():
-            if sp.lfreeze or sp.npart == 0:
-                continue
-            n = sp.npart
-            a = sp.active.arrays
-            E = self._fields_at_particles(n)
-            _lib.call("pxrhip_geteb3d_energy_conserving", n, a["x"].ptr, a["y"].ptr, a["z"].ptr,
-                      *(e.ptr for e in E), s.xmin, s.ymin, s.zmin, s.dx, s.dy, s.dz, s.nx, s.ny,
-                      s.nz, g, g, g, s.order, s.order, s.order, f["ex"].ptr, f["ey"].ptr,
-                      f["ez"].ptr, f["bx"].ptr, f["by"].ptr, f["bz"].ptr, 0,
-                      int(s.l_lower_order_in_v), 8, 0, 0)
-            U = (a["ux"].ptr, a["uy"].ptr, a["uz"].ptr, a["gaminv"].ptr)
-            if s.pusher == 1:
-                _lib.call("pxrhip_ebcancelpush3d", n, *U, *(e.ptr for e in E), sp.q, sp.m, s.dt,
-                          0)
-            else:
-                _lib.call("pxrhip_boris_push_u_3d", n, *U, *(e.ptr for e in E), sp.q, sp.m,
-                          s.dt)
-            _lib.call("pxrhip_pushxyz", n, a["x"].ptr, a["y"].ptr, a["z"].ptr, *U, s.dt)
+        """Gather, push and position push of every species that is not frozen
+        (particle_pusher_manager_3d.F90:431-672), tile by tile with the
+        fields staged in LDS (FP64 or mixed-precision species)."""
+        self.sim.gather_push()
 
     def push_laser_particles(self):
         for ant in self.sim.antennas:
@@ -90,23 +50,11 @@ class Managers:
         """Done by particle_bcs: the re-binning writes every tile sorted."""
 
     def pxrdepose_currents_on_grid_jxjyjz(self):
-        """J = 0, then the current of every species and antenna
-        (current_deposition_manager_3d.F90:224-505)."""
-        s = self.sim
-        f = s.fields
-        g = s.ng
-        for k in ("jx", "jy", "jz"):
-            f[k].fill(0.0)
-        for sp in self._species():
-            if not sp.ldodepos or sp.npart == 0:
-                continue
-            a = sp.active.arrays
-            _lib.call("pxrhip_depose_jxjyjz", f["jx"].ptr, f["jy"].ptr, f["jz"].ptr, sp.npart,
-                      a["x"].ptr, a["y"].ptr, a["z"].ptr, a["ux"].ptr, a["uy"].ptr, a["uz"].ptr,
-                      a["gaminv"].ptr, a["w"].ptr, sp.q, s.xmin, s.ymin, s.zmin, s.dt, s.dx, s.dy,
-                      s.dz, s.nx, s.ny, s.nz, g, g, g, s.order, s.order, s.order, s.depo, 0)
-        for ant in s.antennas:
-            ant.deposit(s)
+        """J = 0, then the current of every species that deposits and of the
+        antennas (current_deposition_manager_3d.F90:224-505), from the
+        particles as particle_bcs left them, tile by tile with the current
+        blocks accumulated in LDS."""
+        self.sim.depose_currents()
 
     def current_bcs(self):
         self.sim.current_bcs()

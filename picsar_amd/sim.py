@@ -18,12 +18,16 @@ The step follows submain.F90:99-199:
     current guards         current_bcs
     B half step, E step, B half step with their guard exchanges
 
-On the GPU the gather, push and deposition of a species are one kernel that
-deposits the pushed particles before they are wrapped; the periodic fold of
-the current guards (current_bcs) makes that identical to depositing after the
-wrap, up to rounding.  Re-binning and sorting are one counting sort that runs
-after the field update, so the particles enter the next step's gather
-wrapped, binned and cell-sorted, as in the reference.
+With periodic particle faces, the gather, push and deposition of a species
+are one kernel that deposits the pushed particles before they are wrapped;
+the periodic fold of the current guards (current_bcs) makes that identical to
+depositing after the wrap, up to rounding.  Re-binning and sorting are one
+counting sort that then runs after the field update, so the particles enter
+the next step's gather wrapped, binned and cell-sorted, as in the reference.
+With an absorbing, reinjecting or (under partcom 1, 2) reflecting face, the
+step keeps the reference's order: the same kernel gathers and pushes, the
+re-binning applies the boundaries and sorts, and a deposition pass of that
+kernel then deposits from the particles as the boundaries left them.
 """
 from __future__ import annotations
 
@@ -46,7 +50,7 @@ class PxrGrid(C.Structure):
                                           "sorting_dx", "sorting_dy", "sorting_dz",
                                           "sorting_shiftx", "sorting_shifty",
                                           "sorting_shiftz")] + \
-               [("pbound", C.c_int64 * 6)]
+               [("pbound", C.c_int64 * 6), ("partcom", C.c_int64)]
 
 
 class PxrSpecies(C.Structure):
@@ -57,7 +61,7 @@ class PxrSpecies(C.Structure):
 
 
 PUSHERS = {"boris": 0, "vay": 1}
-# particle_bcs_mpi_blocking's pbound values (particle_boundaries.F90:1106-1300)
+# pbound values of the reference's particle boundaries (particle_boundaries.F90)
 PBOUND_CODES = {"periodic": 0, "absorbing": 1, "reflecting": 2, "reinjecting": 3}
 PBOUNDS_DEFAULT = ("periodic",) * 6
 DEPOS = {"esirkepov": 0, "direct": 3}
@@ -135,7 +139,8 @@ class Simulation:
     def __init__(self, nx, ny, nz, dx=1.0, dy=1.0, dz=1.0, xmin=0.0, ymin=0.0, zmin=0.0,
                  ntx=1, nty=1, ntz=1, order=1, l_lower_order_in_v=True, pusher="boris",
                  depo="esirkepov", norder=2, dt=None, cfl=0.95, nguard=None, sorting_dx=1.0,
-                 sorting_dy=1.0, sorting_dz=1.0, chunk=0, precision="fp64", pbound=None):
+                 sorting_dy=1.0, sorting_dz=1.0, chunk=0, precision="fp64", pbound=None,
+                 partcom=0):
         _lib.call("pxrhip_init", -1)
         if precision not in PRECISIONS:
             raise ValueError("precision must be 'fp64' or 'mixed'")
@@ -180,7 +185,22 @@ class Simulation:
             if c not in PBOUND_CODES.values():
                 raise ValueError(f"unknown particle boundary {c}")
         self.grid.pbound[:] = codes
+        # particle_bcs's exchange (particle_boundaries.F90:83-217): 0, the
+        # reference's default, has no reflecting case (a reflecting face is
+        # periodic there); 1 and 2 reflect (particle_bcs_mpi_blocking)
+        if int(partcom) not in (0, 1, 2):
+            raise ValueError("partcom must be 0, 1 or 2")
+        self.grid.partcom = int(partcom)
+        self.partcom = int(partcom)
         self.absorbing = PBOUND_CODES["absorbing"] in codes
+        eff = [0 if (self.partcom == 0 and c == PBOUND_CODES["reflecting"]) else c for c in codes]
+        # With periodic faces only, depositing the pushed particles before
+        # their wrap gives the current of the reference's order once
+        # current_bcs has folded the guards, so the step fuses gather, push
+        # and deposition.  Any other face removes, moves or stops particles
+        # before the reference deposits them: the step then runs in
+        # submain.F90's order (push, particle_bcs, deposition).
+        self.split_step = any(c != 0 for c in eff)
         self.ntiles = int(ntx) * int(nty) * int(ntz)
         nb = C.c_int64(0)
         _lib.call("pxrhip_tiles_nbins", C.byref(self.grid), C.byref(nb))
@@ -273,8 +293,18 @@ class Simulation:
     def laser_push_deposit(self):
         """push_laser_particles (laser_pusher_manager_3d.F90:15) and the
         antennas' current deposition."""
+        self.push_antennas()
+        self.deposit_antennas()
+
+    def push_antennas(self):
+        """push_laser_particles (laser_pusher_manager_3d.F90:15)."""
         for a in self.antennas:
             a.push(self)
+
+    def deposit_antennas(self):
+        """The antennas' current, added into J (the reference deposits them as
+        species, tiling.F90:1697)."""
+        for a in self.antennas:
             a.deposit(self)
 
     def _rebin(self, sp: Species):
@@ -295,30 +325,75 @@ class Simulation:
     # ------------------------------------------------------------------
     def gather_push_deposit(self):
         """field_gathering_plus_particle_pusher + pxrdepose_currents_on_grid_jxjyjz
-        for every species (J is zeroed first, current_deposition_manager_3d.F90:247)."""
+        for every species (J is zeroed first, current_deposition_manager_3d.F90:247)
+        in one kernel per species, then the antennas."""
         for k in ("jx", "jy", "jz"):
             self.fields[k].fill(0.0)
         for sp in self.species:
             self.gather_push_deposit_species(sp)
         self.laser_push_deposit()
 
+    def _tiled_args(self, sp: Species):
+        return C.byref(self.grid), C.byref(sp.struct(sp.cur))
+
     def gather_push_deposit_species(self, sp: Species):
-        """One launch of the fused kernel for one species (J accumulates)."""
-        if sp.lfreeze or sp.npart == 0:
+        """One launch of the fused kernel for one species (J accumulates):
+        gather and push unless the species is frozen (lfreeze skips the push,
+        particle_pusher_manager_3d.F90:523), deposition unless it does not
+        deposit (ldodepos, current_deposition_manager_3d.F90:595)."""
+        if sp.npart == 0:
             return
-        if not sp.ldodepos:
-            raise NotImplementedError("species that do not deposit are not built")
+        if sp.lfreeze and not sp.ldodepos:
+            return
         f = self.fields
-        s = sp.struct(sp.cur)
-        _lib.call("pxrhip_tiles_gather_push_deposit", C.byref(self.grid), C.byref(s),
-                  f["ex"].ptr, f["ey"].ptr, f["ez"].ptr, f["bx"].ptr, f["by"].ptr, f["bz"].ptr,
-                  f["jx"].ptr, f["jy"].ptr, f["jz"].ptr, self.dt, self.order, int(self.l_lower_order_in_v),
-                  self.pusher, self.depo, self.chunk)
+        if sp.lfreeze:
+            self.depose_currents_species(sp)
+        elif not sp.ldodepos:
+            self.gather_push_species(sp)
+        else:
+            _lib.call("pxrhip_tiles_gather_push_deposit", *self._tiled_args(sp),
+                      f["ex"].ptr, f["ey"].ptr, f["ez"].ptr, f["bx"].ptr, f["by"].ptr,
+                      f["bz"].ptr, f["jx"].ptr, f["jy"].ptr, f["jz"].ptr, self.dt, self.order,
+                      int(self.l_lower_order_in_v), self.pusher, self.depo, self.chunk)
+
+    def gather_push_species(self, sp: Species):
+        """Gather, push and position push of one binned species (no deposition)."""
+        f = self.fields
+        _lib.call("pxrhip_tiles_gather_push", *self._tiled_args(sp), f["ex"].ptr, f["ey"].ptr,
+                  f["ez"].ptr, f["bx"].ptr, f["by"].ptr, f["bz"].ptr, self.dt, self.order,
+                  int(self.l_lower_order_in_v), self.pusher, self.chunk)
+
+    def depose_currents_species(self, sp: Species):
+        """The current of one binned species from its particles as they stand,
+        added into J."""
+        f = self.fields
+        _lib.call("pxrhip_tiles_depose_currents", *self._tiled_args(sp), f["jx"].ptr,
+                  f["jy"].ptr, f["jz"].ptr, self.dt, self.order, self.depo, self.chunk)
+
+    def gather_push(self):
+        """field_gathering_plus_particle_pusher (particle_pusher_manager_3d.F90:
+        431-672): every species that is not frozen."""
+        for sp in self.species:
+            if not sp.lfreeze and sp.npart:
+                self.gather_push_species(sp)
+
+    def depose_currents(self):
+        """pxrdepose_currents_on_grid_jxjyjz (current_deposition_manager_3d.F90:
+        224-505): J = 0, then the current of every species that deposits and
+        of the antennas."""
+        for k in ("jx", "jy", "jz"):
+            self.fields[k].fill(0.0)
+        for sp in self.species:
+            if sp.ldodepos and sp.npart:
+                self.depose_currents_species(sp)
+        self.deposit_antennas()
 
     def particle_bcs_and_sorting(self):
-        """particle_bcs (periodic wrap + tile exchange) and particle_sorting_sub."""
+        """particle_bcs (domain boundaries + tile exchange) and
+        particle_sorting_sub; a frozen species does not move and keeps its
+        binning."""
         for sp in self.species:
-            if sp.npart:
+            if sp.npart and not sp.lfreeze:
                 self._rebin(sp)
 
     def current_bcs(self):
@@ -382,12 +457,25 @@ class Simulation:
         self.push_bfield()
         self.bfield_bcs()
 
-    def step(self, nst=1, check=True):
-        for _ in range(int(nst)):
+    def _one_step(self):
+        if self.split_step:
+            # submain.F90:99-140: push, particle boundaries and sort, then the
+            # deposition from the particles as the boundaries left them
+            self.gather_push()
+            self.push_antennas()
+            self.particle_bcs_and_sorting()
+            self.depose_currents()
+            self.current_bcs()
+            self.field_step()
+        else:
             self.gather_push_deposit()
             self.current_bcs()
             self.field_step()
             self.particle_bcs_and_sorting()
+
+    def step(self, nst=1, check=True):
+        for _ in range(int(nst)):
+            self._one_step()
             self.it += 1
         if check:
             self.check()
@@ -416,10 +504,7 @@ class Simulation:
         _lib.call("pxrhip_graph_begin")
         try:
             for _ in range(nst):
-                self.gather_push_deposit()
-                self.current_bcs()
-                self.field_step()
-                self.particle_bcs_and_sorting()
+                self._one_step()
         except Exception:
             h = C.c_void_p(0)
             try:

@@ -34,6 +34,7 @@ class OracleState:
         # 0 periodic, 1 absorbing, 2 reflecting, 3 reinjecting; nt: tiles per
         # axis (reinjection keeps a particle inside the tile it was in)
         self.pbound = (0,) * 6
+        self.partcom = 0
         self.nt = (1, 1, 1)
 
     def add_antenna(self, a):
@@ -43,10 +44,13 @@ class OracleState:
         self.antennas.append(dict(a, ux=np.zeros(n), uy=np.zeros(n), uz=np.zeros(n),
                                   gaminv=np.ones(n)))
 
-    def add_species(self, q, m, x, y, z, ux, uy, uz, w):
+    def add_species(self, q, m, x, y, z, ux, uy, uz, w, lfreeze=False, ldodepos=True):
+        """lfreeze: not pushed (particle_pusher_manager_3d.F90:523); ldodepos:
+        deposits current (current_deposition_manager_3d.F90:595)."""
         g = 1.0 / np.sqrt(1.0 + ux ** 2 + uy ** 2 + uz ** 2)
         self.species.append(dict(q=q, m=m, x=x.copy(), y=y.copy(), z=z.copy(), ux=ux.copy(),
-                                 uy=uy.copy(), uz=uz.copy(), gaminv=g, w=w.copy()))
+                                 uy=uy.copy(), uz=uz.copy(), gaminv=g, w=w.copy(),
+                                 lfreeze=lfreeze, ldodepos=ldodepos))
 
     def geom(self):
         return (self.xmin, self.ymin, self.zmin)
@@ -85,6 +89,8 @@ class OracleState:
         for s in self.species:
             n = len(s["x"])
             s["old"] = (s["x"].copy(), s["y"].copy(), s["z"].copy())  # for particle_bcs
+            if s.get("lfreeze"):
+                continue
             e = [np.zeros(n) for _ in range(6)]
             bad = o.geteb3d_energy_conserving(s["x"], s["y"], s["z"], *e, *self.geom(), *d, nx,
                                               ny, nz, ng, ng, ng, self.order, self.order,
@@ -113,7 +119,8 @@ class OracleState:
             for k in ("jx", "jy", "jz"):
                 self.f[k][...] = 0.0
         for s in self.species:
-            self._depose(s, s["w"], s["q"])
+            if s.get("ldodepos", True):
+                self._depose(s, s["w"], s["q"])
 
     def push_antennas(self):
         """push_laser_particles at it*dt - dt/2 (laser_pusher_manager_3d.F90:37)."""
@@ -169,12 +176,15 @@ class OracleState:
         return it
 
     def particle_bcs(self):
-        """Domain boundaries of the particles that left the box
-        (particle_bcs_mpi_blocking, particle_boundaries.F90:1106-1300), axis by
-        axis: periodic x +/- L; absorbing: removed; reflecting: x +/- dx and the
-        normal momentum reversed; reinjecting: x = 2 x_edge - x, u = 0, all
-        coordinates clamped to [tmin + d/2, tmax - d/2] of the particle's tile,
-        no further axis."""
+        """Domain boundaries of the particles that left the box, axis by axis:
+        periodic x +/- L; absorbing: removed; reinjecting: x = 2 x_edge - x,
+        u = 0, all coordinates clamped to [tmin + d/2, tmax - d/2] of the tile
+        the particle was in, no further axis; reflecting: with partcom 1 or 2
+        (particle_bcs_mpi_blocking, particle_boundaries.F90:1106-1300) x +/- dx
+        and the normal momentum reversed, with partcom 0, the reference's
+        default (particle_bcs_tiles_and_mpi_3d, :2280-2480, whose SELECT CASE
+        has no reflecting branch), the periodic wrap.  Each side is tested on
+        the coordinate as it was before the boundaries ran."""
         L = (self.nx * self.dx, self.ny * self.dy, self.nz * self.dz)
         D = (self.dx, self.dy, self.dz)
         x0 = self.geom()
@@ -194,6 +204,8 @@ class OracleState:
                         if not ((v < x0[a]) if side == 0 else (v >= x0[a] + L[a])):
                             continue
                         b = self.pbound[2 * a + side]
+                        if b == 2 and self.partcom == 0:
+                            b = 0
                         if b == 1:
                             keep[i] = False
                             done = True

@@ -28,7 +28,7 @@ def rel(a, b):
 
 def build(torch, orc, order, lo, pusher, depo, norder=2, ntiles=(2, 2, 2), n=(16, 14, 12),
           ppc=4, uth=(0.2, 0.02), chunk=0, field_amp=0.05, seed=1, d=(1.0, 0.9, 1.1),
-          precision="fp64", id_scale=1e-9, pbound=None):
+          precision="fp64", id_scale=1e-9, pbound=None, partcom=0):
     from picsar_amd.sim import Simulation, _fd_weights
     nx, ny, nz = n
     dx, dy, dz = d
@@ -36,10 +36,11 @@ def build(torch, orc, order, lo, pusher, depo, norder=2, ntiles=(2, 2, 2), n=(16
     dt = default_dt(dx, dy, dz)
     sim = Simulation(nx, ny, nz, dx, dy, dz, xmin, ymin, zmin, *ntiles, order=order,
                      l_lower_order_in_v=lo, pusher=pusher, depo=depo, norder=norder, dt=dt,
-                     chunk=chunk, precision=precision, pbound=pbound)
+                     chunk=chunk, precision=precision, pbound=pbound, partcom=partcom)
     ref = OracleState(orc, nx, ny, nz, sim.ng, dx, dy, dz, xmin, ymin, zmin, dt, order, lo,
                       pusher, depo, norder, _fd_weights(norder) if norder > 2 else None)
     ref.pbound = tuple(sim.grid.pbound)
+    ref.partcom = partcom
     ref.nt = tuple(ntiles)
     rng = np.random.default_rng(seed)
     shape = ref.f["ex"].shape
@@ -228,31 +229,41 @@ def test_laser_antennas_in_the_step(torch, orc, depo):
     compare_fields(sim, ref, tng, 1e-9)
 
 
+@pytest.mark.parametrize("partcom", [0, 1])
 @pytest.mark.parametrize("pbound", [
     ("absorbing",) * 6,
     ("reflecting", "reflecting", "absorbing", "absorbing", "periodic", "periodic"),
     ("periodic", "periodic", "periodic", "periodic", "reinjecting", "reinjecting"),
     ("reinjecting", "absorbing", "reflecting", "periodic", "periodic", "reflecting"),
+    ("reflecting",) * 6,
 ])
-def test_particle_boundaries(torch, orc, pbound):
-    """Absorbing, reflecting and reinjecting domain boundaries
-    (particle_bcs_mpi_blocking, particle_boundaries.F90:1106-1300) with fast
-    particles (u_th = 0.6 c): the particles kept, their positions and momenta
-    and the fields match the oracle after one step (1e-12) and three (1e-9)."""
+def test_particle_boundaries(torch, orc, pbound, partcom):
+    """Absorbing, reflecting and reinjecting domain boundaries with fast
+    particles (u_th = 0.6 c), under the reference's default exchange
+    (partcom 0, particle_bcs_tiles_and_mpi_3d, where a reflecting face is
+    periodic) and under partcom 1 (particle_bcs_mpi_blocking, which
+    reflects).  The step runs in the reference's order (push, boundaries,
+    deposition from the particles as the boundaries left them) and matches
+    the oracle in that order: the particles kept, their positions and
+    momenta and the fields after one step (1e-12) and three (1e-9)."""
     sim, ref, tng = build(torch, orc, 3, True, "vay", "esirkepov", uth=(0.6, 0.05),
-                          pbound=pbound)
+                          pbound=pbound, partcom=partcom)
+    eff = [("periodic" if (b == "reflecting" and partcom == 0) else b) for b in pbound]
+    assert sim.split_step == any(b != "periodic" for b in eff)
     n0 = sim.total_particles()
     sim.step(1)
-    ref.step()
+    ref.step_reference_order()
     check_tiles(sim)
     compare_fields(sim, ref, tng, 1e-12)
     L = max(sim.nx * sim.dx, sim.ny * sim.dy, sim.nz * sim.dz)
     compare_particles(sim, ref, 1e-12 * L, 1e-12)
     if "absorbing" in pbound:
         assert sim.total_particles() < n0
+    else:
+        assert sim.total_particles() == n0
     for _ in range(2):
         sim.step(1)
-        ref.step()
+        ref.step_reference_order()
     check_tiles(sim)
     compare_fields(sim, ref, tng, 1e-9)
     assert sim.total_particles() == sum(len(s["x"]) for s in ref.species)
@@ -268,7 +279,7 @@ def test_managers_in_reference_order(torch, orc, order, pusher, depo):
     in the same order."""
     from picsar_amd.managers import Managers
     pb = ("reflecting", "reflecting", "periodic", "periodic", "periodic", "periodic")
-    sim, ref, tng = build(torch, orc, order, True, pusher, depo, pbound=pb)
+    sim, ref, tng = build(torch, orc, order, True, pusher, depo, pbound=pb, partcom=1)
     a = sim.add_laser_antenna(spot=(sim.xmin + 8.0, sim.ymin + 6.1, sim.zmin + 5.3),
                               vector=(0, 0, 1), polvector1=(1, 0, 0), lambda_laser=8.0,
                               laser_w0=4.0, laser_ctau=6.0, t_peak=1.5, emax1=0.05)
@@ -286,12 +297,26 @@ def test_managers_in_reference_order(torch, orc, order, pusher, depo):
     compare_fields(sim, ref, tng, 1e-9)
 
 
-def test_managers_refuse_mixed_precision(torch, orc):
+def test_managers_run_mixed_precision(torch, orc):
+    """The managers drive a mixed-precision species through the tiled
+    kernels, in the reference's order, with reflecting faces (partcom 1):
+    fields and momenta after one step within FP32 round-off of the FP64
+    oracle in that order."""
     from picsar_amd.managers import Managers
+    pb = ("reflecting", "reflecting", "periodic", "periodic", "absorbing", "absorbing")
     sim, ref, tng = build(torch, orc, 3, True, "vay", "esirkepov", precision="mixed",
-                          id_scale=2.0 ** -15, ppc=2)
-    with pytest.raises(NotImplementedError):
-        Managers(sim).field_gathering_plus_particle_pusher()
+                          id_scale=2.0 ** -15, ppc=2, pbound=pb, partcom=1, uth=(0.4, 0.05))
+    for s in ref.species:  # the oracle starts from the values the GPU holds
+        for k in ("ux", "uy", "uz", "w"):
+            s[k] = s[k].astype(np.float32).astype(np.float64)
+        s["gaminv"] = 1.0 / np.sqrt(1.0 + s["ux"] ** 2 + s["uy"] ** 2 + s["uz"] ** 2)
+    Managers(sim).step(1)
+    ref.step_reference_order()
+    check_tiles(sim)
+    for k in ("jx", "jy", "jz"):
+        assert rel(tng(sim.fields[k]), ref.f[k]) <= 1e-5, k
+    assert sim.total_particles() == sum(len(s["x"]) for s in ref.species)
+    compare_particles(sim, ref, 1e-5, 1e-5)
 
 
 def test_absorbing_boundaries_refuse_graph_capture(torch, orc):
@@ -510,6 +535,31 @@ def test_graph_replays_match_eager_steps(torch, orc):
     compare_fields(sim_g, ref, tng, 1e-11)
     compare_particles(sim_g, ref, 1e-11, 1e-11)
     del graph
+
+
+def test_graph_refuses_replay_after_library_memory_moved(torch, orc):
+    """A graph records the library's scratch buffers and tile table.  Once
+    another simulation with a different grid (and a larger species) has run,
+    those are reallocated, and launching the old graph raises instead of
+    replaying on freed memory; a graph captured afresh runs."""
+    from picsar_amd._lib import PxrHipError
+    sim, ref, tng = build(torch, orc, 2, True, "boris", "esirkepov", ppc=2)
+    sim.step(1)
+    graph = sim.capture_steps(2)
+    sim.step_graph(graph)
+    other, _, _ = build(torch, orc, 2, True, "boris", "esirkepov", ppc=6, n=(18, 16, 14),
+                        ntiles=(2, 2, 3))
+    other.step(1)
+    it = sim.it
+    with pytest.raises(PxrHipError):
+        graph.launch()
+    assert sim.it == it
+    del graph
+    sim.step(1)  # scratch and tile table for this grid again
+    g2 = sim.capture_steps(2)
+    sim.step_graph(g2)
+    check_tiles(sim)
+    del g2
 
 
 def test_autotune_keeps_the_physics(torch, orc):
