@@ -125,6 +125,7 @@ struct FusedArgs {
   int2 *ovf;             // ... (work item, stage): 0 untouched, 1 pushed
   int ovf_cap;
   int phase;             // PHASE_PUSH | PHASE_DEPOSIT
+  double fxb;            // bound of |current contribution| per unit |q w| (fixed point)
 };
 
 // phases of k_fused: gather + push + position push, current deposition
@@ -167,6 +168,24 @@ struct LdsAcc {
   PXR_DEV void add_row(double *r, int o, double v) const { atomicAdd(r + o, v); }
 };
 
+// The same in fixed point: v is added as fixed_point(v, scale) to 64-bit
+// integer counters, so a block's sums are exact and do not depend on the
+// order of the atomics.
+struct LdsAccFx {
+  unsigned long long *p[3];
+  int i0, j0, k0, sy, sz;
+  double scale;
+  PXR_DEV void add(int c, int i, int j, int k, double v) const {
+    atomicAdd(p[c] + ((i - i0) + sy * (j - j0) + sz * (k - k0)), fixed_point(v, scale));
+  }
+  PXR_DEV unsigned long long *row(int c, int i, int j, int k) const {
+    return p[c] + ((i - i0) + sy * (j - j0) + sz * (k - k0));
+  }
+  PXR_DEV void add_row(unsigned long long *r, int o, double v) const {
+    atomicAdd(r + o, fixed_point(v, scale));
+  }
+};
+
 struct GAcc {
   double *p[3];
   GridGeom g;
@@ -190,7 +209,7 @@ PXR_DEV bool in_block(const int *lo, const int *hi, const int *R0, const int *e)
 // memory (fg).  Extends the box of the current the particle will deposit.
 template <int N, bool LO, int PUSHER, int DEPO, typename VL, typename VG>
 PXR_DEV void gpd_push(const FusedArgs &a, i64 p, const VL *fl, const VG *fg, const int *FR0,
-                      const int *fe, Box6 &dep, bool box) {
+                      const int *fe, Box6 &dep, bool box, double &wabs) {
   constexpr int M = LO ? N - 1 : N;
   const GridGeom &g = a.g;
   const double x = a.x[p], y = a.y[p], z = a.z[p];
@@ -222,6 +241,7 @@ PXR_DEV void gpd_push(const FusedArgs &a, i64 p, const VL *fl, const VG *fg, con
   a.y[p] = yn;
   a.z[p] = zn;
   if (!box) return;
+  wabs = fmax(wabs, fabs(a.q * a.w[p]));
   int b0[3], b1[3];
   if (DEPO == 0) {
     EsirkepovIn<double> in;
@@ -242,7 +262,8 @@ PXR_DEV void gpd_push(const FusedArgs &a, i64 p, const VL *fl, const VG *fg, con
 // Extends dep by the box of the current that particle p (in memory as it
 // stands) deposits: the deposition-only phase.
 template <int N, int DEPO>
-PXR_DEV void gpd_box(const FusedArgs &a, i64 p, Box6 &dep) {
+PXR_DEV void gpd_box(const FusedArgs &a, i64 p, Box6 &dep, double &wabs) {
+  wabs = fmax(wabs, fabs(a.q * a.w[p]));
   int b0[3], b1[3];
   if (DEPO == 0) {
     EsirkepovIn<double> in;
@@ -316,7 +337,7 @@ PXR_DEV bool mx_box(const FusedArgs &a, const double *pn, const double *v, int *
 
 template <int N, bool LO, int PUSHER, int DEPO, typename VL, typename VG>
 PXR_DEV void gpd_push_mx(const FusedArgs &a, i64 p, const VL *fl, const VG *fg, const int *FR0,
-                         const int *fe, Box6 &dep, bool box) {
+                         const int *fe, Box6 &dep, bool box, double &wabs) {
   constexpr int M = LO ? N - 1 : N;
   const GridGeom &g = a.g;
   const double x = a.x[p], y = a.y[p], z = a.z[p];
@@ -348,6 +369,7 @@ PXR_DEV void gpd_push_mx(const FusedArgs &a, i64 p, const VL *fl, const VG *fg, 
   a.y[p] = yn;
   a.z[p] = zn;
   if (!box) return;
+  wabs = fmax(wabs, fabs(a.q * (double)a.fw[p]));
   const double pn[3] = {(xn - a.xmin) * a.dxi, (yn - a.ymin) * a.dyi, (zn - a.zmin) * a.dzi};
   int b0[3], b1[3];
   if (!mx_box<N, DEPO>(a, pn, v, b0, b1)) return;
@@ -359,7 +381,8 @@ PXR_DEV void gpd_push_mx(const FusedArgs &a, i64 p, const VL *fl, const VG *fg, 
 
 // mixed-precision counterpart of gpd_box
 template <int N, int DEPO>
-PXR_DEV void gpd_box_mx(const FusedArgs &a, i64 p, Box6 &dep) {
+PXR_DEV void gpd_box_mx(const FusedArgs &a, i64 p, Box6 &dep, double &wabs) {
+  wabs = fmax(wabs, fabs(a.q * (double)a.fw[p]));
   const float gi = a.fgi[p];
   const double v[3] = {(double)a.fux[p] * (double)gi, (double)a.fuy[p] * (double)gi,
                        (double)a.fuz[p] * (double)gi};
@@ -436,11 +459,15 @@ template <int BS, int N, bool LO, int PUSHER, int DEPO, bool MX, int JACC>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(
     (N == 3 && DEPO == 3 && !MX) ? 2 : 3))) void k_fused(FusedArgs a) {
   typedef typename std::conditional<MX, float, double>::type FT;  // field block
-  // current block: FP64, or per the mixed-precision accumulation mode
-  typedef typename std::conditional<MX && JACC != 2, float, double>::type JT;
+  // current block: FP64, 64-bit fixed point (JACC 3), or FP32 for the
+  // mixed-precision modes 0 and 1
+  typedef typename std::conditional<
+      JACC == 3, unsigned long long,
+      typename std::conditional<MX && JACC != 2, float, double>::type>::type JT;
   extern __shared__ double smem[];
   __shared__ int s_chunk;
   __shared__ int s_box[6];
+  __shared__ unsigned long long s_wabs;  // largest |q w| of the work item (JACC 3)
   const GridGeom &g = a.g;
   const int glo[3] = {-g.gx, -g.gy, -g.gz};
   const int ghi[3] = {g.nx + g.gx, g.ny + g.gy, g.nz + g.gz};
@@ -453,6 +480,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(
         s_box[d] = INT_MAX;
         s_box[3 + d] = INT_MIN;
       }
+      s_wabs = 0ull;
     }
     __syncthreads();
     const int c = s_chunk;
@@ -462,6 +490,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(
     const bool push = a.phase & PHASE_PUSH, depo = a.phase & PHASE_DEPOSIT;
     Box6 dep;
     box_reset(dep);
+    double wabs = 0.0;
     if (push) {
       // ---- 1. field block -----------------------------------------------
       double ulo[3], uhi[3];
@@ -535,13 +564,13 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(
         FieldViewF fg[6];
         for (int f = 0; f < 6; f++) fg[f] = FieldViewF{a.F[f] + g.idx(0, 0, 0), g.NX, g.NX * g.NY};
         for (int k = threadIdx.x; k < ch.n; k += BS)
-          gpd_push_mx<N, LO, PUSHER, DEPO>(a, p0 + k, fl, fg, FR0, fe, dep, depo);
+          gpd_push_mx<N, LO, PUSHER, DEPO>(a, p0 + k, fl, fg, FR0, fe, dep, depo, wabs);
       } else {
         FieldView<double> fg[6];
         for (int f = 0; f < 6; f++)
           fg[f] = FieldView<double>{a.F[f] + g.idx(0, 0, 0), g.NX, g.NX * g.NY};
         for (int k = threadIdx.x; k < ch.n; k += BS)
-          gpd_push<N, LO, PUSHER, DEPO>(a, p0 + k, fl, fg, FR0, fe, dep, depo);
+          gpd_push<N, LO, PUSHER, DEPO>(a, p0 + k, fl, fg, FR0, fe, dep, depo, wabs);
       }
       if (!depo) {
         __syncthreads();  // field block no longer read
@@ -550,18 +579,40 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(
     } else {
       // deposition only: the box of the currents of the particles as they are
       for (int k = threadIdx.x; k < ch.n; k += BS) {
-        if constexpr (MX) gpd_box_mx<N, DEPO>(a, p0 + k, dep);
-        else gpd_box<N, DEPO>(a, p0 + k, dep);
+        if constexpr (MX) gpd_box_mx<N, DEPO>(a, p0 + k, dep, wabs);
+        else gpd_box<N, DEPO>(a, p0 + k, dep, wabs);
       }
     }
     wave_minmax(dep);
+    if constexpr (JACC == 3) {
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) wabs = fmax(wabs, __shfl_xor(wabs, o));
+    }
     __syncthreads();  // field block no longer read
-    if ((threadIdx.x & 63) == 0)
+    if ((threadIdx.x & 63) == 0) {
       for (int d = 0; d < 3; d++) {
         atomicMin(&s_box[d], dep.lo[d]);
         atomicMax(&s_box[3 + d], dep.hi[d]);
       }
+      // non-negative doubles order as their bit patterns do
+      if (JACC == 3) atomicMax(&s_wabs, (unsigned long long)__double_as_longlong(wabs));
+    }
     __syncthreads();
+    // fixed point (JACC 3): a power-of-two scale under which every
+    // contribution of the work item is below 2^50 in magnitude and a node's
+    // sum of at most ch.n of them stays below 2^62
+    double fxs = 1.0, fxs_inv = 1.0;
+    if constexpr (JACC == 3) {
+      const double bound = a.fxb * __longlong_as_double((long long)s_wabs);
+      if (bound > 0.0) {
+        int eb;
+        frexp(bound, &eb);  // bound < 2^eb
+        const int en = ch.n > 1 ? 32 - __clz(ch.n - 1) : 0;  // ch.n <= 2^en
+        const int k = min(50 - eb, 62 - eb - en);
+        fxs = ldexp(1.0, k);
+        fxs_inv = ldexp(1.0, -k);
+      }
+    }
     int R0[3], e[3];
     for (int d = 0; d < 3; d++) {
       R0[d] = max(s_box[d], glo[d]);
@@ -602,10 +653,16 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(
       if constexpr (MX) {
         float *se = reinterpret_cast<float *>(smem) + 3 * nj;  // compensation (JACC 1)
         const LdsAccMx<JACC> la{{sj, sj + nj, sj + 2 * nj}, {se, se + nj, se + 2 * nj},
-                                R0[0], R0[1], R0[2], py, pz};
+                                R0[0], R0[1], R0[2], py, pz, fxs};
         for (int kk = threadIdx.x; kk < nkk; kk += BS) {
           const int k = pick(kk);
           if (k < ch.n && !gpd_deposit_mx<N, DEPO>(a, p0 + k, la)) count_bad(a.bad);
+        }
+      } else if constexpr (JACC == 3) {
+        const LdsAccFx la{{sj, sj + nj, sj + 2 * nj}, R0[0], R0[1], R0[2], py, pz, fxs};
+        for (int kk = threadIdx.x; kk < nkk; kk += BS) {
+          const int k = pick(kk);
+          if (k < ch.n && !gpd_deposit<N, DEPO>(a, p0 + k, la)) count_bad(a.bad);
         }
       } else {
         double *sjd = reinterpret_cast<double *>(smem);
@@ -625,7 +682,9 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(
       const i64 gi_ = g.idx(R0[0] + i, R0[1] + j, R0[2] + k);
       const int li = i + py * j + pz * k;
       for (int c3 = 0; c3 < 3; c3++) {
-        double v = (double)sj[c3 * nj + li];
+        double v;
+        if constexpr (JACC == 3) v = (double)(long long)sj[c3 * nj + li] * fxs_inv;
+        else v = (double)sj[c3 * nj + li];
         if (MX && JACC == 1) v += (double)sj[(3 + c3) * nj + li];
         if (v != 0.0) unsafeAtomicAdd(a.J[c3] + gi_, v);
       }
@@ -648,17 +707,18 @@ __global__ __launch_bounds__(BS) void k_fused_overflow(FusedArgs a) {
     if (ov.y == 0 && (a.phase & PHASE_PUSH)) {
       Box6 dep;
       box_reset(dep);
+      double wabs = 0.0;
       if constexpr (MX) {
         FieldViewF fg[6];
         for (int f = 0; f < 6; f++) fg[f] = FieldViewF{a.F[f] + g.idx(0, 0, 0), g.NX, g.NX * g.NY};
         for (int k = threadIdx.x; k < ch.n; k += BS)
-          gpd_push_mx<N, LO, PUSHER, DEPO>(a, ch.begin + k, fg, fg, none, none, dep, false);
+          gpd_push_mx<N, LO, PUSHER, DEPO>(a, ch.begin + k, fg, fg, none, none, dep, false, wabs);
       } else {
         FieldView<double> fg[6];
         for (int f = 0; f < 6; f++)
           fg[f] = FieldView<double>{a.F[f] + g.idx(0, 0, 0), g.NX, g.NX * g.NY};
         for (int k = threadIdx.x; k < ch.n; k += BS)
-          gpd_push<N, LO, PUSHER, DEPO>(a, ch.begin + k, fg, fg, none, none, dep, false);
+          gpd_push<N, LO, PUSHER, DEPO>(a, ch.begin + k, fg, fg, none, none, dep, false, wabs);
       }
     }
     __syncthreads();
@@ -682,5 +742,7 @@ __global__ __launch_bounds__(BS) void k_fused_overflow(FusedArgs a) {
 int launch_fused_fp64(int key, const FusedArgs &a, int nwg, size_t lds, hipStream_t st);
 int launch_fused_mixed_f32(int key, const FusedArgs &a, int nwg, size_t lds, hipStream_t st);
 int launch_fused_mixed_f64acc(int key, const FusedArgs &a, int nwg, size_t lds, hipStream_t st);
+int launch_fused_fp64_fx(int key, const FusedArgs &a, int nwg, size_t lds, hipStream_t st);
+int launch_fused_mixed_fx(int key, const FusedArgs &a, int nwg, size_t lds, hipStream_t st);
 
 }  // namespace pxr

@@ -208,20 +208,37 @@ PXR_DEV void direct_prepare_mx(DirectIn<float> &in, const double *pn, const floa
   }
 }
 
+// Fixed-point value of v: round(v * scale) as a 64-bit two's-complement
+// integer, for |v * scale| < 2^51.  Adding 1.5 * 2^52 rounds v * scale to an
+// integer in the low mantissa bits (the ulp there is 1); subtracting the
+// constant's bit pattern leaves that integer.
+PXR_DEV unsigned long long fixed_point(double v, double scale) {
+  const double magic = 6755399441055744.0;  // 1.5 * 2^52
+  const double t = fma(v, scale, magic);
+  return (unsigned long long)(__double_as_longlong(t) - __double_as_longlong(magic));
+}
+
 // LDS accumulators of the current block of a mixed-precision species,
 // selected at compile time (JACC):
 //   0  FP32 atomics;
 //   1  FP32 atomics compensated: the rounding error of every add, recovered
 //      from the returned old value by TwoSum, goes to a second FP32 array;
-//   2  FP64 atomics on the FP32 contributions.
+//   2  FP64 atomics on the FP32 contributions;
+//   3  64-bit integer atomics on the contributions in fixed point (scale a
+//      power of two per work item): exact, order-independent sums.
 template <int JACC> struct LdsAccMx {
-  typedef typename std::conditional<JACC == 2, double, float>::type AT;
+  typedef typename std::conditional<
+      JACC == 3, unsigned long long,
+      typename std::conditional<JACC == 2, double, float>::type>::type AT;
   AT *p[3];
   float *e[3];
   int i0, j0, k0, sy, sz;
+  double scale;  // JACC 3
   PXR_DEV void add(int c, int i, int j, int k, float v) const {
     const int o = (i - i0) + sy * (j - j0) + sz * (k - k0);
-    if constexpr (JACC == 1) {
+    if constexpr (JACC == 3) {
+      atomicAdd(p[c] + o, fixed_point((double)v, scale));
+    } else if constexpr (JACC == 1) {
       const float old = atomicAdd(p[c] + o, v);
       const float s = old + v;
       const float bp = s - old;

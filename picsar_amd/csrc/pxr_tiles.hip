@@ -802,7 +802,9 @@ struct Tuning {
   int margin_pct = 0;     // motion margin of the current block, in 1/100 cell
   int jpad = 0;           // odd row/plane pitch of the LDS current block
   int sstride = 0;        // largest lane stride of the deposition (0: whole work item)
-  int jacc = 2;           // mixed precision LDS current: 0 FP32, 1 compensated FP32, 2 FP64
+  int jacc = 2;           // mixed precision LDS current: 0 FP32, 1 compensated FP32, 2 FP64,
+                          // 3 fixed point
+  int fp64_jacc = 0;      // FP64 species' LDS current: 0 FP64, 3 fixed point
   int stable_rebin = 1;   // re-binning keeps the source order inside a bin (0: arrival order)
   // launch of a mixed-precision species: its kernels take fewer registers,
   // so three workgroups with smaller LDS blocks fit on a CU
@@ -1005,8 +1007,12 @@ int pxrhip_set_tuning(const char *key, int64_t value) {
       return set_error(PXRHIP_ERR_ARG, "mixed.wg_per_cu must be in [0, 16] (0: automatic)");
     g_tune.mx_wg_per_cu = (int)value;
   } else if (!strcmp(key, "mixed.jacc")) {
-    if (value < 0 || value > 2) return set_error(PXRHIP_ERR_ARG, "mixed.jacc must be 0, 1 or 2");
+    if (value < 0 || value > 3)
+      return set_error(PXRHIP_ERR_ARG, "mixed.jacc must be 0, 1, 2 or 3");
     g_tune.jacc = (int)value;
+  } else if (!strcmp(key, "fused.jacc")) {
+    if (value != 0 && value != 3) return set_error(PXRHIP_ERR_ARG, "fused.jacc must be 0 or 3");
+    g_tune.fp64_jacc = (int)value;
   } else if (!strcmp(key, "fused.sstride")) {
     if (value < 0 || value > (1 << 20)) return set_error(PXRHIP_ERR_ARG, "fused.sstride must be in [0, 2^20]");
     g_tune.sstride = (int)value;
@@ -1038,6 +1044,7 @@ int pxrhip_get_tuning(const char *key, int64_t *value) {
   else if (!strcmp(key, "rebin.stable")) *value = g_tune.stable_rebin;
   else if (!strcmp(key, "fused.sstride")) *value = g_tune.sstride;
   else if (!strcmp(key, "mixed.jacc")) *value = g_tune.jacc;
+  else if (!strcmp(key, "fused.jacc")) *value = g_tune.fp64_jacc;
   else if (!strcmp(key, "mixed.lds_bytes")) *value = g_tune.mx_lds_bytes;
   else if (!strcmp(key, "mixed.wg_per_cu")) *value = g_tune.mx_wg_per_cu;
   else if (!strcmp(key, "last.wg_per_cu")) *value = g_last_wg_per_cu;
@@ -1313,8 +1320,11 @@ static int tiles_fused(const pxrhip_grid *grid, const pxrhip_species *sp, const 
   }
   const bool mx = sp->precision == 1;
   const int esize = mx ? 4 : 8;
-  const int jacc = mx ? g_tune.jacc : 2;  // FP64 species: FP64 LDS current
-  const int jnode = jacc == 0 ? 12 : 24;  // 3 FP32 | 3 FP32 + 3 compensations | 3 FP64
+  // LDS current: mixed.jacc (0 FP32, 1 compensated FP32, 2 FP64, 3 fixed
+  // point) for a mixed-precision species, fused.jacc (0 FP64, 3 fixed point)
+  // for an FP64 one
+  const int jacc = mx ? g_tune.jacc : g_tune.fp64_jacc;
+  const int jnode = (mx && jacc == 0) ? 12 : 24;  // 3 FP32 | 3 x 8 bytes
   i64 lds_bytes = mx ? g_tune.mx_lds_bytes : g_tune.lds_bytes;
   int wg_per_cu = mx ? g_tune.mx_wg_per_cu : g_tune.wg_per_cu;
   if (wg_per_cu == 0) {
@@ -1387,12 +1397,20 @@ static int tiles_fused(const pxrhip_grid *grid, const pxrhip_species *sp, const 
   a.ovf = ovf;
   a.ovf_cap = maxchunks < INT_MAX ? (int)maxchunks : INT_MAX;
   a.phase = phase;
+  // largest |current contribution| per unit |q w|: Esirkepov's running sums
+  // of shape differences are at most 1 and its transverse weights at most
+  // 3/2 (esirkepov_3d.F90:3427-3450); direct shapes and |v| are at most 1
+  a.fxb = depo_algo ? 1.0 / (grid->dx * grid->dy * grid->dz)
+                    : 1.5 * fmax(a.ec.invdtdx, fmax(a.ec.invdtdy, a.ec.invdtdz));
   const int nwg = rt().num_cus * wg_per_cu;
   g_last_wg_per_cu = wg_per_cu;
   g_last_lds_bytes = (int)lds_bytes;
   const size_t lds = (size_t)lds_bytes;
   const int key = (int)(order * 100 + (l_lower_order_in_v ? 10 : 0) + pusher * 2 + (depo_algo ? 1 : 0));
-  if (!mx) return launch_fused_fp64(key, a, nwg, lds, st);
+  if (!mx)
+    return jacc == 3 ? launch_fused_fp64_fx(key, a, nwg, lds, st)
+                     : launch_fused_fp64(key, a, nwg, lds, st);
+  if (jacc == 3) return launch_fused_mixed_fx(key, a, nwg, lds, st);
   return jacc == 2 ? launch_fused_mixed_f64acc(key, a, nwg, lds, st)
                    : launch_fused_mixed_f32(key, a, nwg, lds, st);
 }

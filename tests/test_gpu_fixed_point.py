@@ -1,0 +1,96 @@
+"""The LDS current block in 64-bit fixed point (tuning keys fused.jacc = 3
+for FP64 species, mixed.jacc = 3 for mixed-precision ones).
+
+Every contribution is added to an integer counter as round(v * 2^k), with
+2^k chosen per work item from the largest |q w| of its particles, so a
+block's sums are exact and do not depend on the order of the atomics; the
+block is converted back once, when it is added to J.  The step must still
+match the oracle at the FP64 bar, and mixed precision at its own.
+"""
+import numpy as np
+import pytest
+
+from tests.test_gpu_step import CASES, build, compare_fields, compare_particles, rel
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch():
+    from picsar_amd import _lib
+    _lib.call("pxrhip_init", 0)
+    return None
+
+
+def set_tuning(key, value):
+    from picsar_amd import _lib
+    import ctypes as C
+    old = C.c_int64(0)
+    _lib.call("pxrhip_get_tuning", key.encode(), C.byref(old))
+    _lib.call("pxrhip_set_tuning", key.encode(), value)
+    return old.value
+
+
+@pytest.mark.parametrize("order,lo,pusher,depo,norder", CASES)
+def test_fp64_step_in_fixed_point(torch, orc, order, lo, pusher, depo, norder):
+    old = set_tuning("fused.jacc", 3)
+    try:
+        sim, ref, tng = build(torch, orc, order, lo, pusher, depo, norder)
+        sim.step(1)
+        ref.step()
+        compare_fields(sim, ref, tng, 1e-12)
+        L = max(sim.nx * sim.dx, sim.ny * sim.dy, sim.nz * sim.dz)
+        compare_particles(sim, ref, 1e-12 * L, 1e-12)
+        sim.step(2)
+        ref.step()
+        ref.step()
+        compare_fields(sim, ref, tng, 1e-9)
+    finally:
+        set_tuning("fused.jacc", old)
+
+
+@pytest.mark.parametrize("uth", [(0.01, 0.001), (3.0, 0.3)])
+def test_fixed_point_slow_and_relativistic(torch, orc, uth):
+    """A cold plasma (tiny, cancelling contributions) and a relativistic one
+    (wide Esirkepov boxes): 1e-12 after one step."""
+    old = set_tuning("fused.jacc", 3)
+    try:
+        sim, ref, tng = build(torch, orc, 3, True, "vay", "esirkepov", uth=uth, seed=5)
+        sim.step(1)
+        ref.step()
+        compare_fields(sim, ref, tng, 1e-12)
+    finally:
+        set_tuning("fused.jacc", old)
+
+
+@pytest.mark.parametrize("order,depo", [(1, "direct"), (2, "esirkepov"), (3, "esirkepov"),
+                                        (3, "direct")])
+def test_mixed_step_in_fixed_point(torch, orc, order, depo):
+    """Mixed precision with the fixed-point block against the FP64 oracle
+    started from the same FP64 values (the FP32 storage of momenta and
+    weights is part of the error): J, E, B within 1e-5 after one step."""
+    old = set_tuning("mixed.jacc", 3)
+    try:
+        sim, ref, tng = build(torch, orc, order, True, "vay", depo, precision="mixed",
+                              id_scale=2.0 ** -15, ppc=2)
+        sim.step(1)
+        ref.step()
+        for k in ("ex", "ey", "ez", "bx", "by", "bz", "jx", "jy", "jz"):
+            assert rel(tng(sim.fields[k]), ref.f[k]) <= 1e-5, k
+    finally:
+        set_tuning("mixed.jacc", old)
+
+
+def test_fixed_point_matches_fp64_block_in_tiled_step(torch, orc):
+    """The same state stepped with the FP64 block and with the fixed-point
+    block: J agrees to round-off (1e-14)."""
+    sa, _, tng = build(torch, orc, 3, True, "vay", "esirkepov", seed=9)
+    sb, _, _ = build(torch, orc, 3, True, "vay", "esirkepov", seed=9)
+    sa.step(1)
+    old = set_tuning("fused.jacc", 3)
+    try:
+        sb.step(1)
+    finally:
+        set_tuning("fused.jacc", old)
+    for k in ("jx", "jy", "jz"):
+        assert rel(tng(sb.fields[k]), tng(sa.fields[k])) <= 1e-14, k

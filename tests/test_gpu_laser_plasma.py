@@ -71,6 +71,8 @@ def build(torch, orc, precision):
     pi = [rng.random(ni) * NX, rng.random(ni) * NY, Z0 + rng.random(ni) * (NZ - Z0)]
     zero = np.zeros(ni)
     wi = 0.1 * (2 * np.pi / LAMBDA) ** 2 * np.clip((pi[2] - Z0) / RAMP, 0.0, 1.0)
+    wi = wi * (1.0 + np.arange(ni) * 1e-9)  # ids
+    assert len(np.unique(wi)) == ni
     ref.add_species(1.0, 1836.0, *pi, zero, zero, zero, wi, lfreeze=True, ldodepos=False)
     sim.add_species("ions", 1.0, 1836.0, *pi, zero, zero, zero, wi, lfreeze=True,
                     ldodepos=False)
