@@ -260,6 +260,25 @@ int pxrhip_particle_bin_sorting(int64_t np2, double *xp, double *yp,
                                 double xmax2, double ymax2, double zmax2,
                                 double dxf, double dyf, double dzf);
 
+/* No counterpart in the reference.  pxrhip_particle_bin_sorting for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * loops, harnesses that forbid hipMalloc): the same arguments, the same
+ * result, but its keys, histogram, permutation and staging live in a fixed
+ * device array of the library (about 20 MB per GPU, held until the process
+ * ends) instead of buffers it allocates.  Limits: np2 <= 2^20 particles and
+ * at most 2^20 sorting-grid bins, else PXRHIP_ERR_ARG; pxrhip_init must have
+ * been called (it allocates, so this entry point does not call it).  Every
+ * call on one GPU uses that GPU's one array: calls that overlap on a GPU
+ * (other host threads, other streams) race. */
+int pxrhip_particle_bin_sorting_noalloc(int64_t np2, double *xp, double *yp,
+                                        double *zp, double *ux, double *uy,
+                                        double *uz, double *gam, double *pid,
+                                        int64_t wpid, double xmin2,
+                                        double ymin2, double zmin2,
+                                        double xmax2, double ymax2,
+                                        double zmax2, double dxf, double dyf,
+                                        double dzf);
+
 /* ---- tiled step (the managers' work on device-resident tiles) ----------- */
 /* Geometry shared by the tile-level entry points: one periodic rank with
  * nx*ny*nz cells, field and current arrays of identical shape with guards

@@ -68,6 +68,7 @@ SIGNATURES = {
     "pxrhip_field_bc_periodic": [_P] + [_I] * 6,
     "pxrhip_current_bc_periodic": [_P] + [_I] * 6,
     "pxrhip_particle_bin_sorting": [_I] + [_P] * 8 + [_I] + [_F] * 9,
+    "pxrhip_particle_bin_sorting_noalloc": [_I] + [_P] * 8 + [_I] + [_F] * 9,
     "pxrhip_laserp_pusher_gaussian": [_I, _I] + [_P] * 8 + [_F, _I] + [_F] * 3 + [_P, _P, _F, _P]
                                      + [_F] * 4 + [_I, _F],
     "pxrhip_laserp_pusher_hanning": [_I, _I] + [_P] * 8 + [_F, _I] + [_F] * 3 + [_P, _P, _F, _P]

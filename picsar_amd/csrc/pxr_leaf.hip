@@ -746,6 +746,12 @@ __global__ __launch_bounds__(256) void k_fill_i32(int *p, i64 n, int v) {
   if (t < n) p[t] = v;
 }
 
+__global__ __launch_bounds__(64) void k_fill_u64(unsigned long long *p, i64 n,
+                                                 unsigned long long v) {
+  const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = v;
+}
+
 // key (0-based bin = reference key - 1), histogram, range check
 __global__ __launch_bounds__(256) void k_sort_keys(i64 np, const double *xp, const double *yp,
                                                    const double *zp, SortGeom s, int *key,
@@ -1347,19 +1353,21 @@ int pxrhip_current_bc_periodic(double *a, int64_t nx, int64_t ny, int64_t nz, in
   return PXRHIP_OK;
 }
 
-int pxrhip_particle_bin_sorting(int64_t np2, double *xp, double *yp, double *zp, double *ux,
-                                double *uy, double *uz, double *gam, double *pid, int64_t wpid,
-                                double xmin2, double ymin2, double zmin2, double xmax2,
-                                double ymax2, double zmax2, double dxf, double dyf, double dzf) {
-  PXR_INIT();
+}  // extern "C"
+
+// Arguments of pxr_particle_bin_sorting checked and turned into the sorting
+// grid (sorting.F90:274-278).  Returns PXRHIP_OK with *empty set when there
+// is nothing to sort.
+static int bin_sort_geom(int64_t np2, double *const *arr, int64_t wpid, double xmin2,
+                         double ymin2, double zmin2, double xmax2, double ymax2, double zmax2,
+                         double dxf, double dyf, double dzf, SortGeom &s, bool &empty) {
+  empty = true;
   if (np2 < 0) return set_error(PXRHIP_ERR_ARG, "np2 < 0");
   if (wpid != 1) return set_error(PXRHIP_ERR_ARG, "only the first pid column (wpid=1) is permuted");
   if (np2 == 0) return PXRHIP_OK;
   if (np2 > INT32_MAX) return set_error(PXRHIP_ERR_ARG, "np2 too large for one sort");
-  double *arr[8] = {xp, yp, zp, ux, uy, uz, gam, pid};
   for (int c = 0; c < 8; c++)
     if (!arr[c]) return set_error(PXRHIP_ERR_ARG, "null array");
-  SortGeom s;
   s.xmin = xmin2; s.ymin = ymin2; s.zmin = zmin2;
   s.dxi = 1.0 / dxf; s.dyi = 1.0 / dyf; s.dzi = 1.0 / dzf;  // sorting.F90:274-276
   s.nx3 = (i64)ceil((xmax2 - xmin2) * s.dxi);
@@ -1369,23 +1377,32 @@ int pxrhip_particle_bin_sorting(int64_t np2, double *xp, double *yp, double *zp,
   const double nb = (double)s.nx3 * (double)s.ny3 * (double)nz3;
   if (nb > 1.0e8) return set_error(PXRHIP_ERR_ARG, "sorting grid has more than 1e8 bins");
   s.nbins = s.nx3 * s.ny3 * nz3;
-  int *key = (int *)scratch(0, sizeof(int) * np2);
-  int *hist = (int *)scratch(1, sizeof(int) * s.nbins);
-  int *dest = (int *)scratch(2, sizeof(int) * np2);
-  double *tmp = (double *)scratch(3, sizeof(double) * np2);
-  if (!key || !hist || !dest || !tmp) return set_error(PXRHIP_ERR_HIP, "scratch allocation failed");
+  empty = false;
+  return PXRHIP_OK;
+}
+
+// The sort itself, on the library stream, with the caller's buffers: key
+// and dest hold np2 ints, tmp np2 doubles, hist s.nbins ints, bad one
+// counter that is zero on entry and left zero on return.  Keys are checked
+// before anything moves (the host reads the counter, so the call
+// synchronises the stream once).
+static int bin_sort_run(int64_t np2, double *const *arr, const SortGeom &s, int *key, int *hist,
+                        int *dest, double *tmp, unsigned long long *bad_ctr) {
   hipStream_t st = rt().stream;
   hipLaunchKernelGGL(k_fill_i32, dim3(grid1d(s.nbins, 256)), dim3(256), 0, st, hist, s.nbins, 0);
-  hipLaunchKernelGGL(k_sort_keys, dim3(grid1d(np2, 256)), dim3(256), 0, st, np2, xp, yp, zp, s, key,
-                     hist, rt().d_bad);
+  hipLaunchKernelGGL(k_sort_keys, dim3(grid1d(np2, 256)), dim3(256), 0, st, np2, arr[0], arr[1],
+                     arr[2], s, key, hist, bad_ctr);
   PXR_LAUNCHED();
   unsigned long long bad = 0;
-  int r = take_bad(&bad);
-  if (r) return r;
-  if (bad)
+  PXR_HIP(hipMemcpyAsync(&bad, bad_ctr, sizeof(bad), hipMemcpyDeviceToHost, st));
+  PXR_HIP(hipStreamSynchronize(st));
+  if (bad) {
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, bad_ctr, 1, 0ull);
+    PXR_LAUNCHED();
     return set_error(PXRHIP_ERR_BOUNDS,
                      "particle_bin_sorting: %llu particle(s) have a cell key outside [1,%lld]; "
                      "nothing was moved", bad, (long long)s.nbins);
+  }
   hipLaunchKernelGGL(k_scan_1wg, dim3(1), dim3(1024), 0, st, hist, s.nbins);
   hipLaunchKernelGGL(k_sort_scatter_1wg, dim3(1), dim3(1024), 0, st, np2, key, hist, dest);
   PXR_LAUNCHED();
@@ -1395,6 +1412,90 @@ int pxrhip_particle_bin_sorting(int64_t np2, double *xp, double *yp, double *zp,
   }
   PXR_LAUNCHED();
   return PXRHIP_OK;
+}
+
+// Fixed device scratch of pxrhip_particle_bin_sorting_noalloc: key, dest,
+// tmp and hist of one sort of at most SORT_NA_NP particles over at most
+// SORT_NA_NBINS bins, and the bad-key counter.  Each GPU holds its own copy
+// until the process ends; calls on one GPU share it.
+constexpr i64 SORT_NA_NP = 1 << 20;
+constexpr i64 SORT_NA_NBINS = 1 << 20;
+constexpr size_t SORT_NA_OFF_DEST = sizeof(int) * SORT_NA_NP;
+constexpr size_t SORT_NA_OFF_TMP = SORT_NA_OFF_DEST + sizeof(int) * SORT_NA_NP;
+constexpr size_t SORT_NA_OFF_HIST = SORT_NA_OFF_TMP + sizeof(double) * SORT_NA_NP;
+constexpr size_t SORT_NA_OFF_BAD = SORT_NA_OFF_HIST + sizeof(int) * SORT_NA_NBINS;
+constexpr size_t SORT_NA_BYTES = SORT_NA_OFF_BAD + 256;
+static __device__ __align__(256) unsigned char sort_scratch[SORT_NA_BYTES];
+
+// Device address of sort_scratch on each device (the symbol's name in host
+// code is not a device address), looked up on the first call on a device.
+constexpr int SORT_NA_MAXDEV = 64;
+static unsigned char *g_sort_scratch[SORT_NA_MAXDEV] = {};
+
+static int sort_scratch_ptr(unsigned char **out) {
+  const int dev = rt().device;
+  if (dev < 0 || dev >= SORT_NA_MAXDEV)
+    return set_error(PXRHIP_ERR_ARG, "device %d out of range for the fixed sort scratch", dev);
+  if (!g_sort_scratch[dev]) {
+    void *p = nullptr;
+    PXR_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(sort_scratch)));
+    // the counter starts at zero: a fill kernel, not a memset, so that
+    // work recorded into a graph clears it the same way
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, rt().stream,
+                       (unsigned long long *)((unsigned char *)p + SORT_NA_OFF_BAD), 1, 0ull);
+    PXR_LAUNCHED();
+    g_sort_scratch[dev] = (unsigned char *)p;
+  }
+  *out = g_sort_scratch[dev];
+  return PXRHIP_OK;
+}
+
+extern "C" {
+
+int pxrhip_particle_bin_sorting(int64_t np2, double *xp, double *yp, double *zp, double *ux,
+                                double *uy, double *uz, double *gam, double *pid, int64_t wpid,
+                                double xmin2, double ymin2, double zmin2, double xmax2,
+                                double ymax2, double zmax2, double dxf, double dyf, double dzf) {
+  PXR_INIT();
+  double *arr[8] = {xp, yp, zp, ux, uy, uz, gam, pid};
+  SortGeom s;
+  bool empty;
+  int r = bin_sort_geom(np2, arr, wpid, xmin2, ymin2, zmin2, xmax2, ymax2, zmax2, dxf, dyf, dzf, s,
+                        empty);
+  if (r || empty) return r;
+  int *key = (int *)scratch(0, sizeof(int) * np2);
+  int *hist = (int *)scratch(1, sizeof(int) * s.nbins);
+  int *dest = (int *)scratch(2, sizeof(int) * np2);
+  double *tmp = (double *)scratch(3, sizeof(double) * np2);
+  if (!key || !hist || !dest || !tmp) return set_error(PXRHIP_ERR_HIP, "scratch allocation failed");
+  return bin_sort_run(np2, arr, s, key, hist, dest, tmp, rt().d_bad);
+}
+
+int pxrhip_particle_bin_sorting_noalloc(int64_t np2, double *xp, double *yp, double *zp,
+                                        double *ux, double *uy, double *uz, double *gam,
+                                        double *pid, int64_t wpid, double xmin2, double ymin2,
+                                        double zmin2, double xmax2, double ymax2, double zmax2,
+                                        double dxf, double dyf, double dzf) {
+  // no implicit set-up here: pxrhip_init allocates, this entry point may not
+  if (rt().device < 0)
+    return set_error(PXRHIP_ERR_ARG, "call pxrhip_init before pxrhip_particle_bin_sorting_noalloc");
+  double *arr[8] = {xp, yp, zp, ux, uy, uz, gam, pid};
+  SortGeom s;
+  bool empty;
+  int r = bin_sort_geom(np2, arr, wpid, xmin2, ymin2, zmin2, xmax2, ymax2, zmax2, dxf, dyf, dzf, s,
+                        empty);
+  if (r || empty) return r;
+  if (np2 > SORT_NA_NP)
+    return set_error(PXRHIP_ERR_ARG, "np2 = %lld exceeds the %lld particles of the fixed sort scratch",
+                     (long long)np2, (long long)SORT_NA_NP);
+  if (s.nbins > SORT_NA_NBINS)
+    return set_error(PXRHIP_ERR_ARG, "the sorting grid has %lld bins, more than the %lld of the "
+                     "fixed sort scratch", (long long)s.nbins, (long long)SORT_NA_NBINS);
+  unsigned char *base;
+  if ((r = sort_scratch_ptr(&base))) return r;
+  return bin_sort_run(np2, arr, s, (int *)base, (int *)(base + SORT_NA_OFF_HIST),
+                      (int *)(base + SORT_NA_OFF_DEST), (double *)(base + SORT_NA_OFF_TMP),
+                      (unsigned long long *)(base + SORT_NA_OFF_BAD));
 }
 
 }  // extern "C"
