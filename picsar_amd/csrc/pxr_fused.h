@@ -201,6 +201,18 @@ PXR_DEV bool in_block(const int *lo, const int *hi, const int *R0, const int *e)
          hi[1] < R0[1] + e[1] && hi[2] < R0[2] + e[2];
 }
 
+// Bound of a particle's largest current contribution, in units of a.fxb:
+// |q w| times the largest of its displacement in cells (Esirkepov: the
+// running sums of shape differences are bounded by it) or of its velocity
+// components (direct deposition).
+template <int DEPO>
+PXR_DEV double fx_weight(const FusedArgs &a, double qw, double vx, double vy, double vz) {
+  const double f = DEPO == 0 ? fmax(fabs(vx) * a.ec.dtsdx0,
+                                    fmax(fabs(vy) * a.ec.dtsdy0, fabs(vz) * a.ec.dtsdz0))
+                             : fmax(fabs(vx), fmax(fabs(vy), fabs(vz)));
+  return fabs(qw) * f;
+}
+
 // ---------------------------------------------------------------------------
 // FP64 particle
 // ---------------------------------------------------------------------------
@@ -241,7 +253,7 @@ PXR_DEV void gpd_push(const FusedArgs &a, i64 p, const VL *fl, const VG *fg, con
   a.y[p] = yn;
   a.z[p] = zn;
   if (!box) return;
-  wabs = fmax(wabs, fabs(a.q * a.w[p]));
+  wabs = fmax(wabs, fx_weight<DEPO>(a, a.q * a.w[p], u[0] * gi, u[1] * gi, u[2] * gi));
   int b0[3], b1[3];
   if (DEPO == 0) {
     EsirkepovIn<double> in;
@@ -263,7 +275,8 @@ PXR_DEV void gpd_push(const FusedArgs &a, i64 p, const VL *fl, const VG *fg, con
 // stands) deposits: the deposition-only phase.
 template <int N, int DEPO>
 PXR_DEV void gpd_box(const FusedArgs &a, i64 p, Box6 &dep, double &wabs) {
-  wabs = fmax(wabs, fabs(a.q * a.w[p]));
+  wabs = fmax(wabs, fx_weight<DEPO>(a, a.q * a.w[p], a.ux[p] * a.gi[p], a.uy[p] * a.gi[p],
+                                    a.uz[p] * a.gi[p]));
   int b0[3], b1[3];
   if (DEPO == 0) {
     EsirkepovIn<double> in;
@@ -369,7 +382,7 @@ PXR_DEV void gpd_push_mx(const FusedArgs &a, i64 p, const VL *fl, const VG *fg, 
   a.y[p] = yn;
   a.z[p] = zn;
   if (!box) return;
-  wabs = fmax(wabs, fabs(a.q * (double)a.fw[p]));
+  wabs = fmax(wabs, fx_weight<DEPO>(a, a.q * (double)a.fw[p], v[0], v[1], v[2]));
   const double pn[3] = {(xn - a.xmin) * a.dxi, (yn - a.ymin) * a.dyi, (zn - a.zmin) * a.dzi};
   int b0[3], b1[3];
   if (!mx_box<N, DEPO>(a, pn, v, b0, b1)) return;
@@ -382,10 +395,10 @@ PXR_DEV void gpd_push_mx(const FusedArgs &a, i64 p, const VL *fl, const VG *fg, 
 // mixed-precision counterpart of gpd_box
 template <int N, int DEPO>
 PXR_DEV void gpd_box_mx(const FusedArgs &a, i64 p, Box6 &dep, double &wabs) {
-  wabs = fmax(wabs, fabs(a.q * (double)a.fw[p]));
   const float gi = a.fgi[p];
   const double v[3] = {(double)a.fux[p] * (double)gi, (double)a.fuy[p] * (double)gi,
                        (double)a.fuz[p] * (double)gi};
+  wabs = fmax(wabs, fx_weight<DEPO>(a, a.q * (double)a.fw[p], v[0], v[1], v[2]));
   const double pn[3] = {(a.x[p] - a.xmin) * a.dxi, (a.y[p] - a.ymin) * a.dyi,
                         (a.z[p] - a.zmin) * a.dzi};
   int b0[3], b1[3];
@@ -600,7 +613,9 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(
     __syncthreads();
     // fixed point (JACC 3): a power-of-two scale under which every
     // contribution of the work item is below 2^50 in magnitude and a node's
-    // sum of at most ch.n of them stays below 2^62
+    // sum of at most ch.n of them stays below 2^62; the bound is a.fxb times
+    // the largest fx_weight of the work item, so the quantum follows the
+    // size of the item's actual contributions
     double fxs = 1.0, fxs_inv = 1.0;
     if constexpr (JACC == 3) {
       const double bound = a.fxb * __longlong_as_double((long long)s_wabs);

@@ -1020,6 +1020,7 @@ int pxrhip_set_tuning(const char *key, int64_t value) {
     g_tune.jpad = value != 0;
   } else if (!strcmp(key, "rebin.stable")) {
     g_tune.stable_rebin = value != 0;
+
   } else if (!strcmp(key, "fused.margin_pct")) {
     if (value < 0 || value > 200) return set_error(PXRHIP_ERR_ARG, "fused.margin_pct must be in [0, 200]");
     g_tune.margin_pct = (int)value;
@@ -1397,9 +1398,11 @@ static int tiles_fused(const pxrhip_grid *grid, const pxrhip_species *sp, const 
   a.ovf = ovf;
   a.ovf_cap = maxchunks < INT_MAX ? (int)maxchunks : INT_MAX;
   a.phase = phase;
-  // largest |current contribution| per unit |q w|: Esirkepov's running sums
-  // of shape differences are at most 1 and its transverse weights at most
-  // 3/2 (esirkepov_3d.F90:3427-3450); direct shapes and |v| are at most 1
+  // largest |current contribution| per unit of fx_weight (|q w| times the
+  // displacement in cells, or times |v| for the direct form): Esirkepov's
+  // running sums of shape differences along an axis are at most the
+  // displacement along it and its transverse weights at most 3/2
+  // (esirkepov_3d.F90:3427-3450); direct shapes are at most 1
   a.fxb = depo_algo ? 1.0 / (grid->dx * grid->dy * grid->dz)
                     : 1.5 * fmax(a.ec.invdtdx, fmax(a.ec.invdtdy, a.ec.invdtdz));
   const int nwg = rt().num_cus * wg_per_cu;
