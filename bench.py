@@ -449,8 +449,9 @@ def cpu_baseline(args, port=False, seconds=None):
     the oracle ("port") when they are absent or when ``port`` asks for it.
     The sample is whole tiles of the timed workload: the tile size and
     particles per cell of the run, electrons and ions."""
-    from oracle import load_oracle, load_reference
+    from oracle import load_oracle, load_ref_driver, load_reference
     ref = None if port else load_reference(production=True)
+    drv = None if port else load_ref_driver()
     kind = "reference" if ref is not None else "port"
     lib = ref if ref is not None else load_oracle()
     tnodes = (args.n + 1) // args.tiles     # nodes per tile of the workload
@@ -501,15 +502,25 @@ def cpu_baseline(args, port=False, seconds=None):
         return done
 
     budget = args.cpu_seconds if seconds is None else seconds
-    total, t0 = 0, time.perf_counter()
-    i = 0
-    with cf.ThreadPoolExecutor(max_workers=threads) as ex:
-        while time.perf_counter() - t0 < budget:
-            futs = [ex.submit(tile_work, i + k) for k in range(threads)]
-            i += threads
-            for f in futs:
-                total += f.result()
-    wall = time.perf_counter() - t0
+    if drv is not None and ref is not None:
+        # the reference's kernels under an OpenMP loop over tiles (no Python
+        # in the timed loop), one tile per thread at a time as its managers
+        pool_arr = np.stack([np.stack([np.stack(arrs) for _, _, arrs in sp]) for sp in pool])
+        total, wall = drv.time_tiles(threads, budget, nc, ng, fields, pool_arr,
+                                     [sp[0] for sp in pool[0]], [sp[1] for sp in pool[0]], dt)
+        loop = "an OpenMP loop over tiles (oracle/ref_tile_driver.c)"
+    else:
+        total, t0 = 0, time.perf_counter()
+        i = 0
+        with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+            while time.perf_counter() - t0 < budget:
+                futs = [ex.submit(tile_work, i + k) for k in range(threads)]
+                i += threads
+                for f in futs:
+                    total += f.result()
+        wall = time.perf_counter() - t0
+        loop = "Python threads over tiles"
+    total = int(total)
     build = ("the reference's Fortran kernels built with amdflang -O3 -fopenmp -DLIBRARY "
              "(its Makefile's optimisation flags)") if kind == "reference" else \
         "the oracle's C restatement, gcc -O2"
@@ -518,7 +529,7 @@ def cpu_baseline(args, port=False, seconds=None):
             "sample": f"{total} particle-steps in {wall:.1f} s: whole {tnodes}^3-node tiles of "
                       f"the workload ({ppc} ppc electrons + ions), order-3 gather + Vay + "
                       f"pushxyz + order-3 Esirkepov into the tile J block, {threads} host "
-                      f"threads over tiles (FP64)"}
+                      f"threads, {loop} (FP64)"}
 
 
 def roofline_record(r, args, prec):

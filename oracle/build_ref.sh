@@ -102,3 +102,9 @@ build_lib "$FFLAGS_GOLDEN" libpxrref.so "" &
 build_lib "$FFLAGS_PROD" libpxrref_o3.so "-fopenmp" &
 wait %1
 wait %2
+# the CPU baseline's OpenMP tile loop over the production kernels
+CC=${CC_OMP:-/opt/rocm/llvm/bin/clang}
+$CC -O2 -fopenmp -fPIC -shared "$HERE/ref_tile_driver.c" -L"$OUT" -lpxrref_o3 \
+    -Wl,-rpath,'$ORIGIN' -o "$OUT/libpxrref_driver.so.tmp"
+mv "$OUT/libpxrref_driver.so.tmp" "$OUT/libpxrref_driver.so"
+echo "build_ref: wrote $OUT/libpxrref_driver.so"

@@ -21,6 +21,8 @@ ORACLE_SO = os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libpxrref.so")
 # the same kernels built with the reference's production flags (-O3 -fopenmp)
 REF_O3_SO = os.path.join(HERE, "_ref", "libpxrref_o3.so")
+# OpenMP tile loop over the production kernels (ref_tile_driver.c)
+REF_DRIVER_SO = os.path.join(HERE, "_ref", "libpxrref_driver.so")
 
 _i64 = C.c_int64
 _f64 = C.c_double
@@ -491,3 +493,42 @@ def load_reference(production: bool = False):
     if not os.path.exists(path):
         return None
     return Reference(path)
+
+
+class RefTileDriver:
+    """The CPU baseline's tile loop (oracle/ref_tile_driver.c): the
+    reference's production-built kernels over tiles on OpenMP threads."""
+
+    def __init__(self, path: str = REF_DRIVER_SO):
+        self.lib = C.CDLL(path)
+        f = self.lib.pxrref_time_tiles
+        f.restype = C.c_int
+        f.argtypes = [_i64, C.c_double, _i64, _i64, _i64, C.POINTER(C.c_void_p), C.c_void_p,
+                      C.c_void_p, C.c_void_p, C.c_double, C.POINTER(C.c_double),
+                      C.POINTER(C.c_double)]
+
+    def time_tiles(self, nthreads, seconds, nc, ng, fields, pool, q, m, dt):
+        """fields: six Fortran-ordered (nc+1+2ng)^3 blocks; pool: float64
+        array (nthreads, 2, 8, ntile); q, m: the two species.  Returns
+        (particle-steps, wall seconds)."""
+        pool = np.ascontiguousarray(pool, dtype=np.float64)
+        assert pool.ndim == 4 and pool.shape[:3] == (nthreads, 2, 8)
+        for f in fields:
+            assert f.flags.f_contiguous and f.dtype == np.float64 and f.shape == (nc + 1 + 2 * ng,) * 3
+        fp = (C.c_void_p * 6)(*[f.ctypes.data for f in fields])
+        qa = np.ascontiguousarray(q, dtype=np.float64)
+        ma = np.ascontiguousarray(m, dtype=np.float64)
+        steps, wall = C.c_double(0), C.c_double(0)
+        rc = self.lib.pxrref_time_tiles(nthreads, seconds, nc, ng, pool.shape[3], fp,
+                                        pool.ctypes.data, qa.ctypes.data, ma.ctypes.data, dt,
+                                        C.byref(steps), C.byref(wall))
+        if rc != 0:
+            raise RuntimeError("pxrref_time_tiles rejected its arguments")
+        return steps.value, wall.value
+
+
+def load_ref_driver():
+    """The CPU baseline's OpenMP tile loop, or None where it was not built."""
+    if not os.path.exists(REF_DRIVER_SO):
+        return None
+    return RefTileDriver(REF_DRIVER_SO)
