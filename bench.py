@@ -665,7 +665,8 @@ def main():
     tuning = {}
     for k in ("fused.lds_bytes", "fused.wg_per_cu", "fused.spread", "fused.chunk",
               "fused.j_lds_bytes", "fused.jpad", "fused.margin_pct", "fused.sstride",
-              "mixed.jacc", "mixed.lds_bytes", "mixed.wg_per_cu", "last.wg_per_cu",
+              "mixed.jacc", "fused.jacc", "mixed.lds_bytes", "mixed.wg_per_cu",
+              "last.wg_per_cu",
               "last.lds_bytes", "rebin.stable"):
         v = ctypes.c_int64(0)
         _lib.call("pxrhip_get_tuning", k.encode(), ctypes.byref(v))

@@ -510,10 +510,12 @@ __global__ __launch_bounds__(256) void k_rb_scatter(ScatterArgs a, i64 cap) {
 // 1. k_rb_slots: every run of equal bins on neighbouring lanes takes its
 //    slots with one atomic and writes its source indices there, in order;
 //    absorbed particles (key -1) take none.  Inside a bin the runs sit in
-//    the order their atomics arrived.
+//    the order their atomics arrived.  A run that is not the first to
+//    arrive in its bin (its slot is past the bin's start) flags the bin.
 __global__ __launch_bounds__(256) void k_rb_slots(const int *key, const int64_t *src_off,
                                                   int ntiles, unsigned long long *cursor,
-                                                  int *perm, i64 cap) {
+                                                  const i64 *bin_start, int *flag, int *perm,
+                                                  i64 cap) {
   const i64 i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   const i64 n = src_off[ntiles];
   if (i >= n || i >= cap) return;
@@ -522,21 +524,38 @@ __global__ __launch_bounds__(256) void k_rb_slots(const int *key, const int64_t 
   int head, cnt;
   wave_runs(k, head, cnt);
   unsigned long long slot = 0;
-  if (cnt) slot = atomicAdd(cursor + k, (unsigned long long)cnt);
+  if (cnt) {
+    slot = atomicAdd(cursor + k, (unsigned long long)cnt);
+    if (slot != (unsigned long long)bin_start[k]) flag[k] = 1;
+  }
   slot = __shfl(slot, head);
   perm[(i64)slot + (__lane_id() - head)] = (int)i;
 }
 
-// 2. k_rb_order: one thread per bin sorts the source indices of its slots
-//    (insertion sort: the slots hold a few increasing runs, most bins one),
-//    so that a bin lists its particles in source order, as the reference's
-//    stable counting sort does (sorting.F90:333-343), whatever order the
-//    runs arrived in.
-__global__ __launch_bounds__(256) void k_rb_order(const i64 *bin_start, i64 nbins, int *perm) {
-  const i64 b = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nbins) return;
+// Source indices in bin b's slots put in increasing order.  Up to 16 are
+// loaded at once and sorted in registers by a fixed exchange network (no
+// chain of dependent loads); longer bins take an insertion sort in memory.
+PXR_DEV void order_bin(const i64 *bin_start, int b, int *perm) {
   const i64 s = bin_start[b], e = bin_start[b + 1];
-  if (e - s < 2) return;
+  const i64 n = e - s;
+  if (n < 2) return;
+  if (n <= 16) {
+    int v[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) v[j] = j < n ? perm[s + j] : INT_MAX;
+#pragma unroll
+    for (int i = 1; i < 16; i++)
+#pragma unroll
+      for (int j = i; j > 0; j--) {
+        const int lo = min(v[j - 1], v[j]), hi = max(v[j - 1], v[j]);
+        v[j - 1] = lo;
+        v[j] = hi;
+      }
+#pragma unroll
+    for (int j = 0; j < 16; j++)
+      if (j < n) perm[s + j] = v[j];
+    return;
+  }
   int prev = perm[s];
   for (i64 j = s + 1; j < e; j++) {
     const int v = perm[j];
@@ -551,6 +570,18 @@ __global__ __launch_bounds__(256) void k_rb_order(const i64 *bin_start, i64 nbin
     }
     perm[k + 1] = v;
   }
+}
+
+// 2. k_rb_order: one thread per flagged bin sorts the source indices of its
+//    slots (insertion sort: the slots hold a few increasing runs), so that
+//    a bin lists its particles in source order, as the reference's stable
+//    counting sort does (sorting.F90:333-343), whatever order the runs
+//    arrived in.  Bins filled by one run are in order already.
+__global__ __launch_bounds__(256) void k_rb_order(const i64 *bin_start, i64 nbins,
+                                                  const int *flag, int *perm) {
+  const i64 b = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbins || !flag[b]) return;
+  order_bin(bin_start, (int)b, perm);
 }
 
 struct GatherArgs {
@@ -802,9 +833,12 @@ struct Tuning {
   int margin_pct = 0;     // motion margin of the current block, in 1/100 cell
   int jpad = 0;           // odd row/plane pitch of the LDS current block
   int sstride = 0;        // largest lane stride of the deposition (0: whole work item)
-  int jacc = 2;           // mixed precision LDS current: 0 FP32, 1 compensated FP32, 2 FP64,
-                          // 3 fixed point
-  int fp64_jacc = 0;      // FP64 species' LDS current: 0 FP64, 3 fixed point
+  // LDS current block: 64-bit fixed point by default (3): its integer LDS
+  // atomics outrun FP64 ones (at 256^3, 8 ppc the fused launch takes 14.0
+  // against 19.0 ms in mixed precision, 18.6 against 22.7 ms in FP64) and
+  // its sums are exact and order-independent
+  int jacc = 3;           // mixed precision: 0 FP32, 1 compensated FP32, 2 FP64, 3 fixed point
+  int fp64_jacc = 3;      // FP64 species: 0 FP64, 3 fixed point
   int stable_rebin = 1;   // re-binning keeps the source order inside a bin (0: arrival order)
   // launch of a mixed-precision species: its kernels take fewer registers,
   // so three workgroups with smaller LDS blocks fit on a CU
@@ -1020,6 +1054,7 @@ int pxrhip_set_tuning(const char *key, int64_t value) {
     g_tune.jpad = value != 0;
   } else if (!strcmp(key, "rebin.stable")) {
     g_tune.stable_rebin = value != 0;
+
 
   } else if (!strcmp(key, "fused.margin_pct")) {
     if (value < 0 || value > 200) return set_error(PXRHIP_ERR_ARG, "fused.margin_pct must be in [0, 200]");
@@ -1244,11 +1279,14 @@ int pxrhip_tiles_rebin_sort(const pxrhip_grid *grid, pxrhip_species *src, pxrhip
   sa.cursor = cursor;
   if (g_tune.stable_rebin) {
     int *perm = (int *)scratch(16, sizeof(int) * (size_t)(cap > 1 ? cap : 1));
-    if (!perm) return set_error(PXRHIP_ERR_HIP, "scratch allocation failed");
+    int *flag = (int *)scratch(18, sizeof(int) * (size_t)nbins);  // bins to order
+    if (!perm || !flag) return set_error(PXRHIP_ERR_HIP, "scratch allocation failed");
+    hipLaunchKernelGGL(k_fill_i32_t, dim3(grid1d(nbins, 256)), dim3(256), 0, st, flag, nbins);
     hipLaunchKernelGGL(k_rb_slots, dim3(grid1d(cap, 256)), dim3(256), 0, st, (const int *)key,
-                       (const int64_t *)src->tile_off, tt->ntiles, cursor, perm, cap);
+                       (const int64_t *)src->tile_off, tt->ntiles, cursor,
+                       (const i64 *)dst->bin_start, flag, perm, cap);
     hipLaunchKernelGGL(k_rb_order, dim3(grid1d(nbins, 256)), dim3(256), 0, st,
-                       (const i64 *)dst->bin_start, nbins, perm);
+                       (const i64 *)dst->bin_start, nbins, flag, perm);
     GatherArgs ga;
     for (int k = 0; k < 3; k++) {
       ga.src[k] = S[k];

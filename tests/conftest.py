@@ -32,3 +32,17 @@ def golden():
 def ggrid():
     g = dict(np.load(os.path.join(GOLDEN, "grid.npz")))
     return {k: (int(v) if k in ("nx", "ny", "nz", "ng") else float(v)) for k, v in g.items()}
+
+
+@pytest.fixture(scope="session", autouse=True)
+def _tuning_from_env():
+    """PXR_TEST_TUNE="key=value,key=value" runs the tests with those launch
+    parameters of the tiled kernels (pxrhip_set_tuning), e.g. to cover a
+    non-default form of the fused step."""
+    spec = os.environ.get("PXR_TEST_TUNE", "")
+    if spec:
+        from picsar_amd import _lib
+        for kv in spec.split(","):
+            k, v = kv.split("=")
+            _lib.call("pxrhip_set_tuning", k.strip().encode(), int(v))
+    yield

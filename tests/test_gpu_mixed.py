@@ -68,9 +68,10 @@ def unique_float32_weights(ref):
     (3, False, "boris", "esirkepov"),
     (3, True, "vay", "direct"),
 ])
-@pytest.mark.parametrize("jacc", [2, 1, 0])
+@pytest.mark.parametrize("jacc", [3, 2, 1, 0])
 def test_one_step_against_fp64_oracle(torch, orc, order, lo, pusher, depo, jacc):
-    """jacc: the LDS current in FP64 (2), compensated FP32 (1), FP32 (0)."""
+    """jacc: the LDS current in fixed point (3, the default), FP64 (2),
+    compensated FP32 (1), FP32 (0)."""
     old = set_tuning("mixed.jacc", jacc)
     try:
         sim, ref, _ = mixed_build(torch, orc, order, lo, pusher, depo, ppc=2)
