@@ -334,7 +334,7 @@ def timed_steps(sim, args, label):
     bytes_launch = float(np.mean([sim.species[i].npart * bpp + ncells * BYTES_PER_CELL
                                   for i in pushed]))
     per_launch_ms = float(np.mean(lms))
-    N = sim.total_particles()
+    N = sim.pushed_particles()  # frozen species are neither gathered, pushed nor deposited
     ke, fe = sim.kinetic_energy(), sim.field_energy()
     return dict(N=N, wall=wall, gpu_ms=total_ms, K=K, stage_ms={k: v / K for k, v in stage.items()},
                 per_launch_ms=per_launch_ms, bytes_launch=bytes_launch,
@@ -619,7 +619,8 @@ def main():
                                 if compare else [])
     sims = build_workloads(args, precs)
     main_sim = sims[args.precision]
-    log(f"workload ready: {main_sim.total_particles()} particles, "
+    log(f"workload ready: {main_sim.total_particles()} particles "
+        f"({main_sim.pushed_particles()} pushed), "
         f"{main_sim.nx}x{main_sim.ny}x{main_sim.nz} cells, precisions {precs}")
     tuned = None
     if args.autotune:
@@ -692,7 +693,8 @@ def main():
         "dtype": "f64" if args.precision == "fp64" else "f32 (gather, push, per-particle "
                                                           "current) + f64 (positions, J, FDTD)",
         "data": "synthetic (numpy seed 12345, resident on the GPU before timing)",
-        "config": {"workload": workload_name(args), "particles": r["N"], "cells": cells,
+        "config": {"workload": workload_name(args), "particles": r["N"],
+                   "particles_frozen": main_sim.total_particles() - r["N"], "cells": cells,
                    "tuning": tuning, "autotuned": tuned is not None,
                    "tiles_autotuned": tiles_tuned,
                    "step": "HIP graph of two steps, replayed" if graph else "eager"},

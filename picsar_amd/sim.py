@@ -562,6 +562,10 @@ class Simulation:
     def total_particles(self) -> int:
         return sum(sp.npart for sp in self.species)
 
+    def pushed_particles(self) -> int:
+        """Particles that go through gather + push (frozen species excluded)."""
+        return sum(sp.npart for sp in self.species if not sp.lfreeze)
+
     # ------------------------------------------------------------------
     # charge density and Gauss's law (charge_deposition_manager.F90:63-743,
     # field_boundaries.F90:1910-1930, diags.F90:63-197, 1222-1345)
