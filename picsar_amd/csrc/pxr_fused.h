@@ -126,6 +126,7 @@ struct FusedArgs {
   int ovf_cap;
   int phase;             // PHASE_PUSH | PHASE_DEPOSIT
   double fxb;            // bound of |current contribution| per unit |q w| (fixed point)
+  const int *order;      // work items in the order the queue hands them out (null: as built)
 };
 
 // phases of k_fused: gather + push + position push, current deposition
@@ -488,7 +489,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(
   const int nch = *a.nchunks;
   for (;;) {
     if (threadIdx.x == 0) {
-      s_chunk = atomicAdd(a.queue, 1);
+      const int q = atomicAdd(a.queue, 1);
+      s_chunk = (a.order && q < nch) ? a.order[q] : q;
       for (int d = 0; d < 3; d++) {
         s_box[d] = INT_MAX;
         s_box[3 + d] = INT_MIN;
