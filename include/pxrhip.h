@@ -316,6 +316,8 @@ typedef struct {
                                        reversed; partcom 0: periodic */
 #define PXRHIP_PBOUND_REINJECTING 3 /* mirrored in, at rest, kept in its tile */
 
+#define PXRHIP_MAX_PID_EXTRA 4
+
 typedef struct {
   double *x, *y, *z, *ux, *uy, *uz, *gaminv, *w; /* SoA particle arrays */
   int64_t *tile_off;  /* device, ntiles+1 entries: packed tile offsets, last = total */
@@ -330,6 +332,14 @@ typedef struct {
    * pointers are cast); gather, push and the LDS current are FP32 and the
    * global current is FP64. */
   int64_t precision;
+  /* Columns 2 .. npid_extra+1 of the reference's pid array (particle_tile%pid,
+   * modules.F90:339-452 with npid, ssnpid, xoldpid at :608-617: particle
+   * ids, old positions ...), float64 whatever the precision, capacity
+   * values each.  The re-binning carries them with their particles (as
+   * particle_bcs_tiles does); no kernel reads them.  Column 1, the weight,
+   * is w.  npid_extra is 0 .. PXRHIP_MAX_PID_EXTRA. */
+  int64_t npid_extra;
+  double *pid_extra[PXRHIP_MAX_PID_EXTRA];
 } pxrhip_species;
 
 /* Laser antenna particles: laserp_pusher_gaussian

@@ -475,6 +475,9 @@ struct ScatterArgs {
   const void *srcq[5];   // ux, uy, uz, gaminv, w: FP64, or FP32 when mixed
   void *dstq[5];
   int mixed;
+  int npe;               // extra pid columns (FP64) carried along
+  const double *srcp[PXRHIP_MAX_PID_EXTRA];
+  double *dstp[PXRHIP_MAX_PID_EXTRA];
   const int64_t *src_off;
   int ntiles;
   const int *key;
@@ -504,6 +507,7 @@ __global__ __launch_bounds__(256) void k_rb_scatter(ScatterArgs a, i64 cap) {
 #pragma unroll
     for (int c = 0; c < 5; c++) ((double *)a.dstq[c])[d] = ((const double *)a.srcq[c])[i];
   }
+  for (int c = 0; c < a.npe; c++) a.dstp[c][d] = a.srcp[c][i];
 }
 
 // Stable counting sort, in three passes after the scan.
@@ -590,6 +594,9 @@ struct GatherArgs {
   const void *srcq[5];   // ux, uy, uz, gaminv, w: FP64, or FP32 when mixed
   void *dstq[5];
   int mixed;
+  int npe;               // extra pid columns (FP64) carried along
+  const double *srcp[PXRHIP_MAX_PID_EXTRA];
+  double *dstp[PXRHIP_MAX_PID_EXTRA];
   const int *perm;       // source index of every destination slot
   const i64 *total;      // number of particles kept
 };
@@ -609,6 +616,7 @@ __global__ __launch_bounds__(256) void k_rb_gather(GatherArgs a, i64 cap) {
 #pragma unroll
     for (int c = 0; c < 5; c++) ((double *)a.dstq[c])[j] = ((const double *)a.srcq[c])[i];
   }
+  for (int c = 0; c < a.npe; c++) a.dstp[c][j] = a.srcp[c][i];
 }
 
 __global__ __launch_bounds__(256) void k_rb_tiles(int ntiles, const int64_t *bin_base,
@@ -1162,6 +1170,10 @@ static int check_species(const pxrhip_species *s, bool need_bins) {
     return set_error(PXRHIP_ERR_ARG, "capacity must be in [0, 2^31)");
   if (s->precision != 0 && s->precision != 1)
     return set_error(PXRHIP_ERR_ARG, "precision must be 0 (FP64) or 1 (mixed)");
+  if (s->npid_extra < 0 || s->npid_extra > PXRHIP_MAX_PID_EXTRA)
+    return set_error(PXRHIP_ERR_ARG, "npid_extra must be in [0, %d]", PXRHIP_MAX_PID_EXTRA);
+  for (int c = 0; c < s->npid_extra; c++)
+    if (!s->pid_extra[c]) return set_error(PXRHIP_ERR_ARG, "null pid column %d", c + 2);
   return PXRHIP_OK;
 }
 
@@ -1268,6 +1280,8 @@ int pxrhip_tiles_rebin_sort(const pxrhip_grid *grid, pxrhip_species *src, pxrhip
   if ((r = check_species(src, false)) || (r = check_species(dst, true))) return r;
   if (dst->precision != src->precision)
     return set_error(PXRHIP_ERR_ARG, "source and destination species differ in precision");
+  if (dst->npid_extra != src->npid_extra)
+    return set_error(PXRHIP_ERR_ARG, "source and destination species differ in pid columns");
   if (dst->capacity < src->capacity)
     return set_error(PXRHIP_ERR_CAPACITY, "dst capacity smaller than src capacity");
   const i64 cap = src->capacity;
@@ -1313,6 +1327,11 @@ int pxrhip_tiles_rebin_sort(const pxrhip_grid *grid, pxrhip_species *src, pxrhip
     sa.dstq[k] = DQ[k];
   }
   sa.mixed = src->precision == 1;
+  sa.npe = (int)src->npid_extra;
+  for (int c = 0; c < PXRHIP_MAX_PID_EXTRA; c++) {
+    sa.srcp[c] = c < sa.npe ? src->pid_extra[c] : nullptr;
+    sa.dstp[c] = c < sa.npe ? dst->pid_extra[c] : nullptr;
+  }
   sa.src_off = src->tile_off;
   sa.ntiles = tt->ntiles;
   sa.key = key;
@@ -1337,6 +1356,11 @@ int pxrhip_tiles_rebin_sort(const pxrhip_grid *grid, pxrhip_species *src, pxrhip
       ga.dstq[k] = DQ[k];
     }
     ga.mixed = sa.mixed;
+    ga.npe = sa.npe;
+    for (int c = 0; c < PXRHIP_MAX_PID_EXTRA; c++) {
+      ga.srcp[c] = sa.srcp[c];
+      ga.dstp[c] = sa.dstp[c];
+    }
     ga.perm = perm;
     ga.total = (const i64 *)dst->bin_start + nbins;
     hipLaunchKernelGGL(k_rb_gather, dim3(grid1d(cap, 256)), dim3(256), 0, st, ga, cap);

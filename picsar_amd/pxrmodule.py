@@ -208,10 +208,13 @@ class PicsarModule:
         (pid of shape (npart, npiid), column 1 the weight); the older form
         x, y, z, ux, uy, uz, gaminv, w is accepted too."""
         n = int(npart)
+        extra = None
         if len(args) == 9:
             npiid, x, y, z, ux, uy, uz, gaminv, pid = args
             pid = np.asarray(pid, dtype=np.float64).reshape(n, int(npiid), order="F")
             w = pid[:, 0]
+            if int(npiid) > 1:  # columns 2.. (ids, old positions ...) travel with the particles
+                extra = np.ascontiguousarray(pid[:, 1:])
         elif len(args) == 8:
             x, y, z, ux, uy, uz, gaminv, w = args
         else:
@@ -223,7 +226,14 @@ class PicsarModule:
         for k, v in arrs.items():
             if v.size != n:
                 raise ValueError(f"{k}: {v.size} values, expected {n}")
-        self._staged.setdefault(int(nsp), []).append(arrs)
+        if extra is not None:
+            arrs["pid"] = extra
+        staged = self._staged.setdefault(int(nsp), [])
+        if staged and (("pid" in staged[0]) != ("pid" in arrs) or
+                       ("pid" in arrs and staged[0]["pid"].shape[1] != arrs["pid"].shape[1])):
+            raise ValueError("every call for one species must pass the same number of pid "
+                             "columns")
+        staged.append(arrs)
 
     def init_stencil_coefficients(self):
         """submain.F90:890-908: FD_weights of the FDTD orders."""
@@ -260,15 +270,27 @@ class PicsarModule:
             parts = []
             idx = self._species_index(nsp)
             if idx is not None:  # species already on the device: append
-                sp = sim.species.pop(idx)
-                parts.append({k: v.astype(np.float64) for k, v in sp.arrays().items()})
-                self._ready.discard(nsp)
+                sp = sim.species[idx]
+                a = sp.arrays()
+                old = {k: v.astype(np.float64) for k, v in a.items() if not k.startswith("pid")}
+                if sp.npid_extra:
+                    old["pid"] = np.stack([a[f"pid{c + 2}"] for c in range(sp.npid_extra)],
+                                          axis=1)
+                parts.append(old)
             parts.extend(new)
+            if any(set(p) != set(parts[0]) or
+                   ("pid" in p and p["pid"].shape[1] != parts[0]["pid"].shape[1])
+                   for p in parts):
+                raise ValueError(f"species {nsp}: particles added with different numbers of "
+                                 "pid columns")
+            if idx is not None:
+                sim.species.pop(idx)
+                self._ready.discard(nsp)
             cat = {k: np.concatenate([p[k] for p in parts]) for k in parts[0]}
             sp = sim.add_species(props["name"], props["charge"], props["mass"], cat["x"],
                                  cat["y"], cat["z"], cat["ux"], cat["uy"], cat["uz"], cat["w"],
                                  gaminv=cat["gaminv"], lfreeze=props["lfreeze"],
-                                 ldodepos=props["ldodepos"])
+                                 ldodepos=props["ldodepos"], pid=cat.get("pid"))
             sp.pxr_index = nsp
             self._ready.add(nsp)
             self._staged[nsp] = []

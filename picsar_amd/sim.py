@@ -53,11 +53,15 @@ class PxrGrid(C.Structure):
                [("pbound", C.c_int64 * 6), ("partcom", C.c_int64)]
 
 
+MAX_PID_EXTRA = 4  # PXRHIP_MAX_PID_EXTRA
+
+
 class PxrSpecies(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in ("x", "y", "z", "ux", "uy", "uz", "gaminv", "w",
                                           "tile_off", "tile_np", "bin_start")] + \
                [("capacity", C.c_int64), ("q", C.c_double), ("m", C.c_double),
-                ("precision", C.c_int64)]
+                ("precision", C.c_int64), ("npid_extra", C.c_int64),
+                ("pid_extra", C.c_void_p * MAX_PID_EXTRA)]
 
 
 PUSHERS = {"boris": 0, "vay": 1}
@@ -70,9 +74,10 @@ PRECISIONS = {"fp64": 0, "mixed": 1}
 
 
 def component_dtype(name: str, precision: str):
-    """Storage type of a particle component: positions are always float64;
-    in mixed precision momenta, gaminv and weights are float32."""
-    if precision == "mixed" and name not in ("x", "y", "z"):
+    """Storage type of a particle component: positions and pid columns are
+    always float64; in mixed precision momenta, gaminv and weights are
+    float32."""
+    if precision == "mixed" and name not in ("x", "y", "z") and not name.startswith("pid"):
         return np.float32
     return np.float64
 
@@ -99,6 +104,7 @@ class Species:
     lfreeze: bool = False   # particle_species%lfreeze: not pushed
     ldodepos: bool = True   # particle_species%ldodepos: deposits current
     precision: str = "fp64"
+    npid_extra: int = 0     # pid columns beyond the weight, arrays "pid2", "pid3", ...
 
     @property
     def active(self) -> _Buffers:
@@ -113,6 +119,9 @@ class Species:
         s = PxrSpecies()
         for k in COMPONENTS:
             setattr(s, k, b.arrays[k].ptr)
+        s.npid_extra = self.npid_extra
+        for c in range(self.npid_extra):
+            s.pid_extra[c] = b.arrays[f"pid{c + 2}"].ptr
         s.tile_off = b.tile_off.ptr
         s.tile_np = b.tile_np.ptr
         s.bin_start = b.bin_start.ptr
@@ -220,30 +229,44 @@ class Simulation:
     # particles
     # ------------------------------------------------------------------
     def add_species(self, name, q, m, x, y, z, ux, uy, uz, w, gaminv=None, capacity=None,
-                    lfreeze=False, ldodepos=True) -> Species:
+                    lfreeze=False, ldodepos=True, pid=None) -> Species:
         """Add a species from host arrays.  gaminv defaults to
-        1/sqrt(1+u^2) (pxr_set_gamma)."""
+        1/sqrt(1+u^2) (pxr_set_gamma).  pid: the reference's pid columns
+        beyond the weight, an (npart, k) array (k <= 4; particle ids, old
+        positions ...), carried with their particles."""
         src = {k: np.asarray(v, dtype=np.float64).ravel()
                for k, v in zip(("x", "y", "z", "ux", "uy", "uz", "w"), (x, y, z, ux, uy, uz, w))}
         if gaminv is not None:
             src["gaminv"] = np.asarray(gaminv, dtype=np.float64).ravel()
+        npid_extra = 0
+        if pid is not None:
+            pid = np.asarray(pid, dtype=np.float64).reshape(len(src["x"]), -1)
+            src["pid"] = pid
+            npid_extra = pid.shape[1]
         return self.add_species_chunks(name, q, m, len(src["x"]), [src], capacity=capacity,
-                                       lfreeze=lfreeze, ldodepos=ldodepos)
+                                       lfreeze=lfreeze, ldodepos=ldodepos, npid_extra=npid_extra)
 
     def add_species_chunks(self, name, q, m, npart, chunks, capacity=None, lfreeze=False,
-                           ldodepos=True) -> Species:
+                           ldodepos=True, npid_extra=0) -> Species:
         """Add a species of ``npart`` particles given as an iterable of dicts
-        of host arrays (x, y, z, ux, uy, uz, w and optionally gaminv), each
-        holding the next run of particles; large species can so be built
-        slab by slab without holding every array in host memory at once."""
+        of host arrays (x, y, z, ux, uy, uz, w and optionally gaminv and, with
+        npid_extra > 0, pid: an (n, npid_extra) array of the pid columns beyond
+        the weight), each holding the next run of particles; large species
+        can so be built slab by slab without holding every array in host
+        memory at once."""
         n = int(npart)
         cap = int(capacity) if capacity is not None else max(n, 1)
         if cap < n:
             raise ValueError("capacity < number of particles")
+        npid_extra = int(npid_extra)
+        if not 0 <= npid_extra <= MAX_PID_EXTRA:
+            raise ValueError(f"at most {MAX_PID_EXTRA} pid columns beyond the weight")
         sp = Species(name=name, q=float(q), m=float(m), capacity=cap, npart=n,
-                     lfreeze=lfreeze, ldodepos=ldodepos, precision=self.precision)
+                     lfreeze=lfreeze, ldodepos=ldodepos, precision=self.precision,
+                     npid_extra=npid_extra)
+        comps = COMPONENTS + tuple(f"pid{c + 2}" for c in range(npid_extra))
         for _ in range(2):
-            arrays = {k: DeviceArray(cap, component_dtype(k, self.precision)) for k in COMPONENTS}
+            arrays = {k: DeviceArray(cap, component_dtype(k, self.precision)) for k in comps}
             b = _Buffers(arrays=arrays, tile_off=DeviceArray(self.ntiles + 1, np.int64),
                          tile_np=DeviceArray(self.ntiles, np.int64),
                          bin_start=DeviceArray(self.nbins + 1, np.int64))
@@ -263,12 +286,16 @@ class Simulation:
                 src["gaminv"] = np.ascontiguousarray(np.asarray(ch["gaminv"],
                                                                 dtype=np.float64).ravel())
             k0 = len(src["x"])
+            if npid_extra:
+                pid = np.asarray(ch["pid"], dtype=np.float64).reshape(k0, npid_extra)
+                for c in range(npid_extra):
+                    src[f"pid{c + 2}"] = np.ascontiguousarray(pid[:, c])
             for k, v in src.items():
                 if v.size != k0:
                     raise ValueError(f"{k}: {v.size} values, expected {k0}")
             if done + k0 > n:
                 raise ValueError(f"the chunks hold more than {n} particles")
-            for k in COMPONENTS:
+            for k in comps:
                 sp.bufs[1].arrays[k].upload(src[k], offset=done)
             done += k0
         if done != n:

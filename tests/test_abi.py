@@ -74,3 +74,35 @@ def test_host_argument_checks_raise_before_device_work(lib_path):
     with pytest.raises(ValueError):
         picsar.pxrpush_em3d_evec_norder(*[np.zeros((5, 5, 5), order="F")] * 9, 0.1, [1.0],
                                         [1.0], [1.0], 2, 2, 2, 4, 4, 4, 1, 1, 1, 0, 0, 0)
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """The ctypes mirrors of pxrhip_grid and pxrhip_species have the C
+    layout: a C program compiled against include/pxrhip.h prints the sizes
+    and field offsets."""
+    import ctypes as C
+    import shutil
+    import subprocess
+    from picsar_amd.sim import PxrGrid, PxrSpecies
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    src = tmp_path / "layout.c"
+    src.write_text(
+        '#include <stddef.h>\n#include <stdio.h>\n#include "pxrhip.h"\n'
+        "int main(void) {\n"
+        '  printf("%zu %zu %zu %zu\\n", sizeof(pxrhip_grid), offsetof(pxrhip_grid, pbound),\n'
+        "         offsetof(pxrhip_grid, partcom), sizeof(pxrhip_species));\n"
+        '  printf("%zu %zu %zu %zu\\n", offsetof(pxrhip_species, capacity),\n'
+        "         offsetof(pxrhip_species, precision), offsetof(pxrhip_species, npid_extra),\n"
+        "         offsetof(pxrhip_species, pid_extra));\n"
+        "  return 0;\n}\n")
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
+    got = [int(v) for v in out]
+    want = [C.sizeof(PxrGrid), PxrGrid.pbound.offset, PxrGrid.partcom.offset, C.sizeof(PxrSpecies),
+            PxrSpecies.capacity.offset, PxrSpecies.precision.offset,
+            PxrSpecies.npid_extra.offset, PxrSpecies.pid_extra.offset]
+    assert got == want

@@ -72,13 +72,22 @@ def test_species_properties_and_staging():
     assert p.nspecies == 2
     assert p._species_props[1]["ldodepos"] and not p._species_props[2]["ldodepos"]
     x = np.arange(3.0)
-    pid = np.stack([np.full(3, 2.0), np.zeros(3)], axis=1)  # weight, a second column
+    pid = np.stack([np.full(3, 2.0), np.arange(3.0) + 10], axis=1)  # weight, particle ids
     p.py_add_particles_to_species(1, 3, 2, x, x, x, x, x, x, np.ones(3), pid)
-    p.py_add_particles_to_species(1, 3, x, x, x, x, x, x, np.ones(3), np.full(3, 5.0))
+    pid2 = np.stack([np.full(3, 5.0), np.arange(3.0) + 20], axis=1)
+    p.py_add_particles_to_species(1, 3, 2, x, x, x, x, x, x, np.ones(3), pid2)
     st = p._staged[1]
     assert len(st) == 2
     np.testing.assert_array_equal(st[0]["w"], 2.0)
     np.testing.assert_array_equal(st[1]["w"], 5.0)
+    # the columns beyond the weight are kept, to travel with their particles
+    np.testing.assert_array_equal(st[1]["pid"][:, 0], [20.0, 21.0, 22.0])
+    # one species keeps one number of pid columns
+    with pytest.raises(ValueError):
+        p.py_add_particles_to_species(1, 3, x, x, x, x, x, x, np.ones(3), np.full(3, 5.0))
+    # the older form (weights only) on a species of its own
+    p.py_add_particles_to_species(2, 3, x, x, x, x, x, x, np.ones(3), np.full(3, 7.0))
+    assert "pid" not in p._staged[2][0]
     with pytest.raises(TypeError):
         p.py_add_particles_to_species(1, 3, x, x)
 
