@@ -132,10 +132,15 @@ int pxrhip_get_loc_norm_2(const double *array, int64_t nx2, int64_t ny2, int64_t
 
 /* ---- field gathering ---------------------------------------------------- */
 /* geteb3d_energy_conserving, field_gathering_manager_3d.F90:295-351.
- * nox = noy = noz in {1,2,3}; adds the Yee-staggered E and B at each
- * particle into ex..bz.  field_gathe_algo, lvect and ll4symtry select CPU
- * code variants in the reference and are accepted but do not change the
- * result.  l_nodal is fields%l_nodalgrid (modules.F90). */
+ * Adds the Yee-staggered E and B at each particle into ex..bz.  Shape
+ * orders 1..3 on each axis (others: PXRHIP_ERR_ARG, the reference has no
+ * weights for them).  As geteb3d_energy_conserving_generic (:397-700)
+ * dispatches: field_gathe_algo = 2, or orders other than 1_1_1, 2_2_2,
+ * 3_3_3, run pxr_gete3d_n / pxr_getb3d_n_energy_conserving
+ * (field_gathering_on_3d.F90:1042, 697), with their single-precision cubic
+ * constants and ll4symtry; otherwise the order-N routines (lvect and the
+ * other algorithms choose CPU code variants of the same sums).  l_nodal is
+ * fields%l_nodalgrid (modules.F90). */
 int pxrhip_geteb3d_energy_conserving(
     int64_t np, const double *xp, const double *yp, const double *zp,
     double *ex, double *ey, double *ez, double *bx, double *by, double *bz,
@@ -169,9 +174,12 @@ int pxrhip_set_gamma(int64_t np, const double *uxp, const double *uyp,
 /* ---- current deposition ------------------------------------------------- */
 /* depose_jxjyjz, current_deposition_manager_3d.F90:47-67 ->
  * depose_jxjyjz_generic (:106-222).  current_depo_algo: 0 Esirkepov
- * (esirkepov_3d.F90:82/1734/3263), 1 arbitrary-order Esirkepov (orders 1..3
- * here), 2 and 3 direct (direct_current_deposition_3d.F90:99/879/1992).
- * nox = noy = noz in {1,2,3}.  Adds into jx, jy, jz. */
+ * (esirkepov_3d.F90:82/1734/3263), 1 the generic Esirkepov
+ * pxr_depose_jxjyjz_esirkepov_n (:3480), 2 and 3 direct
+ * (direct_current_deposition_3d.F90:99/879/1992).  Shape orders 0..3 on
+ * each axis (others: PXRHIP_ERR_ARG); orders other than 1_1_1, 2_2_2, 3_3_3
+ * run the generic Esirkepov whatever the algorithm, as in the reference.
+ * Adds into jx, jy, jz. */
 int pxrhip_depose_jxjyjz(double *jx, double *jy, double *jz, int64_t np,
                          const double *xp, const double *yp, const double *zp,
                          const double *uxp, const double *uyp,
@@ -183,10 +191,10 @@ int pxrhip_depose_jxjyjz(double *jx, double *jy, double *jz, int64_t np,
                          int64_t nox, int64_t noy, int64_t noz,
                          int64_t current_depo_algo, int64_t l_nodal);
 
-/* pxr_depose_rho_n, charge_deposition_3d.F90:2081-2228: charge density of
- * order nox = noy = noz in {1,2,3} at the nodes (floor for odd, nint for
- * even orders), ADDED into rho.  l_particles_weight = 0 uses w[0] for every
- * particle; l4symtry folds x and y to their absolute values. */
+/* pxr_depose_rho_n, charge_deposition_3d.F90:2081-2228: charge density
+ * with shape orders nox, noy, noz in 0..3 at the nodes (floor for odd, nint
+ * for even orders), ADDED into rho.  l_particles_weight = 0 uses w[0] for
+ * every particle; l4symtry folds x and y to their absolute values. */
 int pxrhip_depose_rho_n(double *rho, int64_t np, const double *xp, const double *yp,
                         const double *zp, const double *w, double q, double xmin,
                         double ymin, double zmin, double dx, double dy, double dz,

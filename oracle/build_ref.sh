@@ -77,6 +77,7 @@ build_lib() {
   $FC $FFLAGS -c ../pxr_consts.F90 -o pxr_consts.o
   local OBJS="pxr_consts.o"
   for f in field_gathering/energy_conserving/field_gathering_o1_3d.F90 \
+           field_gathering/energy_conserving/field_gathering_on_3d.F90 \
            field_gathering/energy_conserving/field_gathering_o2_3d.F90 \
            field_gathering/energy_conserving/field_gathering_o3_3d.F90 \
            particle_pushers/boris_pusher/boris_3d.F90 \

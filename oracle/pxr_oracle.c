@@ -480,37 +480,285 @@ i64 orc_depose_esirkepov(double *jx, double *jy, double *jz, i64 np,
 }
 
 /* ------------------------------------------------------------------------
- * Charge density, pxr_depose_rho_n (charge_deposition_3d.F90:2081-2228)
- * with nox = noy = noz = n, l_particles_weight, no 4-fold symmetry: nodal
- * shape of order n at floor (n odd) / nint (n even), the node convention of
- * the Esirkepov routines above, so that rho and J share one shape (the
- * discrete continuity residual of the tests).  rho is ADDED into.      */
+ * Generic-order routines: one shape order per axis.
+ *
+ * Weights of order n (0..3) at offset f from the base node, as the
+ * generic routines write them (esirkepov_3d.F90:3588-3608,
+ * charge_deposition_3d.F90:2160-2178, field_gathering_on_3d.F90:1170-1215):
+ * order 0 is the single weight 1, order 3 the cubic with the constants
+ * given (the gather's generic routines pass their own, below).          */
+static inline void w_order(int n, double f, double *w, double sixth, double twothird) {
+  switch (n) {
+  case 0: w[0] = 1.0; break;
+  case 1: w_lin(f, w); break;
+  case 2: w_quad(f, w); break;
+  default: {
+    double o = 1.0 - f, f2 = f * f, o2 = o * o;
+    w[0] = sixth * o2 * o;
+    w[1] = twothird - f2 * (1.0 - f / 2.0);
+    w[2] = twothird - o2 * (1.0 - o / 2.0);
+    w[3] = sixth * f2 * f;
+  }
+  }
+}
+
+/* Node index of a shape of order n: nint for even orders, floor for odd
+ * ones (esirkepov_3d.F90:3560-3574, charge_deposition_3d.F90:2128-2142). */
+static inline i64 n_index(int n, double x) { return (n % 2 == 0) ? f_nint(x) : f_floor(x); }
+
+/* ------------------------------------------------------------------------
+ * Charge density, pxr_depose_rho_n (charge_deposition_3d.F90:2081-2228):
+ * orders 0..3 on each axis, l_particles_weight (else every particle has
+ * the weight w[0]) and l4symtry (|x|, |y|).  The first node of an axis is
+ * j - n/2; rho(j+a, k+b, l+c) += sx(a) sy(b) sz(c) wq with wq = q w/vol.
+ * rho is ADDED into.                                                    */
+i64 orc_depose_rho_n(double *rho, i64 np, const double *xp, const double *yp,
+                     const double *zp, const double *w, double q, double xmin,
+                     double ymin, double zmin, double dx, double dy, double dz,
+                     i64 nx, i64 ny, i64 nz, i64 ngx, i64 ngy, i64 ngz,
+                     i64 nox, i64 noy, i64 noz, i64 l_particles_weight, i64 l4symtry) {
+  if (nox < 0 || nox > 3 || noy < 0 || noy > 3 || noz < 0 || noz > 3) return -1;
+  const int no[3] = {(int)nox, (int)noy, (int)noz};
+  const double dxi = 1.0 / dx, dyi = 1.0 / dy, dzi = 1.0 / dz;
+  const double invvol = dxi * dyi * dzi;
+  grid_t g = mkgrid(nx, ny, nz, ngx, ngy, ngz);
+  i64 bad = 0;
+  for (i64 ip = 0; ip < np; ip++) {
+    double p[3] = {(xp[ip] - xmin) * dxi, (yp[ip] - ymin) * dyi, (zp[ip] - zmin) * dzi};
+    if (l4symtry) {
+      p[0] = fabs(p[0]);
+      p[1] = fabs(p[1]);
+    }
+    double s[3][4];
+    i64 first[3];
+    for (int a = 0; a < 3; a++) {
+      i64 j = n_index(no[a], p[a]);
+      w_order(no[a], p[a] - (double)j, s[a], ONESIXTH, TWOTHIRD);
+      first[a] = j - no[a] / 2;
+    }
+    if (!ginside(&g, first[0], first[1], first[2]) ||
+        !ginside(&g, first[0] + no[0], first[1] + no[1], first[2] + no[2])) {
+      bad++;
+      continue;
+    }
+    const double wq = l_particles_weight ? q * w[ip] * invvol : q * invvol * w[0];
+    for (int c = 0; c <= no[2]; c++)
+      for (int b = 0; b <= no[1]; b++)
+        for (int a = 0; a <= no[0]; a++) {
+          i64 ix = gidx(&g, first[0] + a, first[1] + b, first[2] + c);
+          rho[ix] = rho[ix] + s[0][a] * s[1][b] * s[2][c] * wq;
+        }
+  }
+  return bad;
+}
+
+/* pxr_depose_rho_n with nox = noy = noz = order (1..3), per-particle
+ * weights, no symmetry: the node convention of the Esirkepov routines
+ * above, so that rho and J share one shape (the continuity residual of
+ * the tests). */
 i64 orc_depose_rho(double *rho, i64 np, const double *xp, const double *yp,
                    const double *zp, const double *w, double q, double xmin,
                    double ymin, double zmin, double dx, double dy, double dz,
                    i64 nx, i64 ny, i64 nz, i64 ngx, i64 ngy, i64 ngz,
                    i64 order) {
   if (order < 1 || order > 3) return -1;
-  const int n = (int)order;
+  return orc_depose_rho_n(rho, np, xp, yp, zp, w, q, xmin, ymin, zmin, dx, dy, dz, nx, ny, nz,
+                          ngx, ngy, ngz, order, order, order, 1, 0);
+}
+
+/* ------------------------------------------------------------------------
+ * pxr_depose_jxjyjz_esirkepov_n (esirkepov_3d.F90:3480-3836), as
+ * depose_jxjyjz_generic calls it (l_particles_weight, no symmetry): orders
+ * 0..3 on each axis.  Differences from the fixed-order routines that this
+ * restatement keeps:
+ *   - v = u * gaminv for every order, and the new position is formed again
+ *     as xold + (dt/dx) v (the one-cell loop of :3553-3559), which may
+ *     differ from the given position in the last bit;
+ *   - the node index is nint for even orders (order 0 included).
+ * The box of an axis is [min(0,d) - n/2, max(0,d) + (n+1)/2] around the
+ * new node; particles that move more than one node are skipped (the
+ * reference's work arrays do not hold them).                            */
+i64 orc_depose_esirkepov_n(double *jx, double *jy, double *jz, i64 np,
+                           const double *xp, const double *yp, const double *zp,
+                           const double *uxp, const double *uyp, const double *uzp,
+                           const double *gaminv, const double *w, double q, double xmin,
+                           double ymin, double zmin, double dt, double dx, double dy,
+                           double dz, i64 nx, i64 ny, i64 nz, i64 ngx, i64 ngy, i64 ngz,
+                           i64 nox, i64 noy, i64 noz) {
+  if (nox < 0 || nox > 3 || noy < 0 || noy > 3 || noz < 0 || noz > 3) return -1;
+  const int no[3] = {(int)nox, (int)noy, (int)noz};
   const double dxi = 1.0 / dx, dyi = 1.0 / dy, dzi = 1.0 / dz;
-  const double invvol = dxi * dyi * dzi;
+  const double dts[3] = {dt * dxi, dt * dyi, dt * dzi};
+  const double invdtdx = 1.0 / (dt * dy * dz);
+  const double invdtdy = 1.0 / (dt * dx * dz);
+  const double invdtdz = 1.0 / (dt * dx * dy);
   grid_t g = mkgrid(nx, ny, nz, ngx, ngy, ngz);
   i64 bad = 0;
   for (i64 ip = 0; ip < np; ip++) {
-    double x = (xp[ip] - xmin) * dxi, y = (yp[ip] - ymin) * dyi, z = (zp[ip] - zmin) * dzi;
-    double sx[4], sy[4], sz[4];
-    i64 j = shape_nodal(n, x, sx), k = shape_nodal(n, y, sy), l = shape_nodal(n, z, sz);
-    if (!ginside(&g, j, k, l) || !ginside(&g, j + n, k + n, l + n)) {
+    const double p0[3] = {(xp[ip] - xmin) * dxi, (yp[ip] - ymin) * dyi, (zp[ip] - zmin) * dzi};
+    const double v[3] = {uxp[ip] * gaminv[ip], uyp[ip] * gaminv[ip], uzp[ip] * gaminv[ip]};
+    double s0[3][7], s1[3][7], ds[3][7];
+    i64 base[3];
+    int lo[3], hi[3], skip = 0;
+    for (int a = 0; a < 3; a++) {
+      const double xo = p0[a] - dts[a] * v[a];
+      const double xn = xo + dts[a] * v[a];
+      const int n = no[a], first = -(n / 2);
+      double ww[4];
+      memset(s0[a], 0, sizeof s0[a]);
+      memset(s1[a], 0, sizeof s1[a]);
+      const i64 i0 = n_index(n, xn);
+      w_order(n, xn - (double)i0, ww, ONESIXTH, TWOTHIRD);
+      for (int r = 0; r <= n; r++) s0[a][EOFF + first + r] = ww[r];
+      const i64 i1 = n_index(n, xo);
+      const int d = (int)(i1 - i0);
+      if (d < -1 || d > 1) {
+        skip = 1;
+        break;
+      }
+      w_order(n, xo - (double)i1, ww, ONESIXTH, TWOTHIRD);
+      for (int r = 0; r <= n; r++) s1[a][EOFF + first + d + r] = ww[r];
+      for (int r = 0; r < 7; r++) ds[a][r] = s1[a][r] - s0[a][r];
+      base[a] = i0;
+      lo[a] = (d < 0 ? d : 0) - n / 2;
+      hi[a] = (d > 0 ? d : 0) + (n + 1) / 2;
+    }
+    if (skip || !ginside(&g, base[0] + lo[0], base[1] + lo[1], base[2] + lo[2]) ||
+        !ginside(&g, base[0] + hi[0], base[1] + hi[1], base[2] + hi[2])) {
       bad++;
       continue;
     }
-    double wq = q * w[ip] * invvol;
-    for (int c = 0; c <= n; c++)
-      for (int b = 0; b <= n; b++)
-        for (int a = 0; a <= n; a++) {
-          i64 ix = gidx(&g, j + a, k + b, l + c);
-          rho[ix] = rho[ix] + sx[a] * sy[b] * sz[c] * wq;
+    const double wq = q * w[ip];
+    const double wqx = wq * invdtdx, wqy = wq * invdtdy, wqz = wq * invdtdz;
+    const double *sx0 = s0[0], *sy0 = s0[1], *sz0 = s0[2];
+    const double *dsx = ds[0], *dsy = ds[1], *dsz = ds[2];
+    double sdyj[7] = {0}, sdzk[7][7] = {{0}};
+    for (int k = lo[2]; k <= hi[2]; k++) {
+      for (int j = lo[1]; j <= hi[1]; j++) {
+        double sdxi = 0.0;
+        for (int i = lo[0]; i <= hi[0]; i++) {
+          i64 ic = base[0] + i, jc = base[1] + j, kc = base[2] + k;
+          const int I = i + EOFF, J = j + EOFF, K = k + EOFF;
+          if (i < hi[0]) {
+            double t = wqx * dsx[I] * ((sy0[J] + 0.5 * dsy[J]) * sz0[K] +
+                                        (0.5 * sy0[J] + ONETHIRD * dsy[J]) * dsz[K]);
+            sdxi = (i > lo[0]) ? t + sdxi : t;
+            i64 ix = gidx(&g, ic, jc, kc);
+            jx[ix] = jx[ix] + sdxi;
+          }
+          if (j < hi[1]) {
+            double t = wqy * dsy[J] * ((sz0[K] + 0.5 * dsz[K]) * sx0[I] +
+                                        (0.5 * sz0[K] + ONETHIRD * dsz[K]) * dsx[I]);
+            sdyj[I] = (j > lo[1]) ? t + sdyj[I] : t;
+            i64 ix = gidx(&g, ic, jc, kc);
+            jy[ix] = jy[ix] + sdyj[I];
+          }
+          if (k < hi[2]) {
+            double t = wqz * dsz[K] * ((sx0[I] + 0.5 * dsx[I]) * sy0[J] +
+                                        (0.5 * sx0[I] + ONETHIRD * dsx[I]) * dsy[J]);
+            sdzk[I][J] = (k > lo[2]) ? t + sdzk[I][J] : t;
+            i64 ix = gidx(&g, ic, jc, kc);
+            jz[ix] = jz[ix] + sdzk[I][J];
+          }
         }
+      }
+    }
+  }
+  return bad;
+}
+
+/* ------------------------------------------------------------------------
+ * pxr_gete3d_n_energy_conserving and pxr_getb3d_n_energy_conserving
+ * (field_gathering_on_3d.F90:1042-1365, 697-1026): orders 1..3 on each
+ * axis, with l_lower_order_in_v, l_nodal and l4symtry.  Their constants
+ * onesixth = 1./6. and twothird = 2./3. are default-real (single
+ * precision) literals widened to FP64, so the cubic weights differ from
+ * the order-3 routines' in the eighth digit; the restatement keeps them.
+ * Per axis: the node-centred shape of order n at j (nint for even n,
+ * floor for odd), first node j - n/2; the staggered shape of order
+ * m = n - 1 (lower order in v) or n at j0, with
+ *   lower: j0 = floor(x - s) for even n, floor(x + 1/2 - s) for odd n;
+ *   else:  j0 = floor(x + 1/2 - s) for even n, floor(x - s) for odd n;
+ * offset (x - s) - j0 and first node j0 - (n-1)/2 (lower) or j0 - n/2.
+ * With l4symtry, x < 0 and y < 0 are mirrored and Ex, Ey, Bx, By take the
+ * mirror's sign.  Results are ADDED into ex..bz (E first, then B).      */
+static const double GN_SIXTH = (double)(1.0f / 6.0f);
+static const double GN_TWOTHIRD = (double)(2.0f / 3.0f);
+
+i64 orc_geteb3d_n(i64 np, const double *xp, const double *yp, const double *zp, double *ex,
+                  double *ey, double *ez, double *bx, double *by, double *bz, double xmin,
+                  double ymin, double zmin, double dx, double dy, double dz, i64 nx, i64 ny,
+                  i64 nz, i64 ngx, i64 ngy, i64 ngz, i64 nox, i64 noy, i64 noz,
+                  const double *exg, const double *eyg, const double *ezg, const double *bxg,
+                  const double *byg, const double *bzg, i64 l4symtry, i64 l_lower_order_in_v,
+                  i64 l_nodal) {
+  if (nox < 1 || nox > 3 || noy < 1 || noy > 3 || noz < 1 || noz > 3) return -1;
+  const int no[3] = {(int)nox, (int)noy, (int)noz};
+  const int lower = l_lower_order_in_v != 0;
+  const double s = l_nodal ? 0.0 : 0.5;
+  const double dxi = 1.0 / dx, dyi = 1.0 / dy, dzi = 1.0 / dz;
+  grid_t g = mkgrid(nx, ny, nz, ngx, ngy, ngz);
+  i64 bad = 0;
+  for (i64 ip = 0; ip < np; ip++) {
+    double p[3] = {(xp[ip] - xmin) * dxi, (yp[ip] - ymin) * dyi, (zp[ip] - zmin) * dzi};
+    double sgn[2] = {1.0, 1.0};
+    if (l4symtry) {
+      for (int a = 0; a < 2; a++)
+        if (p[a] < 0.0) {
+          p[a] = -p[a];
+          sgn[a] = -1.0;
+        }
+    }
+    double sn[3][4], ss[3][4];
+    i64 fn[3], fs[3];
+    int m[3];
+    for (int a = 0; a < 3; a++) {
+      const int n = no[a];
+      const int even = (n % 2 == 0);
+      const i64 j = even ? f_nint(p[a]) : f_floor(p[a]);
+      const i64 j0 = (even == lower) ? f_floor(p[a] - s) : f_floor((p[a] + 0.5) - s);
+      m[a] = lower ? n - 1 : n;
+      w_order(n, p[a] - (double)j, sn[a], GN_SIXTH, GN_TWOTHIRD);
+      w_order(m[a], (p[a] - s) - (double)j0, ss[a], GN_SIXTH, GN_TWOTHIRD);
+      fn[a] = j - n / 2;
+      fs[a] = j0 - (lower ? (n - 1) / 2 : n / 2);
+    }
+    i64 lo[3], hi[3];
+    for (int a = 0; a < 3; a++) {
+      lo[a] = fn[a] < fs[a] ? fn[a] : fs[a];
+      hi[a] = (fn[a] + no[a] > fs[a] + m[a]) ? fn[a] + no[a] : fs[a] + m[a];
+    }
+    if (!ginside(&g, lo[0], lo[1], lo[2]) || !ginside(&g, hi[0], hi[1], hi[2])) {
+      bad++;
+      continue;
+    }
+    const double *sx = sn[0], *sy = sn[1], *sz = sn[2];
+    const double *sx0 = ss[0], *sy0 = ss[1], *sz0 = ss[2];
+    const i64 j = fn[0], k = fn[1], l = fn[2], j0 = fs[0], k0 = fs[1], l0 = fs[2];
+    for (int c = 0; c <= no[2]; c++)
+      for (int b = 0; b <= no[1]; b++)
+        for (int a = 0; a <= m[0]; a++)
+          ex[ip] = ex[ip] + sx0[a] * sy[b] * sz[c] * exg[gidx(&g, j0 + a, k + b, l + c)] * sgn[0];
+    for (int c = 0; c <= no[2]; c++)
+      for (int b = 0; b <= m[1]; b++)
+        for (int a = 0; a <= no[0]; a++)
+          ey[ip] = ey[ip] + sx[a] * sy0[b] * sz[c] * eyg[gidx(&g, j + a, k0 + b, l + c)] * sgn[1];
+    for (int c = 0; c <= m[2]; c++)
+      for (int b = 0; b <= no[1]; b++)
+        for (int a = 0; a <= no[0]; a++)
+          ez[ip] = ez[ip] + sx[a] * sy[b] * sz0[c] * ezg[gidx(&g, j + a, k + b, l0 + c)];
+    for (int c = 0; c <= m[2]; c++)
+      for (int b = 0; b <= m[1]; b++)
+        for (int a = 0; a <= no[0]; a++)
+          bx[ip] = bx[ip] + sx[a] * sy0[b] * sz0[c] * bxg[gidx(&g, j + a, k0 + b, l0 + c)] * sgn[0];
+    for (int c = 0; c <= m[2]; c++)
+      for (int b = 0; b <= no[1]; b++)
+        for (int a = 0; a <= m[0]; a++)
+          by[ip] = by[ip] + sx0[a] * sy[b] * sz0[c] * byg[gidx(&g, j0 + a, k + b, l0 + c)] * sgn[1];
+    for (int c = 0; c <= no[2]; c++)
+      for (int b = 0; b <= m[1]; b++)
+        for (int a = 0; a <= m[0]; a++)
+          bz[ip] = bz[ip] + sx0[a] * sy0[b] * sz[c] * bzg[gidx(&g, j0 + a, k0 + b, l + c)];
   }
   return bad;
 }

@@ -79,6 +79,12 @@ class Oracle:
         L.orc_depose_esirkepov.argtypes = [P] * 3 + [I] + [P] * 8 + [F] * 8 + [I] * 6 + [I]
         L.orc_depose_rho.restype = I
         L.orc_depose_rho.argtypes = [P, I] + [P] * 4 + [F] * 7 + [I] * 6 + [I]
+        L.orc_depose_rho_n.restype = I
+        L.orc_depose_rho_n.argtypes = [P, I] + [P] * 4 + [F] * 7 + [I] * 6 + [I] * 5
+        L.orc_depose_esirkepov_n.restype = I
+        L.orc_depose_esirkepov_n.argtypes = [P] * 3 + [I] + [P] * 8 + [F] * 8 + [I] * 6 + [I] * 3
+        L.orc_geteb3d_n.restype = I
+        L.orc_geteb3d_n.argtypes = [I] + [P] * 9 + [F] * 6 + [I] * 9 + [P] * 6 + [I] * 3
         L.orc_push_em3d_evec.restype = None
         L.orc_push_em3d_evec.argtypes = [I] * 6 + [P] * 9 + [F] * 4 + [I] * 6
         L.orc_push_em3d_bvec.restype = None
@@ -167,6 +173,41 @@ class Oracle:
         return self.lib.orc_depose_rho(_ptr(rho), len(xp), _ptr(xp), _ptr(yp), _ptr(zp),
                                        _ptr(w), q, xmin, ymin, zmin, dx, dy, dz, nx, ny, nz,
                                        ngx, ngy, ngz, order)
+
+    # -- generic orders (one shape order per axis) ---------------------------
+    def depose_rho_n(self, rho, xp, yp, zp, w, q, xmin, ymin, zmin, dx, dy, dz, nx, ny, nz,
+                     ngx, ngy, ngz, nox, noy, noz, l_particles_weight=True, l4symtry=False):
+        """pxr_depose_rho_n (charge_deposition_3d.F90:2081), orders 0..3 per axis."""
+        _chk_grid(rho)
+        _chk_part(xp, yp, zp)
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        return self.lib.orc_depose_rho_n(_ptr(rho), len(xp), _ptr(xp), _ptr(yp), _ptr(zp),
+                                         _ptr(w), q, xmin, ymin, zmin, dx, dy, dz, nx, ny, nz,
+                                         ngx, ngy, ngz, nox, noy, noz,
+                                         int(bool(l_particles_weight)), int(bool(l4symtry)))
+
+    def depose_esirkepov_n(self, jx, jy, jz, xp, yp, zp, uxp, uyp, uzp, gaminv, w, q, xmin,
+                           ymin, zmin, dt, dx, dy, dz, nx, ny, nz, ngx, ngy, ngz, nox, noy, noz):
+        """pxr_depose_jxjyjz_esirkepov_n (esirkepov_3d.F90:3480), orders 0..3 per axis."""
+        _chk_grid(jx, jy, jz)
+        _chk_part(xp, yp, zp, uxp, uyp, uzp, gaminv, w)
+        return self.lib.orc_depose_esirkepov_n(
+            _ptr(jx), _ptr(jy), _ptr(jz), len(xp),
+            *map(_ptr, (xp, yp, zp, uxp, uyp, uzp, gaminv, w)),
+            q, xmin, ymin, zmin, dt, dx, dy, dz, nx, ny, nz, ngx, ngy, ngz, nox, noy, noz)
+
+    def geteb3d_n(self, xp, yp, zp, ex, ey, ez, bx, by, bz, xmin, ymin, zmin, dx, dy, dz, nx,
+                  ny, nz, ngx, ngy, ngz, nox, noy, noz, exg, eyg, ezg, bxg, byg, bzg,
+                  l4symtry=False, l_lower_order_in_v=True, l_nodal=False):
+        """pxr_gete3d_n_energy_conserving then pxr_getb3d_n_energy_conserving
+        (field_gathering_on_3d.F90:1042, 697), orders 1..3 per axis."""
+        _chk_part(xp, yp, zp, ex, ey, ez, bx, by, bz)
+        _chk_grid(exg, eyg, ezg, bxg, byg, bzg)
+        return self.lib.orc_geteb3d_n(
+            len(xp), _ptr(xp), _ptr(yp), _ptr(zp), _ptr(ex), _ptr(ey), _ptr(ez), _ptr(bx),
+            _ptr(by), _ptr(bz), xmin, ymin, zmin, dx, dy, dz, nx, ny, nz, ngx, ngy, ngz,
+            nox, noy, noz, _ptr(exg), _ptr(eyg), _ptr(ezg), _ptr(bxg), _ptr(byg), _ptr(bzg),
+            int(bool(l4symtry)), int(bool(l_lower_order_in_v)), int(bool(l_nodal)))
 
     # -- FDTD ---------------------------------------------------------------
     def push_em3d_evec(self, lo, hi, ex, ey, ez, bx, by, bz, jx, jy, jz, mudt, dtsdx, dtsdy,
@@ -390,6 +431,54 @@ class Reference:
         else:
             self._f("pxr_depose_rho_n_")(*head, _ri(order), _ri(order), _ri(order), _ri(1),
                                          _ri(0))
+
+    def depose_rho_n(self, rho, xp, yp, zp, w, q, xmin, ymin, zmin, dx, dy, dz, nx, ny, nz,
+                     ngx, ngy, ngz, nox, noy, noz, l_particles_weight=True, l4symtry=False):
+        """pxr_depose_rho_n with one order per axis."""
+        _chk_grid(rho)
+        _chk_part(xp, yp, zp)
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        self._f("pxr_depose_rho_n_")(
+            _ptr(rho), _ri(len(xp)), _ptr(xp), _ptr(yp), _ptr(zp), _ptr(w), _rf(q), _rf(xmin),
+            _rf(ymin), _rf(zmin), _rf(dx), _rf(dy), _rf(dz), _ri(nx), _ri(ny), _ri(nz),
+            _ri(ngx), _ri(ngy), _ri(ngz), _ri(nox), _ri(noy), _ri(noz),
+            _ri(1 if l_particles_weight else 0), _ri(1 if l4symtry else 0))
+
+    def depose_esirkepov_n(self, jx, jy, jz, xp, yp, zp, uxp, uyp, uzp, gaminv, w, q, xmin,
+                           ymin, zmin, dt, dx, dy, dz, nx, ny, nz, ngx, ngy, ngz, nox, noy, noz):
+        """pxr_depose_jxjyjz_esirkepov_n as depose_jxjyjz_generic calls it
+        (current_deposition_manager_3d.F90:186: particle weights, no symmetry)."""
+        _chk_grid(jx, jy, jz)
+        _chk_part(xp, yp, zp, uxp, uyp, uzp, gaminv, w)
+        ng = _iv(ngx, ngy, ngz)
+        nv = _iv(nx + 1, ny + 1, nz + 1)
+        self._f("pxr_depose_jxjyjz_esirkepov_n_")(
+            _ptr(jx), _ptr(ng), _ptr(nv), _ptr(jy), _ptr(ng), _ptr(nv), _ptr(jz), _ptr(ng),
+            _ptr(nv), _ri(len(xp)), *map(_ptr, (xp, yp, zp, uxp, uyp, uzp, gaminv, w)),
+            _rf(q), _rf(xmin), _rf(ymin), _rf(zmin), _rf(dt), _rf(dx), _rf(dy), _rf(dz),
+            _ri(nox), _ri(noy), _ri(noz), _ri(1), _ri(0))
+
+    def geteb3d_n(self, xp, yp, zp, ex, ey, ez, bx, by, bz, xmin, ymin, zmin, dx, dy, dz, nx,
+                  ny, nz, ngx, ngy, ngz, nox, noy, noz, exg, eyg, ezg, bxg, byg, bzg,
+                  l4symtry=False, l_lower_order_in_v=True, l_nodal=False):
+        """pxr_gete3d_n_energy_conserving then pxr_getb3d_n_energy_conserving,
+        as geteb3d_energy_conserving_generic calls them for field_gathe_algo 2
+        or orders other than 1_1_1, 2_2_2, 3_3_3."""
+        _chk_part(xp, yp, zp, ex, ey, ez, bx, by, bz)
+        _chk_grid(exg, eyg, ezg, bxg, byg, bzg)
+        ng = _iv(ngx, ngy, ngz)
+        nv = _iv(nx + 1, ny + 1, nz + 1)
+        flags = [_ri(1 if l4symtry else 0), _ri(1 if l_lower_order_in_v else 0),
+                 _ri(1 if l_nodal else 0)]
+        head = [_ri(len(xp)), _ptr(xp), _ptr(yp), _ptr(zp)]
+        tail = [_rf(xmin), _rf(ymin), _rf(zmin), _rf(dx), _rf(dy), _rf(dz), _ri(nox), _ri(noy),
+                _ri(noz)]
+        self._f("pxr_gete3d_n_energy_conserving_")(
+            *head, _ptr(ex), _ptr(ey), _ptr(ez), *tail, _ptr(exg), _ptr(ng), _ptr(nv),
+            _ptr(eyg), _ptr(ng), _ptr(nv), _ptr(ezg), _ptr(ng), _ptr(nv), *flags)
+        self._f("pxr_getb3d_n_energy_conserving_")(
+            *head, _ptr(bx), _ptr(by), _ptr(bz), *tail, _ptr(bxg), _ptr(ng), _ptr(nv),
+            _ptr(byg), _ptr(ng), _ptr(nv), _ptr(bzg), _ptr(ng), _ptr(nv), *flags)
 
     def push_em3d_evec(self, lo, hi, ex, ey, ez, bx, by, bz, jx, jy, jz, mudt, dtsdx, dtsdy,
                        dtsdz, nx, ny, nz, ngx, ngy, ngz):

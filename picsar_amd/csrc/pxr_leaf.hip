@@ -10,6 +10,7 @@
 #include <stdio.h>
 
 #include "pxr_device.h"
+#include "pxr_generic.h"
 #include "pxr_kernels.h"
 #include "pxr_runtime.h"
 
@@ -876,22 +877,40 @@ int pxrhip_geteb3d_energy_conserving(
     const double *eyg, const double *ezg, const double *bxg, const double *byg, const double *bzg,
     int64_t ll4symtry, int64_t l_lower_order_in_v, int64_t lvect, int64_t field_gathe_algo,
     int64_t l_nodal) {
-  (void)ll4symtry;
   (void)lvect;
-  (void)field_gathe_algo;
   PXR_INIT();
   if (np < 0) return set_error(PXRHIP_ERR_ARG, "np < 0");
   if (np == 0) return PXRHIP_OK;  // field_gathering_manager_3d.F90:418
   int r = check_grid(nx, ny, nz, nxguard, nyguard, nzguard);
   if (r) return r;
-  if (nox != noy || nox != noz)
-    return set_error(PXRHIP_ERR_UNSUPPORTED, "unequal shape orders (%lld,%lld,%lld): only nox=noy=noz is built",
-                     (long long)nox, (long long)noy, (long long)noz);
-  if (nox < 1 || nox > 3)
-    return set_error(PXRHIP_ERR_UNSUPPORTED, "shape order %lld: orders 1..3 are built", (long long)nox);
+  if (nox < 1 || nox > 3 || noy < 1 || noy > 3 || noz < 1 || noz > 3)
+    return set_error(PXRHIP_ERR_ARG,
+                     "shape orders (%lld,%lld,%lld): the reference's gather has weights for "
+                     "orders 1..3 on each axis", (long long)nox, (long long)noy, (long long)noz);
   if (!xp || !yp || !zp || !ex || !ey || !ez || !bx || !by || !bz || !exg || !eyg || !ezg || !bxg ||
       !byg || !bzg)
     return set_error(PXRHIP_ERR_ARG, "null array");
+  // geteb3d_energy_conserving_generic (field_gathering_manager_3d.F90:397-700):
+  // field_gathe_algo 2 and orders other than 1_1_1, 2_2_2, 3_3_3 run the
+  // generic routines, the only ones that read ll4symtry
+  if (field_gathe_algo == 2 || nox != noy || nox != noz) {
+    GatherNArgs n;
+    n.np = np;
+    n.xp = xp; n.yp = yp; n.zp = zp;
+    n.ex = ex; n.ey = ey; n.ez = ez; n.bx = bx; n.by = by; n.bz = bz;
+    n.xmin = xmin; n.ymin = ymin; n.zmin = zmin;
+    n.dxi = 1.0 / dx; n.dyi = 1.0 / dy; n.dzi = 1.0 / dz;
+    n.s = l_nodal ? 0.0 : 0.5;
+    n.g = mkgeom(nx, ny, nz, nxguard, nyguard, nzguard);
+    n.exg = exg; n.eyg = eyg; n.ezg = ezg; n.bxg = bxg; n.byg = byg; n.bzg = bzg;
+    n.no[0] = (int)nox; n.no[1] = (int)noy; n.no[2] = (int)noz;
+    n.lower = l_lower_order_in_v != 0;
+    n.l4symtry = ll4symtry != 0;
+    n.bad = rt().d_bad;
+    launch_gather_n(n, rt().stream);
+    PXR_LAUNCHED();
+    return finish_checked("geteb3d_energy_conserving");
+  }
   GatherArgs a;
   a.np = np;
   a.xp = xp; a.yp = yp; a.zp = zp;
@@ -1049,12 +1068,35 @@ int pxrhip_depose_jxjyjz(double *jx, double *jy, double *jz, int64_t np, const d
   if (np == 0) return PXRHIP_OK;
   int r = check_grid(nx, ny, nz, nxguard, nyguard, nzguard);
   if (r) return r;
-  if (nox != noy || nox != noz)
-    return set_error(PXRHIP_ERR_UNSUPPORTED, "unequal shape orders: only nox=noy=noz is built");
-  if (nox < 1 || nox > 3)
-    return set_error(PXRHIP_ERR_UNSUPPORTED, "shape order %lld: orders 1..3 are built", (long long)nox);
+  if (nox < 0 || nox > 3 || noy < 0 || noy > 3 || noz < 0 || noz > 3)
+    return set_error(PXRHIP_ERR_ARG,
+                     "shape orders (%lld,%lld,%lld): the reference's deposition has weights for "
+                     "orders 0..3 on each axis", (long long)nox, (long long)noy, (long long)noz);
   if (current_depo_algo < 0 || current_depo_algo > 3)
     return set_error(PXRHIP_ERR_ARG, "current_depo_algo must be 0..3");
+  if (!jx || !jy || !jz || !xp || !yp || !zp || !uxp || !uyp || !uzp || !gaminv || !w)
+    return set_error(PXRHIP_ERR_ARG, "null array");
+  // depose_jxjyjz_generic (current_deposition_manager_3d.F90:139-206):
+  // current_depo_algo 1, and any algorithm with orders other than 1_1_1,
+  // 2_2_2, 3_3_3, run pxr_depose_jxjyjz_esirkepov_n
+  if (current_depo_algo == 1 || nox != noy || nox != noz || nox == 0) {
+    EsirkNArgs n;
+    n.np = np;
+    n.xp = xp; n.yp = yp; n.zp = zp; n.ux = uxp; n.uy = uyp; n.uz = uzp; n.gi = gaminv; n.w = w;
+    n.q = q; n.xmin = xmin; n.ymin = ymin; n.zmin = zmin;
+    n.dxi = 1.0 / dx; n.dyi = 1.0 / dy; n.dzi = 1.0 / dz;  // esirkepov_3d.F90:3523-3534
+    n.dts[0] = dt * n.dxi; n.dts[1] = dt * n.dyi; n.dts[2] = dt * n.dzi;
+    n.invdt[0] = 1.0 / (dt * dy * dz);
+    n.invdt[1] = 1.0 / (dt * dx * dz);
+    n.invdt[2] = 1.0 / (dt * dx * dy);
+    n.g = mkgeom(nx, ny, nz, nxguard, nyguard, nzguard);
+    n.jx = jx; n.jy = jy; n.jz = jz;
+    n.no[0] = (int)nox; n.no[1] = (int)noy; n.no[2] = (int)noz;
+    n.bad = rt().d_bad;
+    launch_esirkepov_n(n, rt().stream);
+    PXR_LAUNCHED();
+    return finish_checked("depose_jxjyjz");
+  }
   DepoArgs a;
   a.np = np;
   a.xp = xp; a.yp = yp; a.zp = zp; a.ux = uxp; a.uy = uyp; a.uz = uzp; a.gi = gaminv; a.w = w;
@@ -1091,10 +1133,27 @@ int pxrhip_depose_rho_n(double *rho, int64_t np, const double *xp, const double 
   int r = check_grid(nx, ny, nz, nxguard, nyguard, nzguard);
   if (r) return r;
   if (!rho || !xp || !yp || !zp || !w) return set_error(PXRHIP_ERR_ARG, "null array");
-  if (nox != noy || nox != noz)
-    return set_error(PXRHIP_ERR_UNSUPPORTED, "unequal shape orders: only nox=noy=noz is built");
-  if (nox < 1 || nox > 3)
-    return set_error(PXRHIP_ERR_UNSUPPORTED, "shape order %lld: orders 1..3 are built", (long long)nox);
+  if (nox < 0 || nox > 3 || noy < 0 || noy > 3 || noz < 0 || noz > 3)
+    return set_error(PXRHIP_ERR_ARG,
+                     "shape orders (%lld,%lld,%lld): pxr_depose_rho_n has weights for orders "
+                     "0..3 on each axis", (long long)nox, (long long)noy, (long long)noz);
+  if (nox != noy || nox != noz || nox == 0) {
+    RhoNArgs n;
+    n.np = np;
+    n.xp = xp; n.yp = yp; n.zp = zp; n.w = w;
+    n.q = q; n.xmin = xmin; n.ymin = ymin; n.zmin = zmin;
+    n.dxi = 1.0 / dx; n.dyi = 1.0 / dy; n.dzi = 1.0 / dz;
+    n.invvol = n.dxi * n.dyi * n.dzi;
+    n.per_particle_weight = l_particles_weight != 0;
+    n.l4symtry = l4symtry != 0;
+    n.g = mkgeom(nx, ny, nz, nxguard, nyguard, nzguard);
+    n.rho = rho;
+    n.no[0] = (int)nox; n.no[1] = (int)noy; n.no[2] = (int)noz;
+    n.bad = rt().d_bad;
+    launch_rho_n(n, rt().stream);
+    PXR_LAUNCHED();
+    return finish_checked("pxr_depose_rho_n");
+  }
   RhoArgs a;
   a.np = np;
   a.xp = xp; a.yp = yp; a.zp = zp; a.w = w;

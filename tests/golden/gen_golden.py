@@ -132,6 +132,61 @@ def gen_rho(ref, rng):
     np.savez_compressed(os.path.join(HERE, "rho.npz"), **d)
 
 
+# one shape order per axis: the reference's generic routines
+GENERIC_GATHER = [  # (nox, noy, noz), l_lower_order_in_v, l_nodal, l4symtry
+    ((1, 2, 3), 1, 0, 0), ((3, 1, 2), 1, 0, 0), ((2, 3, 1), 0, 0, 0), ((3, 3, 3), 1, 0, 0),
+    ((3, 3, 3), 0, 0, 0), ((1, 1, 1), 1, 0, 0), ((2, 2, 3), 1, 1, 0), ((3, 2, 1), 0, 1, 0),
+    ((2, 3, 3), 1, 0, 1),
+]
+GENERIC_DEPOSIT = [(0, 1, 2), (1, 2, 3), (3, 2, 1), (3, 3, 3), (1, 1, 1), (0, 0, 0), (2, 0, 3)]
+GENERIC_RHO = [  # orders, l_particles_weight, l4symtry
+    ((0, 1, 2), 1, 0), ((3, 2, 1), 1, 0), ((0, 0, 0), 1, 0), ((2, 3, 1), 0, 0), ((3, 1, 3), 1, 1),
+]
+
+
+def gen_generic(ref, rng):
+    """pxr_gete3d_n/pxr_getb3d_n_energy_conserving (field_gathering_on_3d.F90),
+    pxr_depose_jxjyjz_esirkepov_n (esirkepov_3d.F90:3480) and
+    pxr_depose_rho_n (charge_deposition_3d.F90:2081) with a shape order per
+    axis, including order 0 where the routine has it, the nodal grid and
+    the 4-fold symmetry flag."""
+    n = 128
+    xp, yp, zp = positions(rng, n)
+    F = [grid(rng) for _ in range(6)]
+    d = dict(g_xp=xp, g_yp=yp, g_zp=zp, exg=F[0], eyg=F[1], ezg=F[2], bxg=F[3], byg=F[4],
+             bzg=F[5],
+             gather_cases=np.array([[*o, lo, nod, l4] for o, lo, nod, l4 in GENERIC_GATHER]),
+             esirk_cases=np.array(GENERIC_DEPOSIT),
+             rho_cases=np.array([[*o, lpw, l4] for o, lpw, l4 in GENERIC_RHO]))
+    for c, (o, lo, nod, l4) in enumerate(GENERIC_GATHER):
+        e = [np.zeros(n) for _ in range(6)]
+        ref.geteb3d_n(xp, yp, zp, *e, XMIN, YMIN, ZMIN, DX, DY, DZ, NX, NY, NZ, NG, NG, NG, *o,
+                      *F, l4symtry=l4, l_lower_order_in_v=lo, l_nodal=nod)
+        d[f"gather{c}"] = np.stack(e)
+    dt = 0.5
+    xp, yp, zp = positions(rng, n, lo=0.0)
+    u = [rng.uniform(-2.0, 2.0, n) for _ in range(3)]
+    for a in u:
+        a[:8] = 0.0
+    g = 1.0 / np.sqrt(1.0 + u[0] ** 2 + u[1] ** 2 + u[2] ** 2)
+    w = rng.uniform(0.5, 1.5, n)
+    d.update(d_xp=xp, d_yp=yp, d_zp=zp, ux=u[0], uy=u[1], uz=u[2], gaminv=g, w=w, q=-1.3,
+             dt=dt)
+    for c, o in enumerate(GENERIC_DEPOSIT):
+        j = [np.zeros(SHAPE, order="F") for _ in range(3)]
+        ref.depose_esirkepov_n(*j, xp, yp, zp, *u, g, w, -1.3, XMIN, YMIN, ZMIN, dt, DX, DY, DZ,
+                               NX, NY, NZ, NG, NG, NG, *o)
+        d[f"esirk{c}"] = np.stack(j)
+    xp, yp, zp = positions(rng, n, lo=-1.0)
+    d.update(r_xp=xp, r_yp=yp, r_zp=zp)
+    for c, (o, lpw, l4) in enumerate(GENERIC_RHO):
+        rho = np.zeros(SHAPE, order="F")
+        ref.depose_rho_n(rho, xp, yp, zp, w, 0.7, XMIN, YMIN, ZMIN, DX, DY, DZ, NX, NY, NZ, NG,
+                         NG, NG, *o, l_particles_weight=lpw, l4symtry=l4)
+        d[f"rho{c}"] = rho
+    np.savez_compressed(os.path.join(HERE, "generic.npz"), **d)
+
+
 def gen_fdtd(ref, rng):
     F = [grid(rng) for _ in range(9)]
     d = {f"in{i}": f for i, f in enumerate(F)}
@@ -237,7 +292,7 @@ def main():
     if ref is None:
         sys.exit("oracle/_ref/libpxrref.so is missing: run bash oracle/build_ref.sh first")
     which = set(sys.argv[1:]) or {"gather", "push", "deposit", "fdtd", "sort", "rho", "laser",
-                                  "div"}
+                                  "div", "generic"}
     rng = np.random.default_rng(20241202)  # one stream for the first five families
     for name, fn in (("gather", gen_gather), ("push", gen_push), ("deposit", gen_deposit),
                      ("fdtd", gen_fdtd), ("sort", gen_sort)):
@@ -249,6 +304,8 @@ def main():
         gen_laser(ref, np.random.default_rng(20241204))
     if "div" in which:
         gen_div(ref, np.random.default_rng(20241205))
+    if "generic" in which:
+        gen_generic(ref, np.random.default_rng(20241206))
     meta = dict(nx=NX, ny=NY, nz=NZ, ng=NG, dx=DX, dy=DY, dz=DZ, xmin=XMIN, ymin=YMIN, zmin=ZMIN)
     np.savez(os.path.join(HERE, "grid.npz"), **meta)
     for f in sorted(os.listdir(HERE)):

@@ -149,3 +149,51 @@ def test_field_divergence(orc, golden):
         out = np.asfortranarray(np.full(F[0].shape, 7.0))
         fn(out, *F, nx, ny, nz, g, g, g, *h)
         np.testing.assert_array_equal(out, d[name])
+
+
+# One shape order per axis: the reference's generic routines, which the
+# dispatchers call for unequal orders, order 0 (deposition), field_gathe_algo
+# 2 and current_depo_algo 1.  Each case is stored with the fixture.
+def test_generic_gather(orc, golden, ggrid):
+    """pxr_gete3d_n/pxr_getb3d_n_energy_conserving bit for bit, including
+    their single-precision 1./6. and 2./3. in the cubic weights (about 1e-7
+    away from the order-3 routines' gather)."""
+    d = golden("generic")
+    g = ggrid
+    n = len(d["g_xp"])
+    F = [np.asfortranarray(d[k]) for k in ("exg", "eyg", "ezg", "bxg", "byg", "bzg")]
+    for c, (nox, noy, noz, lo, nod, l4) in enumerate(d["gather_cases"]):
+        e = [np.zeros(n) for _ in range(6)]
+        bad = orc.geteb3d_n(d["g_xp"], d["g_yp"], d["g_zp"], *e, g["xmin"], g["ymin"],
+                            g["zmin"], g["dx"], g["dy"], g["dz"], g["nx"], g["ny"], g["nz"],
+                            g["ng"], g["ng"], g["ng"], int(nox), int(noy), int(noz), *F,
+                            l4symtry=l4, l_lower_order_in_v=lo, l_nodal=nod)
+        assert bad == 0
+        np.testing.assert_array_equal(np.stack(e), d[f"gather{c}"], err_msg=f"case {c}")
+
+
+def test_generic_esirkepov(orc, golden, ggrid):
+    d = golden("generic")
+    g = ggrid
+    for c, o in enumerate(d["esirk_cases"]):
+        j = [np.zeros(d[f"esirk{c}"].shape[1:], order="F") for _ in range(3)]
+        bad = orc.depose_esirkepov_n(*j, d["d_xp"], d["d_yp"], d["d_zp"], d["ux"], d["uy"],
+                                     d["uz"], d["gaminv"], d["w"], float(d["q"]), g["xmin"],
+                                     g["ymin"], g["zmin"], float(d["dt"]), g["dx"], g["dy"],
+                                     g["dz"], g["nx"], g["ny"], g["nz"], g["ng"], g["ng"],
+                                     g["ng"], *map(int, o))
+        assert bad == 0
+        np.testing.assert_array_equal(np.stack(j), d[f"esirk{c}"], err_msg=f"case {c}")
+
+
+def test_generic_rho(orc, golden, ggrid):
+    d = golden("generic")
+    g = ggrid
+    for c, (nox, noy, noz, lpw, l4) in enumerate(d["rho_cases"]):
+        rho = np.zeros(d[f"rho{c}"].shape, order="F")
+        bad = orc.depose_rho_n(rho, d["r_xp"], d["r_yp"], d["r_zp"], d["w"], 0.7, g["xmin"],
+                               g["ymin"], g["zmin"], g["dx"], g["dy"], g["dz"], g["nx"], g["ny"],
+                               g["nz"], g["ng"], g["ng"], g["ng"], int(nox), int(noy), int(noz),
+                               l_particles_weight=lpw, l4symtry=l4)
+        assert bad == 0
+        np.testing.assert_array_equal(rho, d[f"rho{c}"], err_msg=f"case {c}")
