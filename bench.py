@@ -28,6 +28,13 @@ steps.
     python bench.py --n 128 --ppc 16 --tiles 8 --precision fp64 --no-compare   # 128^3 FP64
     python bench.py --workload bunch --n 256 --tiles 32 --precision mixed       # uneven tiles
     python bench.py --workload laser --n 512 --precision mixed                  # laser plasma
+    python bench.py --workload leaf --n 128 --ppc 16 --order 3                  # leaf deposition
+
+`--workload leaf` times the drop-in leaf routines depose_jxjyjz (Esirkepov)
+and pxr_depose_rho_n on n^3 * ppc particles of one species handed over as
+plain device arrays, in cell order and in random order, in their global-
+atomic form and their binned form (particles counted into blocks of nodes
+and summed in LDS), and prints one JSON line of those timings.
 """
 from __future__ import annotations
 
@@ -556,6 +563,66 @@ def roofline_record(r, args, prec):
     return rec
 
 
+def leaf_bench(args):
+    """Leaf deposition through the C ABI: depose_jxjyjz (current_depo_algo
+    0, Esirkepov) and pxr_depose_rho_n of args.order on n^3 * ppc particles
+    (u ~ 0.05 c), cell-ordered and in random order, with the runtime's
+    switch-over to the binned form set to never (global atomics) and to its
+    default.  Each call is timed with HIP events on the library stream; the
+    best of args.steps calls is reported."""
+    import ctypes
+    from picsar_amd import _lib
+    from picsar_amd.devmem import DeviceArray, DeviceGrid
+    from picsar_amd.picsarpy import picsar as pxr
+    pxr.l_nodalgrid = False
+    n, ng, o = args.n, 3, args.order
+    rng = np.random.default_rng(12345)
+    npart = n ** 3 * args.ppc
+    x, y, z = (rng.random(npart) * n for _ in range(3))
+    u = [rng.standard_normal(npart) * 0.05 for _ in range(3)]
+    g = 1 / np.sqrt(1 + u[0] ** 2 + u[1] ** 2 + u[2] ** 2)
+    w = np.full(npart, 1e-3)
+    J = [DeviceGrid(n, n, n, ng) for _ in range(3)]
+    R = DeviceGrid(n, n, n, ng)
+    default_min = ctypes.c_int64(0)
+    _lib.call("pxrhip_get_tuning", b"leaf.bin_min", ctypes.byref(default_min))
+
+    def timed(fn):
+        best = float("inf")
+        for _ in range(max(1, args.steps)):
+            _lib.call("pxrhip_timer_record", 60000)
+            fn()
+            _lib.call("pxrhip_timer_record", 60001)
+            best = min(best, elapsed(60000, 60001))
+        return best
+
+    out = {}
+    orders = {"cell_ordered": np.lexsort((np.floor(x), np.floor(y), np.floor(z))),
+              "random": np.arange(npart)}
+    for name, perm in orders.items():
+        P = [DeviceArray.from_numpy(np.ascontiguousarray(a[perm])) for a in (x, y, z, *u, g, w)]
+        for form, v in (("global_atomics", 0), ("binned", default_min.value)):
+            _lib.call("pxrhip_set_tuning", b"leaf.bin_min", int(v))
+            tj = timed(lambda: pxr.depose_jxjyjz(*J, npart, *P, -1.0, 0, 0, 0, 0.5, 1, 1, 1, n, n,
+                                                n, ng, ng, ng, o, o, o, 0))
+            tr = timed(lambda: pxr.pxr_depose_rho_n(R, npart, P[0], P[1], P[2], P[7], -1.0, 0, 0,
+                                                    0, 1, 1, 1, n, n, n, ng, ng, ng, o, o, o))
+            out[f"{name}/{form}"] = {"depose_jxjyjz_ms": tj, "depose_rho_n_ms": tr}
+            log(f"leaf {name} {form}: J {tj:.2f} ms, rho {tr:.2f} ms")
+        del P
+    _lib.call("pxrhip_set_tuning", b"leaf.bin_min", int(default_min.value))
+    best = out["cell_ordered/binned"]["depose_jxjyjz_ms"]
+    print(json.dumps({
+        "metric": "leaf Esirkepov current deposition, particles/s",
+        "value": npart / (best * 1e-3), "unit": "particles/s", "n_gpus": 1,
+        "higher_is_better": True, "dtype": "f64", "data": "synthetic (numpy seed 12345)",
+        "config": {"workload": f"leaf depose_jxjyjz and pxr_depose_rho_n, order {o}, {n}^3 "
+                               f"cells, {npart} particles of one species (u ~ 0.05 c), "
+                               f"{ng} guards", "calls_per_timing": max(1, args.steps),
+                   "timing": "best call, HIP events on the library stream"},
+        "ms": out}))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -567,7 +634,7 @@ def main():
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--tile-nodes", type=int, default=0,
                     help="nodes per tile along each axis of the laser workload (default 8)")
-    ap.add_argument("--workload", choices=("thermal", "bunch", "laser"), default="thermal",
+    ap.add_argument("--workload", choices=("thermal", "bunch", "laser", "leaf"), default="thermal",
                     help="thermal: periodic thermal plasma (electrons + ions); bunch: Gaussian "
                          "gamma=10 electron bunch in a 1 ppc background (uneven tiles); laser: "
                          "laser-driven underdense plasma on n x n x 2n cells (512: the large run)")
@@ -613,6 +680,9 @@ def main():
     for kv in args.tune:
         k, v = kv.split("=")
         _lib.call("pxrhip_set_tuning", k.encode(), int(v))
+    if args.workload == "leaf":
+        leaf_bench(args)
+        return
     tiles_tuned = choose_tiles(args) if args.autotune_tiles else None
     compare = (not args.no_compare and args.workload != "laser" and not args.graph)
     precs = [args.precision] + ([p for p in ("fp64", "mixed") if p != args.precision]

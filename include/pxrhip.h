@@ -179,7 +179,10 @@ int pxrhip_set_gamma(int64_t np, const double *uxp, const double *uyp,
  * (direct_current_deposition_3d.F90:99/879/1992).  Shape orders 0..3 on
  * each axis (others: PXRHIP_ERR_ARG); orders other than 1_1_1, 2_2_2, 3_3_3
  * run the generic Esirkepov whatever the algorithm, as in the reference.
- * Adds into jx, jy, jz. */
+ * Adds into jx, jy, jz.  Calls of many particles (tuning key
+ * "leaf.bin_min") order them by 8^3-node block and sum each block in LDS
+ * before adding it to the arrays; the result differs from the per-particle
+ * form only in the order of the FP64 sums. */
 int pxrhip_depose_jxjyjz(double *jx, double *jy, double *jz, int64_t np,
                          const double *xp, const double *yp, const double *zp,
                          const double *uxp, const double *uyp,
@@ -454,11 +457,19 @@ int pxrhip_tiles_depose_rho(const pxrhip_grid *grid, const pxrhip_species *sp,
  * the LDS current block), "fused.margin_pct" (motion margin, in 1/100 cell,
  * of the current block estimated when work items are cut), "mixed.jacc"
  * (LDS current of a mixed-precision species: 0 FP32 atomics, 1 FP32
- * atomics compensated by TwoSum into a second array, 2 FP64 atomics),
- * "mixed.lds_bytes" and "mixed.wg_per_cu" (the launch of a mixed-precision
- * species; fused.lds_bytes and fused.wg_per_cu are those of an FP64 one),
- * "rebin.stable" (1, the default: pxrhip_tiles_rebin_sort keeps the source
- * order inside a cell; 0: the order in which the GPU's waves arrive).
+ * atomics compensated by TwoSum into a second array, 2 FP64 atomics, 3 the
+ * default: 64-bit fixed point), "fused.jacc" (LDS current of an FP64
+ * species: 0 FP64 atomics, 3 the default: 64-bit fixed point),
+ * "fused.order" (0, the default: work items in the order they are built;
+ * 1 largest first), "mixed.lds_bytes" and "mixed.wg_per_cu" (the launch of
+ * a mixed-precision species; fused.lds_bytes and fused.wg_per_cu are those
+ * of an FP64 one), "rebin.stable" (1, the default: pxrhip_tiles_rebin_sort
+ * keeps the source order inside a cell; 0: the order in which the GPU's
+ * waves arrive), "leaf.bin_min" (pxrhip_depose_jxjyjz with the Esirkepov
+ * current and pxrhip_depose_rho_n sum calls of at least this many particles,
+ * and one per 8^3-node block of the arrays, block by block in LDS; default
+ * 65536, 0: never) and "leaf.bin_chunk" (largest work item of that form,
+ * default 4096 particles).
  * wg_per_cu 0 (the default) is automatic: three workgroups per CU with
  * 52224 bytes each when a work item of four sorting rows of the widest tile
  * fits that, else two with 79616 bytes; a wg_per_cu above 0 takes the
@@ -466,7 +477,8 @@ int pxrhip_tiles_depose_rho(const pxrhip_grid *grid, const pxrhip_species *sp,
  * direct deposition runs at most two workgroups per CU, since its kernel
  * holds registers for two waves per SIMD; it then takes 2 and 79616 bytes.
  * pxrhip_get_tuning also reads "last.wg_per_cu" and "last.lds_bytes", the
- * launch of the last pxrhip_tiles_gather_push_deposit.  Work items whose
+ * launch of the last pxrhip_tiles_gather_push_deposit, and
+ * "device.num_cus".  Work items whose
  * field or current block exceeds its LDS budget are completed from global
  * memory by a second kernel. */
 int pxrhip_set_tuning(const char *key, int64_t value);

@@ -12,6 +12,7 @@
 #include "pxr_device.h"
 #include "pxr_generic.h"
 #include "pxr_kernels.h"
+#include "pxr_leafbin.h"
 #include "pxr_runtime.h"
 
 using namespace pxr;
@@ -217,25 +218,6 @@ __global__ __launch_bounds__(256) void k_set_gamma(i64 np, const double *ux, con
 // ---------------------------------------------------------------------------
 // current deposition (global atomics)
 // ---------------------------------------------------------------------------
-struct DepoArgs {
-  i64 np;
-  const double *xp, *yp, *zp, *ux, *uy, *uz, *gi, *w;
-  double q, xmin, ymin, zmin, dt, dx, dy, dz, s;
-  EsirkConst<double> ec;
-  GridGeom g;
-  double *jx, *jy, *jz;
-  unsigned long long *bad;
-};
-
-struct GlobalAcc {
-  double *p[3];
-  i64 sy, sz;
-  PXR_DEV void add(int c, int i, int j, int k, double v) const {
-    unsafeAtomicAdd(p[c] + (i + sy * j + sz * k), v);
-  }
-  PXR_DEV double *row(int c, int i, int j, int k) const { return p[c] + (i + sy * j + sz * k); }
-  PXR_DEV void add_row(double *r, int o, double v) const { unsafeAtomicAdd(r + o, v); }
-};
 
 // Particles of the deposition kernels: a block takes 256 * DEP_S of them
 // and the lanes of a wave take particles 4 * DEP_S apart (each lane DEP_S
@@ -300,15 +282,6 @@ __global__ __launch_bounds__(256) void k_direct(DepoArgs a) {
 // charge density, pxr_depose_rho_n (charge_deposition_3d.F90:2081-2228):
 // nodal shape of order N at floor (N odd) / nint (N even), wq = q w / vol,
 // rho(j+a, k+b, l+c) += sx(a) sy(b) sz(c) wq
-struct RhoArgs {
-  i64 np;
-  const double *xp, *yp, *zp, *w;
-  double q, xmin, ymin, zmin, dxi, dyi, dzi, invvol;
-  int per_particle_weight, l4symtry;
-  GridGeom g;
-  double *rho;
-  unsigned long long *bad;
-};
 
 template <int N>
 PXR_DEV void rho_one(const RhoArgs &a, i64 ip) {
@@ -1109,7 +1082,10 @@ int pxrhip_depose_jxjyjz(double *jx, double *jy, double *jz, int64_t np, const d
   dim3 grd(grid1d(np, 256 * DEP_S)), blk(256);
   hipStream_t st = rt().stream;
   const bool direct = current_depo_algo >= 2;  // depose_jxjyjz_generic CASE(2), CASE(3)
-  if (direct) {
+  if (!direct && leaf_binned_esirkepov_pays(np, a.g)) {
+    r = leaf_binned_esirkepov(a, (int)nox, st);
+    if (r) return r;
+  } else if (direct) {
     if (nox == 1) hipLaunchKernelGGL(k_direct<1>, grd, blk, 0, st, a);
     else if (nox == 2) hipLaunchKernelGGL(k_direct<2>, grd, blk, 0, st, a);
     else hipLaunchKernelGGL(k_direct<3>, grd, blk, 0, st, a);
@@ -1167,7 +1143,10 @@ int pxrhip_depose_rho_n(double *rho, int64_t np, const double *xp, const double 
   a.bad = rt().d_bad;
   dim3 grd(grid1d(np, 256 * DEP_S)), blk(256);
   hipStream_t st = rt().stream;
-  if (nox == 1) hipLaunchKernelGGL(k_rho<1>, grd, blk, 0, st, a);
+  if (leaf_binned_rho_pays(np, a.g)) {
+    r = leaf_binned_rho(a, (int)nox, st);
+    if (r) return r;
+  } else if (nox == 1) hipLaunchKernelGGL(k_rho<1>, grd, blk, 0, st, a);
   else if (nox == 2) hipLaunchKernelGGL(k_rho<2>, grd, blk, 0, st, a);
   else hipLaunchKernelGGL(k_rho<3>, grd, blk, 0, st, a);
   PXR_LAUNCHED();

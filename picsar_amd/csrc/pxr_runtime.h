@@ -17,6 +17,10 @@ struct Runtime {
   bool checks = true;
   unsigned long long *d_bad = nullptr;  // device counter of skipped particles
   int num_cus = 256;
+  // leaf deposition summed in LDS blocks (pxr_leafbin.hip) from this many
+  // particles on (0: never), work items of at most leaf_bin_chunk particles
+  int64_t leaf_bin_min = 1 << 16;
+  int leaf_bin_chunk = 4096;
   // incremented whenever library-owned device memory that recorded work may
   // point at is freed (a scratch slot that grows, the cached tile table):
   // a HIP graph captured before that refers to freed memory

@@ -16,7 +16,7 @@ struct Scratch {
   void *p = nullptr;
   size_t bytes = 0;
 };
-static std::vector<Scratch> g_scratch(24);
+static std::vector<Scratch> g_scratch(32);
 
 Runtime &rt() { return g_rt; }
 

@@ -1100,8 +1100,12 @@ int pxrhip_set_tuning(const char *key, int64_t value) {
     g_tune.stable_rebin = value != 0;
   } else if (!strcmp(key, "fused.order")) {
     g_tune.order = value != 0;
-
-
+  } else if (!strcmp(key, "leaf.bin_min")) {
+    if (value < 0) return set_error(PXRHIP_ERR_ARG, "leaf.bin_min must be >= 0 (0: never)");
+    rt().leaf_bin_min = value;
+  } else if (!strcmp(key, "leaf.bin_chunk")) {
+    if (value < 1 || value > (1 << 20)) return set_error(PXRHIP_ERR_ARG, "leaf.bin_chunk must be in [1, 2^20]");
+    rt().leaf_bin_chunk = (int)value;
   } else if (!strcmp(key, "fused.margin_pct")) {
     if (value < 0 || value > 200) return set_error(PXRHIP_ERR_ARG, "fused.margin_pct must be in [0, 200]");
     g_tune.margin_pct = (int)value;
@@ -1125,6 +1129,8 @@ int pxrhip_get_tuning(const char *key, int64_t *value) {
   else if (!strcmp(key, "fused.jpad")) *value = g_tune.jpad;
   else if (!strcmp(key, "rebin.stable")) *value = g_tune.stable_rebin;
   else if (!strcmp(key, "fused.order")) *value = g_tune.order;
+  else if (!strcmp(key, "leaf.bin_min")) *value = rt().leaf_bin_min;
+  else if (!strcmp(key, "leaf.bin_chunk")) *value = rt().leaf_bin_chunk;
   else if (!strcmp(key, "fused.sstride")) *value = g_tune.sstride;
   else if (!strcmp(key, "mixed.jacc")) *value = g_tune.jacc;
   else if (!strcmp(key, "fused.jacc")) *value = g_tune.fp64_jacc;
