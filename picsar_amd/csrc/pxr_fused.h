@@ -561,14 +561,19 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(
       }
       const int nb = (int)nb64;
       FT *sf = reinterpret_cast<FT *>(smem);
-      for (int f = 0; f < 6; f++) {
-        const double *src = a.F[f];
-        for (int idx = threadIdx.x; idx < nb; idx += BS) {
-          const int i = FR0[0] + idx % fe[0];
-          const int j = FR0[1] + (idx / fe[0]) % fe[1];
-          const int k = FR0[2] + idx / (fe[0] * fe[1]);
-          sf[f * nb + idx] = (FT)src[g.idx(i, j, k)];
-        }
+      // one node per lane and step, its six field values loaded together:
+      // the node's position is decomposed once for all six, and the six
+      // loads are in flight at the same time
+      for (int idx = threadIdx.x; idx < nb; idx += BS) {
+        const int i = FR0[0] + idx % fe[0];
+        const int j = FR0[1] + (idx / fe[0]) % fe[1];
+        const int k = FR0[2] + idx / (fe[0] * fe[1]);
+        const i64 go = g.idx(i, j, k);
+        double v[6];
+#pragma unroll
+        for (int f = 0; f < 6; f++) v[f] = a.F[f][go];
+#pragma unroll
+        for (int f = 0; f < 6; f++) sf[f * nb + idx] = (FT)v[f];
       }
       __syncthreads();
       // ---- 2. gather, push, position push -------------------------------
