@@ -220,12 +220,21 @@ PXR_DEV double fx_weight(const FusedArgs &a, double qw, double vx, double vy, do
 // Gather, push and position push of particle p.  Fields are read from the
 // LDS block when the stencil lies inside it (views fl), else from global
 // memory (fg).  Extends the box of the current the particle will deposit.
+// an FP64 particle as the push reads it
+struct Part64 {
+  double x, y, z, ux, uy, uz, gi, w;
+};
+
+PXR_DEV Part64 load_part64(const FusedArgs &a, i64 p) {
+  return Part64{a.x[p], a.y[p], a.z[p], a.ux[p], a.uy[p], a.uz[p], a.gi[p], a.w[p]};
+}
+
 template <int N, bool LO, int PUSHER, int DEPO, typename VL, typename VG>
-PXR_DEV void gpd_push(const FusedArgs &a, i64 p, const VL *fl, const VG *fg, const int *FR0,
-                      const int *fe, Box6 &dep, bool box, double &wabs) {
+PXR_DEV void gpd_push(const FusedArgs &a, i64 p, const Part64 &P, const VL *fl, const VG *fg,
+                      const int *FR0, const int *fe, Box6 &dep, bool box, double &wabs) {
   constexpr int M = LO ? N - 1 : N;
   const GridGeom &g = a.g;
-  const double x = a.x[p], y = a.y[p], z = a.z[p];
+  const double x = P.x, y = P.y, z = P.z;
   GatherStencil<N, M, double> st;
   gather_stencil<N, M, double>(st, (x - a.xmin) * a.dxi, (y - a.ymin) * a.dyi,
                                (z - a.zmin) * a.dzi, a.s);
@@ -238,8 +247,8 @@ PXR_DEV void gpd_push(const FusedArgs &a, i64 p, const VL *fl, const VG *fg, con
   double E[6];
   if (in_block(l2, h2, FR0, fe)) gather_apply<N, M, double>(st, fl, E);
   else gather_apply<N, M, double>(st, fg, E);
-  double u[3] = {a.ux[p], a.uy[p], a.uz[p]};
-  double gi = a.gi[p];
+  double u[3] = {P.ux, P.uy, P.uz};
+  double gi = P.gi;
   if (PUSHER == 1) vay_push<double>(u, gi, E[0], E[1], E[2], E[3], E[4], E[5], a.cst);
   else boris_push<double>(u, gi, E[0], E[1], E[2], E[3], E[4], E[5], a.cst);
   // pxr_pushxyz (boris_3d.F90:880-882)
@@ -254,15 +263,15 @@ PXR_DEV void gpd_push(const FusedArgs &a, i64 p, const VL *fl, const VG *fg, con
   a.y[p] = yn;
   a.z[p] = zn;
   if (!box) return;
-  wabs = fmax(wabs, fx_weight<DEPO>(a, a.q * a.w[p], u[0] * gi, u[1] * gi, u[2] * gi));
+  wabs = fmax(wabs, fx_weight<DEPO>(a, a.q * P.w, u[0] * gi, u[1] * gi, u[2] * gi));
   int b0[3], b1[3];
   if (DEPO == 0) {
     EsirkepovIn<double> in;
-    esirkepov_prepare<N, double>(in, xn, yn, zn, u[0], u[1], u[2], gi, a.w[p], a.ec);
+    esirkepov_prepare<N, double>(in, xn, yn, zn, u[0], u[1], u[2], gi, P.w, a.ec);
     if (!esirkepov_box<N, double>(in, b0, b1)) return;
   } else {
     DirectIn<double> in;
-    direct_prepare<N, double>(in, xn, yn, zn, u[0], u[1], u[2], gi, a.w[p], a.q, a.xmin, a.ymin,
+    direct_prepare<N, double>(in, xn, yn, zn, u[0], u[1], u[2], gi, P.w, a.q, a.xmin, a.ymin,
                               a.zmin, a.dt, a.dx, a.dy, a.dz, a.s);
     direct_box<N>(in, b0, b1);
   }
@@ -349,12 +358,22 @@ PXR_DEV bool mx_box(const FusedArgs &a, const double *pn, const double *v, int *
   return true;
 }
 
+// a mixed-precision particle as the push reads it
+struct PartMx {
+  double x, y, z;
+  float ux, uy, uz, gi, w;
+};
+
+PXR_DEV PartMx load_part_mx(const FusedArgs &a, i64 p) {
+  return PartMx{a.x[p], a.y[p], a.z[p], a.fux[p], a.fuy[p], a.fuz[p], a.fgi[p], a.fw[p]};
+}
+
 template <int N, bool LO, int PUSHER, int DEPO, typename VL, typename VG>
-PXR_DEV void gpd_push_mx(const FusedArgs &a, i64 p, const VL *fl, const VG *fg, const int *FR0,
-                         const int *fe, Box6 &dep, bool box, double &wabs) {
+PXR_DEV void gpd_push_mx(const FusedArgs &a, i64 p, const PartMx &P, const VL *fl, const VG *fg,
+                         const int *FR0, const int *fe, Box6 &dep, bool box, double &wabs) {
   constexpr int M = LO ? N - 1 : N;
   const GridGeom &g = a.g;
-  const double x = a.x[p], y = a.y[p], z = a.z[p];
+  const double x = P.x, y = P.y, z = P.z;
   GatherStencil<N, M, float> st;
   gather_stencil_mx<N, M>(st, (x - a.xmin) * a.dxi, (y - a.ymin) * a.dyi, (z - a.zmin) * a.dzi,
                           a.s);
@@ -367,8 +386,8 @@ PXR_DEV void gpd_push_mx(const FusedArgs &a, i64 p, const VL *fl, const VG *fg, 
   float E[6];
   if (in_block(l2, h2, FR0, fe)) gather_apply<N, M, float>(st, fl, E);
   else gather_apply<N, M, float>(st, fg, E);
-  float u[3] = {a.fux[p], a.fuy[p], a.fuz[p]};
-  float gi = a.fgi[p];
+  float u[3] = {P.ux, P.uy, P.uz};
+  float gi = P.gi;
   if (PUSHER == 1) vay_push<float>(u, gi, E[0], E[1], E[2], E[3], E[4], E[5], (float)a.cst);
   else boris_push<float>(u, gi, E[0], E[1], E[2], E[3], E[4], E[5], (float)a.cst);
   // position push in FP64 (the products of two floats are exact in FP64)
@@ -383,7 +402,7 @@ PXR_DEV void gpd_push_mx(const FusedArgs &a, i64 p, const VL *fl, const VG *fg, 
   a.y[p] = yn;
   a.z[p] = zn;
   if (!box) return;
-  wabs = fmax(wabs, fx_weight<DEPO>(a, a.q * (double)a.fw[p], v[0], v[1], v[2]));
+  wabs = fmax(wabs, fx_weight<DEPO>(a, a.q * (double)P.w, v[0], v[1], v[2]));
   const double pn[3] = {(xn - a.xmin) * a.dxi, (yn - a.ymin) * a.dyi, (zn - a.zmin) * a.dzi};
   int b0[3], b1[3];
   if (!mx_box<N, DEPO>(a, pn, v, b0, b1)) return;
@@ -563,7 +582,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(
       FT *sf = reinterpret_cast<FT *>(smem);
       // one node per lane and step, its six field values loaded together:
       // the node's position is decomposed once for all six, and the six
-      // loads are in flight at the same time
+      // loads are in flight at the same time (two nodes per step, twelve
+      // loads, made the launch slower)
       for (int idx = threadIdx.x; idx < nb; idx += BS) {
         const int i = FR0[0] + idx % fe[0];
         const int j = FR0[1] + (idx / fe[0]) % fe[1];
@@ -583,14 +603,38 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(
       if constexpr (MX) {
         FieldViewF fg[6];
         for (int f = 0; f < 6; f++) fg[f] = FieldViewF{a.F[f] + g.idx(0, 0, 0), g.NX, g.NX * g.NY};
-        for (int k = threadIdx.x; k < ch.n; k += BS)
-          gpd_push_mx<N, LO, PUSHER, DEPO>(a, p0 + k, fl, fg, FR0, fe, dep, depo, wabs);
+        // the next particle's values are loaded before the current one is
+        // pushed, so that their round trip overlaps its arithmetic
+        int k = threadIdx.x;
+        if (k < ch.n) {
+          PartMx cur = load_part_mx(a, p0 + k);
+          for (;;) {
+            const int kn = k + BS;
+            PartMx nxt = cur;
+            if (kn < ch.n) nxt = load_part_mx(a, p0 + kn);
+            gpd_push_mx<N, LO, PUSHER, DEPO>(a, p0 + k, cur, fl, fg, FR0, fe, dep, depo, wabs);
+            if (kn >= ch.n) break;
+            k = kn;
+            cur = nxt;
+          }
+        }
       } else {
         FieldView<double> fg[6];
         for (int f = 0; f < 6; f++)
           fg[f] = FieldView<double>{a.F[f] + g.idx(0, 0, 0), g.NX, g.NX * g.NY};
-        for (int k = threadIdx.x; k < ch.n; k += BS)
-          gpd_push<N, LO, PUSHER, DEPO>(a, p0 + k, fl, fg, FR0, fe, dep, depo, wabs);
+        int k = threadIdx.x;
+        if (k < ch.n) {
+          Part64 cur = load_part64(a, p0 + k);
+          for (;;) {
+            const int kn = k + BS;
+            Part64 nxt = cur;
+            if (kn < ch.n) nxt = load_part64(a, p0 + kn);
+            gpd_push<N, LO, PUSHER, DEPO>(a, p0 + k, cur, fl, fg, FR0, fe, dep, depo, wabs);
+            if (kn >= ch.n) break;
+            k = kn;
+            cur = nxt;
+          }
+        }
       }
       if (!depo) {
         __syncthreads();  // field block no longer read
@@ -676,6 +720,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(
         float *se = reinterpret_cast<float *>(smem) + 3 * nj;  // compensation (JACC 1)
         const LdsAccMx<JACC> la{{sj, sj + nj, sj + 2 * nj}, {se, se + nj, se + 2 * nj},
                                 R0[0], R0[1], R0[2], py, pz, fxs};
+        // (loading the next particle ahead, as the push loop does, makes this
+        // loop spill registers and slows the launch)
         for (int kk = threadIdx.x; kk < nkk; kk += BS) {
           const int k = pick(kk);
           if (k < ch.n && !gpd_deposit_mx<N, DEPO>(a, p0 + k, la)) count_bad(a.bad);
@@ -734,13 +780,15 @@ __global__ __launch_bounds__(BS) void k_fused_overflow(FusedArgs a) {
         FieldViewF fg[6];
         for (int f = 0; f < 6; f++) fg[f] = FieldViewF{a.F[f] + g.idx(0, 0, 0), g.NX, g.NX * g.NY};
         for (int k = threadIdx.x; k < ch.n; k += BS)
-          gpd_push_mx<N, LO, PUSHER, DEPO>(a, ch.begin + k, fg, fg, none, none, dep, false, wabs);
+          gpd_push_mx<N, LO, PUSHER, DEPO>(a, ch.begin + k, load_part_mx(a, ch.begin + k), fg, fg,
+                                           none, none, dep, false, wabs);
       } else {
         FieldView<double> fg[6];
         for (int f = 0; f < 6; f++)
           fg[f] = FieldView<double>{a.F[f] + g.idx(0, 0, 0), g.NX, g.NX * g.NY};
         for (int k = threadIdx.x; k < ch.n; k += BS)
-          gpd_push<N, LO, PUSHER, DEPO>(a, ch.begin + k, fg, fg, none, none, dep, false, wabs);
+          gpd_push<N, LO, PUSHER, DEPO>(a, ch.begin + k, load_part64(a, ch.begin + k), fg, fg, none,
+                                        none, dep, false, wabs);
       }
     }
     __syncthreads();
