@@ -455,7 +455,13 @@ int pxrhip_tiles_depose_rho(const pxrhip_grid *grid, const pxrhip_species *sp,
  * "fused.j_lds_bytes" (largest current block accumulated in LDS; negative:
  * all of fused.lds_bytes), "fused.jpad" (0/1, odd row and plane pitches of
  * the LDS current block), "fused.margin_pct" (motion margin, in 1/100 cell,
- * of the current block estimated when work items are cut), "mixed.jacc"
+ * of the current block estimated when work items are cut), "mixed.jmargin"
+ * (nodes of motion room per side that the work items of a mixed-precision
+ * species with the fixed-point current leave around the current block of
+ * their particles' old positions: 0, the default, or 1; a work item whose
+ * pushed particles outgrow its LDS block is then deposited by a second
+ * kernel through a block cut down to fit, the particles outside it adding
+ * into global memory; other species keep 1), "mixed.jacc"
  * (LDS current of a mixed-precision species: 0 FP32 atomics, 1 FP32
  * atomics compensated by TwoSum into a second array, 2 FP64 atomics, 3 the
  * default: 64-bit fixed point), "fused.jacc" (LDS current of an FP64
@@ -485,7 +491,8 @@ int pxrhip_set_tuning(const char *key, int64_t value);
 int pxrhip_get_tuning(const char *key, int64_t *value);
 
 /* Work items of the last pxrhip_tiles_gather_push_deposit and how many of
- * them did not fit in LDS and ran from global memory (synchronises). */
+ * them did not fit in LDS: run from global memory, or deposited through a
+ * cut-down current block (synchronises). */
 int pxrhip_tiles_stats(int64_t *nitems, int64_t *noverflow);
 
 /* Reads the device-side error counter of the tiled path (particles found

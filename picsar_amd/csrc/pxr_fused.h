@@ -127,6 +127,12 @@ struct FusedArgs {
   int phase;             // PHASE_PUSH | PHASE_DEPOSIT
   double fxb;            // bound of |current contribution| per unit |q w| (fixed point)
   const int *order;      // work items in the order the queue hands them out (null: as built)
+  // Fixed-point current of a mixed-precision species (defer = 1): a work
+  // item whose current box does not fit is pushed by k_fused and deposited
+  // by k_fused_cut through a block cut down to fit; cutbox keeps, per work
+  // item, the box (R0[3], e[3]) and the largest fixed-point weight (2 ints)
+  int defer;
+  int *cutbox;
 };
 
 // phases of k_fused: gather + push + position push, current deposition
@@ -305,6 +311,20 @@ PXR_DEV void gpd_box(const FusedArgs &a, i64 p, Box6 &dep, double &wabs) {
   }
 }
 
+// results of a deposit call
+constexpr int DEP_OK = 0;   // deposited
+constexpr int DEP_BAD = 1;  // the particle left the arrays: skipped (the caller counts it)
+constexpr int DEP_OUT = 2;  // its node box is not on the side of the block SEL asks for
+
+// whether node box lo..hi is on the side of block (R0, e) that SEL selects:
+// 0 either (no block), 1 inside, 2 not inside
+template <int SEL>
+PXR_DEV bool sel_block(const int *lo, const int *hi, const int *R0, const int *e) {
+  if (SEL == 0) return true;
+  const bool in = in_block(lo, hi, R0, e);
+  return SEL == 1 ? in : !in;
+}
+
 // Deposit pushed particle p (its state is read back from memory).  Returns
 // false when the particle left the arrays (counted, skipped).
 template <int N, int DEPO, typename Acc>
@@ -429,8 +449,9 @@ PXR_DEV void gpd_box_mx(const FusedArgs &a, i64 p, Box6 &dep, double &wabs) {
   }
 }
 
-template <int N, int DEPO, typename Acc>
-PXR_DEV bool gpd_deposit_mx(const FusedArgs &a, i64 p, const Acc &acc) {
+template <int N, int DEPO, int SEL = 0, typename Acc>
+PXR_DEV int gpd_deposit_sel_mx(const FusedArgs &a, i64 p, const Acc &acc, const int *R0,
+                               const int *e) {
   const GridGeom &g = a.g;
   const float gi = a.fgi[p];
   const float uf[3] = {a.fux[p], a.fuy[p], a.fuz[p]};
@@ -441,7 +462,8 @@ PXR_DEV bool gpd_deposit_mx(const FusedArgs &a, i64 p, const Acc &acc) {
   int b0[3], b1[3];
   if (!mx_box<N, DEPO>(a, pn, v, b0, b1) || !g.inside(b0[0], b0[1], b0[2]) ||
       !g.inside(b1[0], b1[1], b1[2]))
-    return false;
+    return DEP_BAD;
+  if (!sel_block<SEL>(b0, b1, R0, e)) return DEP_OUT;
   const float wq = (float)(a.q * (double)a.fw[p]);
   if (DEPO == 0) {
     EAxisMx<N> X, Y, Z;
@@ -458,7 +480,33 @@ PXR_DEV bool gpd_deposit_mx(const FusedArgs &a, i64 p, const Acc &acc) {
     direct_prepare_mx<N>(in, pn, vf, h, (float)(a.q * (double)a.fw[p] * invvol), a.s);
     direct_deposit<N, float>(in, acc);
   }
-  return true;
+  return DEP_OK;
+}
+
+// Deposit pushed mixed-precision particle p.  Returns false when the particle
+// left the arrays (counted, skipped).
+template <int N, int DEPO, typename Acc>
+PXR_DEV bool gpd_deposit_mx(const FusedArgs &a, i64 p, const Acc &acc) {
+  return gpd_deposit_sel_mx<N, DEPO>(a, p, acc, nullptr, nullptr) == DEP_OK;
+}
+
+// Fixed-point scale of a work item of n particles whose largest fx_weight
+// has the bits wbits: a power of two under which every contribution is
+// below 2^50 in magnitude and a node's sum of at most n of them stays below
+// 2^62 (the bound is a.fxb times that weight, so the quantum follows the
+// size of the item's actual contributions).
+PXR_DEV void fx_scale(const FusedArgs &a, unsigned long long wbits, int n, double &fxs,
+                      double &fxs_inv) {
+  fxs = fxs_inv = 1.0;
+  const double bound = a.fxb * __longlong_as_double((long long)wbits);
+  if (bound > 0.0) {
+    int eb;
+    frexp(bound, &eb);  // bound < 2^eb
+    const int en = n > 1 ? 32 - __clz(n - 1) : 0;  // n <= 2^en
+    const int k = min(50 - eb, 62 - eb - en);
+    fxs = ldexp(1.0, k);
+    fxs_inv = ldexp(1.0, -k);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -662,23 +710,9 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(
       if (JACC == 3) atomicMax(&s_wabs, (unsigned long long)__double_as_longlong(wabs));
     }
     __syncthreads();
-    // fixed point (JACC 3): a power-of-two scale under which every
-    // contribution of the work item is below 2^50 in magnitude and a node's
-    // sum of at most ch.n of them stays below 2^62; the bound is a.fxb times
-    // the largest fx_weight of the work item, so the quantum follows the
-    // size of the item's actual contributions
+    // fixed point (JACC 3): the scale of the work item
     double fxs = 1.0, fxs_inv = 1.0;
-    if constexpr (JACC == 3) {
-      const double bound = a.fxb * __longlong_as_double((long long)s_wabs);
-      if (bound > 0.0) {
-        int eb;
-        frexp(bound, &eb);  // bound < 2^eb
-        const int en = ch.n > 1 ? 32 - __clz(ch.n - 1) : 0;  // ch.n <= 2^en
-        const int k = min(50 - eb, 62 - eb - en);
-        fxs = ldexp(1.0, k);
-        fxs_inv = ldexp(1.0, -k);
-      }
-    }
+    if constexpr (JACC == 3) fx_scale(a, s_wabs, ch.n, fxs, fxs_inv);
     int R0[3], e[3];
     for (int d = 0; d < 3; d++) {
       R0[d] = max(s_box[d], glo[d]);
@@ -693,10 +727,21 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(
     const int narr = (MX && JACC == 1) ? 6 : 3;  // with the compensation arrays
     const i64 ne = (i64)e[0] * e[1] * e[2];
     if (ne <= 0 || narr * nj64 * (i64)sizeof(JT) > a.j_bytes) {
-      // pushed; the deposition is left to the overflow kernel
+      // pushed; the deposition is left to k_fused_cut (stage 3: a block cut
+      // down to fit) or to the overflow kernel (stage 1: global memory)
       if (threadIdx.x == 0 && ne > 0) {
+        const bool cut = JACC == 3 && MX && a.defer && ch.z >= 0;
         const int slot = atomicAdd(a.ovf_count, 1);
-        if (slot < a.ovf_cap) a.ovf[slot] = make_int2(c, 1);
+        if (slot < a.ovf_cap) a.ovf[slot] = make_int2(c, cut ? 3 : 1);
+        if (cut) {
+          int *cb = a.cutbox + 8 * (i64)c;
+          for (int d = 0; d < 3; d++) {
+            cb[d] = R0[d];
+            cb[3 + d] = e[d];
+          }
+          cb[6] = (int)(s_wabs & 0xffffffffull);
+          cb[7] = (int)(s_wabs >> 32);
+        }
       }
       __syncthreads();
       continue;
@@ -771,6 +816,7 @@ __global__ __launch_bounds__(BS) void k_fused_overflow(FusedArgs a) {
   const int none[3] = {0, 0, 0};  // empty block: every gather reads global memory
   for (int o = blockIdx.x; o < n; o += gridDim.x) {
     const int2 ov = a.ovf[o];
+    if (ov.y == 3) continue;  // k_fused_cut
     const Chunk ch = a.chunks[ov.x];
     if (ov.y == 0 && (a.phase & PHASE_PUSH)) {
       Box6 dep;
@@ -801,6 +847,112 @@ __global__ __launch_bounds__(BS) void k_fused_overflow(FusedArgs a) {
       if constexpr (MX) ok = gpd_deposit_mx<N, DEPO>(a, ch.begin + k, GAccF{{a.J[0], a.J[1], a.J[2]}, g});
       else ok = gpd_deposit<N, DEPO>(a, ch.begin + k, GAcc{{a.J[0], a.J[1], a.J[2]}, g});
       if (!ok) count_bad(a.bad);
+    }
+    __syncthreads();
+  }
+}
+
+// Work items of a mixed-precision species whose current box did not fit in
+// LDS (stage 3: pushed by k_fused).  The current block is cut down to the
+// block of the particles' old positions, as the work-item builder sized it,
+// grown towards the box of the pushed particles while it fits; the
+// particles whose box leaves it add into global memory.  The fixed-point
+// scale is the one k_fused found for the work item.  One workgroup per work
+// item.
+template <int BS, int N, bool LO, int PUSHER, int DEPO>
+__global__ __launch_bounds__(BS) void k_fused_cut(FusedArgs a) {
+  extern __shared__ double smem[];
+  typedef unsigned long long JT;
+  const int n = min(*a.ovf_count, a.ovf_cap);
+  const GridGeom &g = a.g;
+  const int glo[3] = {-g.gx, -g.gy, -g.gz};
+  const int ghi[3] = {g.nx + g.gx, g.ny + g.gy, g.nz + g.gz};
+  auto layout = [&](const int *ee, int &py_, int &pz_) -> i64 {
+    py_ = a.jpad ? (ee[0] | 1) : ee[0];
+    const int pzr = py_ * ee[1];
+    pz_ = a.jpad ? (pzr | 1) : pzr;
+    return (i64)pz_ * ee[2];
+  };
+  for (int o = blockIdx.x; o < n; o += gridDim.x) {
+    const int2 ov = a.ovf[o];
+    if (ov.y != 3) continue;
+    const Chunk ch = a.chunks[ov.x];
+    const int *cb = a.cutbox + 8 * (i64)ov.x;
+    const int B0[3] = {cb[0], cb[1], cb[2]}, Be[3] = {cb[3], cb[4], cb[5]};
+    const unsigned long long wbits =
+        (unsigned long long)(unsigned)cb[6] | ((unsigned long long)(unsigned)cb[7] << 32);
+    // the old positions' block, intersected with the box of the particles
+    double ulo[3], uhi[3];
+    rows_range(a.bg, ch.tile, ch.z, ch.z1, ch.y0, ch.y1, ulo, uhi);
+    int lo[3], hi[3], ee[3];
+    bool any = true;
+    for (int d = 0; d < 3; d++) {
+      int l, h;
+      deposit_range(a.bg, ulo[d], uhi[d], a.bg.mv[d], l, h);
+      lo[d] = max(max(l, glo[d]), B0[d]);
+      hi[d] = min(min(h, ghi[d]), B0[d] + Be[d] - 1);
+      any = any && hi[d] >= lo[d];
+    }
+    auto fits = [&]() {
+      int q0, q1;
+      for (int d = 0; d < 3; d++) ee[d] = hi[d] - lo[d] + 1;
+      return 3 * layout(ee, q0, q1) * (i64)sizeof(JT) <= a.j_bytes;
+    };
+    const GAccF ga{{a.J[0], a.J[1], a.J[2]}, g};
+    if (!(any && fits())) {
+      // no block fits: every particle into global memory
+      for (int k = threadIdx.x; k < ch.n; k += BS)
+        if (!gpd_deposit_mx<N, DEPO>(a, ch.begin + k, ga)) count_bad(a.bad);
+      continue;
+    }
+    for (int it = 0; it < 4; it++) {
+      bool grown = false;
+      for (int d = 0; d < 3; d++) {
+        if (lo[d] > B0[d]) {
+          lo[d]--;
+          if (fits()) grown = true;
+          else lo[d]++;
+        }
+        if (hi[d] < B0[d] + Be[d] - 1) {
+          hi[d]++;
+          if (fits()) grown = true;
+          else hi[d]--;
+        }
+      }
+      if (!grown) break;
+    }
+    int R0[3], e[3];
+    for (int d = 0; d < 3; d++) {
+      R0[d] = lo[d];
+      e[d] = hi[d] - lo[d] + 1;
+    }
+    int py, pz;
+    const int nj = (int)layout(e, py, pz);
+    double fxs, fxs_inv;
+    fx_scale(a, wbits, ch.n, fxs, fxs_inv);
+    JT *sj = reinterpret_cast<JT *>(smem);
+    for (int idx = threadIdx.x; idx < 3 * nj; idx += BS) sj[idx] = 0ull;
+    __syncthreads();
+    const LdsAccMx<3> la{{sj, sj + nj, sj + 2 * nj}, {nullptr, nullptr, nullptr},
+                         R0[0], R0[1], R0[2], py, pz, fxs};
+    for (int k = threadIdx.x; k < ch.n; k += BS) {
+      const i64 p = ch.begin + k;
+      const int r = gpd_deposit_sel_mx<N, DEPO, 1>(a, p, la, R0, e);
+      if (r == DEP_BAD) count_bad(a.bad);
+      else if (r == DEP_OUT) gpd_deposit_mx<N, DEPO>(a, p, ga);
+    }
+    __syncthreads();
+    const int ne = e[0] * e[1] * e[2];
+    for (int idx = threadIdx.x; idx < ne; idx += BS) {
+      const int i = idx % e[0];
+      const int j = (idx / e[0]) % e[1];
+      const int k = idx / (e[0] * e[1]);
+      const i64 gi_ = g.idx(R0[0] + i, R0[1] + j, R0[2] + k);
+      const int li = i + py * j + pz * k;
+      for (int c3 = 0; c3 < 3; c3++) {
+        const double v = (double)(long long)sj[c3 * nj + li] * fxs_inv;
+        if (v != 0.0) unsafeAtomicAdd(a.J[c3] + gi_, v);
+      }
     }
     __syncthreads();
   }

@@ -1,6 +1,7 @@
 // pxr_fused_mxfx.hip -- mixed-precision instantiations of the fused gather + push +
-// deposit kernel, LDS current accumulated in 64-bit fixed point (compiled apart
-// from the other variants to shorten the build).
+// deposit kernel, LDS current accumulated in 64-bit fixed point, with its
+// cut-block deposition kernel (compiled apart from the other variants to
+// shorten the build).
 #include "pxr_fused.h"
 #include "pxr_runtime.h"
 
@@ -17,6 +18,12 @@ static int launch(int key, const FusedArgs &a, int nwg, size_t lds, hipStream_t 
       PXR_HIP(hipFuncSetAttribute((const void *)&k_fused<FUSED_BS, N_, LO_, P_, D_, true, JACC>, \
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));       \
     hipLaunchKernelGGL((k_fused<FUSED_BS, N_, LO_, P_, D_, true, JACC>), grd, blk, lds, st, a);  \
+    if (a.defer) {                                                                               \
+      if (lds > 65536)                                                                           \
+        PXR_HIP(hipFuncSetAttribute((const void *)&k_fused_cut<FUSED_BS, N_, LO_, P_, D_>,       \
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));     \
+      hipLaunchKernelGGL((k_fused_cut<FUSED_BS, N_, LO_, P_, D_>), grd, blk, lds, st, a);        \
+    }                                                                                            \
     hipLaunchKernelGGL((k_fused_overflow<FUSED_BS, N_, LO_, P_, D_, true>), dim3(2 * rt().num_cus), \
                        blk, 0, st, a);                                                           \
     break;

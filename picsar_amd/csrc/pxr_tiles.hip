@@ -649,6 +649,7 @@ struct ChunkArgs {
   int *nchunks, *queue;
   i64 maxchunks;    // bound on the chunks of a launch; beyond it ...
   unsigned long long *bad;  // ... chunks would be dropped, and are counted here
+  int jroom;                // motion room of the current block, nodes per side
 };
 
 // plane roles in ChunkArgs::group (a value >= 0 is the last plane of the
@@ -664,10 +665,11 @@ PXR_DEV bool blocks_fit(const ChunkArgs &a, int t, int z0, int z1, int y0, int y
   int FR0[3], fe[3], JR0[3], je[3];
   rows_range(a.bg, t, z0, z1, y0, y1, ulo, uhi);
   blocks_for(a.bg, ulo, uhi, FR0, fe, JR0, je);
-  // current block, with room for one node of motion on each side (the
-  // block of k_fused is the box of the pushed particles) and for its odd
+  // current block, with room for jroom nodes of motion on each side (the
+  // block of k_fused is the box of the pushed particles; for a species whose
+  // blocks k_fused_cut can cut down no room is needed) and for its odd
   // pitches
-  const int j0 = je[0] + 2, j1 = je[1] + 2, j2 = je[2] + 2;
+  const int j0 = je[0] + 2 * a.jroom, j1 = je[1] + 2 * a.jroom, j2 = je[2] + 2 * a.jroom;
   const i64 jpz = ((i64)(j0 | 1) * j1) | 1;
   return a.fnode_bytes * (i64)fe[0] * fe[1] * fe[2] <= a.lds_bytes &&
          a.jnode_bytes * jpz * j2 <= a.j_bytes;
@@ -888,6 +890,11 @@ struct Tuning {
   // so three workgroups with smaller LDS blocks fit on a CU
   int mx_lds_bytes = LDS_THREE_WG;
   int mx_wg_per_cu = 0;
+  // motion room (nodes per side) that the work items of a mixed-precision
+  // species with a fixed-point current leave around the current block of
+  // their particles' old positions: 0 by default, since k_fused_cut deposits
+  // the work items whose particles outgrow it; other species keep 1
+  int jmargin = 0;
 };
 static Tuning g_tune;
 // the launch chosen by the last pxrhip_tiles_gather_push_deposit
@@ -996,7 +1003,8 @@ __global__ __launch_bounds__(256) void k_rho_tiles(RhoTileArgs a) {
 // and its counters live in the library's scratch slots 8-10 and 12-15.
 static int build_chunks(const TileTable *tt, const pxrhip_species *sp, const BoxGeom &bg, int ch,
                         i64 lds_bytes, i64 j_bytes, int fnode, int jnode, hipStream_t st,
-                        Chunk **chunks_out, int **ctl_out, int2 **ovf_out, i64 *maxchunks_out) {
+                        Chunk **chunks_out, int **ctl_out, int2 **ovf_out, i64 *maxchunks_out,
+                        int jroom = 1) {
   const i64 cap = sp->capacity;
   const int nzmax = tt->nzmax;
   const i64 nunits = (i64)tt->ntiles * nzmax;
@@ -1036,6 +1044,7 @@ static int build_chunks(const TileTable *tt, const pxrhip_species *sp, const Box
   ca.queue = ctl + 1;
   ca.maxchunks = maxchunks;
   ca.bad = rt().d_bad;
+  ca.jroom = jroom;
   hipLaunchKernelGGL(k_chunk_groups, dim3(grid1d(tt->ntiles, 64)), dim3(64), 0, st, ca);
   hipLaunchKernelGGL(k_chunk_count, dim3(grid1d(nunits, 256)), dim3(256), 0, st, ca);
   hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nblk), dim3(256), 0, st, cnt, nunits, bsum);
@@ -1112,6 +1121,9 @@ int pxrhip_set_tuning(const char *key, int64_t value) {
   } else if (!strcmp(key, "fused.chunk")) {
     if (value < 1 || value > (1 << 20)) return set_error(PXRHIP_ERR_ARG, "fused.chunk must be in [1, 2^20]");
     g_tune.chunk = (int)value;
+  } else if (!strcmp(key, "mixed.jmargin")) {
+    if (value < 0 || value > 1) return set_error(PXRHIP_ERR_ARG, "mixed.jmargin must be 0 or 1");
+    g_tune.jmargin = (int)value;
   } else {
     return set_error(PXRHIP_ERR_ARG, "unknown tuning key '%s'", key);
   }
@@ -1124,6 +1136,7 @@ int pxrhip_get_tuning(const char *key, int64_t *value) {
   else if (!strcmp(key, "fused.wg_per_cu")) *value = g_tune.wg_per_cu;
   else if (!strcmp(key, "fused.spread")) *value = g_tune.spread;
   else if (!strcmp(key, "fused.chunk")) *value = g_tune.chunk;
+  else if (!strcmp(key, "mixed.jmargin")) *value = g_tune.jmargin;
   else if (!strcmp(key, "fused.j_lds_bytes")) *value = g_tune.j_lds_bytes;
   else if (!strcmp(key, "fused.margin_pct")) *value = g_tune.margin_pct;
   else if (!strcmp(key, "fused.jpad")) *value = g_tune.jpad;
@@ -1458,8 +1471,11 @@ static int tiles_fused(const pxrhip_grid *grid, const pxrhip_species *sp, const 
   // blocks that the work items' phases stage: a phase left out needs no LDS
   const int fnode = (phase & PHASE_PUSH) ? 6 * esize : 0;
   const int jnode_b = (phase & PHASE_DEPOSIT) ? jnode : 0;
+  // a mixed-precision species with a fixed-point current: work items whose
+  // current outgrows its LDS block are deposited by k_fused_cut
+  const bool defer = mx && jacc == 3 && (phase & PHASE_DEPOSIT);
   if ((r = build_chunks(tt, sp, bg, ch, lds_bytes, j_bytes, fnode, jnode_b, st, &chunks, &ctl,
-                        &ovf, &maxchunks)))
+                        &ovf, &maxchunks, defer ? g_tune.jmargin : 1)))
     return r;
   // fused kernel
   FusedArgs a;
@@ -1505,6 +1521,12 @@ static int tiles_fused(const pxrhip_grid *grid, const pxrhip_species *sp, const 
   a.ovf_count = ctl + 2;
   a.ovf = ovf;
   a.ovf_cap = maxchunks < INT_MAX ? (int)maxchunks : INT_MAX;
+  a.defer = defer;
+  a.cutbox = nullptr;
+  if (defer) {
+    a.cutbox = (int *)scratch(31, sizeof(int) * 8 * (size_t)maxchunks);
+    if (!a.cutbox) return set_error(PXRHIP_ERR_HIP, "scratch allocation failed");
+  }
   a.order = nullptr;
   if (g_tune.order) {
     int *bucket = (int *)scratch(20, sizeof(int) * 32);

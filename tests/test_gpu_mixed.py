@@ -122,6 +122,73 @@ def test_relativistic_step_against_fp64_oracle(torch, orc, order, depo):
             assert v <= 4e-7, (k, v)
 
 
+@pytest.mark.parametrize("order,depo", [(1, "direct"), (2, "esirkepov"), (3, "esirkepov")])
+def test_cut_current_blocks_against_fp64_oracle(torch, orc, order, depo):
+    """Work items built with no motion room around the current block of
+    their particles' old positions (mixed.jmargin 0, the default) under a
+    16 KB cap on the LDS current block, and two gather+push+deposit passes
+    without re-binning, so that in the second the particles (u_th = 0.6 c)
+    have left the sorting cells their work items were built from: the
+    current boxes that no longer fit are deposited by k_fused_cut through a
+    block cut down around the old positions, and the particles whose box
+    leaves it through global memory.  J is the oracle's at 1e-5 (the oracle
+    is given the FP32-rounded momenta and weights)."""
+    from picsar_amd import _lib
+    old_m = set_tuning("mixed.jmargin", 0)
+    old_j = set_tuning("fused.j_lds_bytes", 16384)
+    cut = 0
+    try:
+        sim, ref, _ = mixed_build(torch, orc, order, True, "vay", depo, ppc=2, uth=(0.6, 0.6),
+                                  seed=11)
+        unique_float32_weights(ref)
+        for sp, rs in zip(sim.species, ref.species):
+            for k in ("ux", "uy", "uz", "w"):
+                rs[k][...] = rs[k].astype(np.float32).astype(np.float64)
+            rs["gaminv"][...] = 1.0 / np.sqrt(1 + rs["ux"] ** 2 + rs["uy"] ** 2 + rs["uz"] ** 2)
+        sim.gather_push_deposit()
+        for sp in sim.species:
+            sim.gather_push_deposit_species(sp)  # J accumulates
+            ni, no = C.c_int64(0), C.c_int64(0)
+            _lib.call("pxrhip_tiles_stats", C.byref(ni), C.byref(no))
+            cut += no.value
+        sim.check()
+    finally:
+        set_tuning("mixed.jmargin", old_m)
+        set_tuning("fused.j_lds_bytes", old_j)
+    ref.gather_push_deposit(zero_j=True)
+    ref.gather_push_deposit(zero_j=False)
+    assert cut > 0, "no work item was cut"
+    err = errors(sim, ref)
+    for k in ("jx", "jy", "jz"):
+        assert err[k] <= MIXED_TOL, (k, err[k])
+    for k, v in err.items():
+        if k.endswith(("ux", "uy", "uz")):
+            assert v <= MIXED_TOL, (k, v)
+
+
+@pytest.mark.parametrize("jmargin", [0, 1])
+def test_motion_room_settings_give_the_same_steps(torch, orc, jmargin):
+    """Three steps of a fast and a slow species with either motion room
+    around the work items' current blocks (mixed.jmargin 0, the default, or
+    1 node per side) stay within the ten-step bound of the oracle."""
+    old_m = set_tuning("mixed.jmargin", jmargin)
+    try:
+        sim, ref, _ = mixed_build(torch, orc, 3, True, "vay", "esirkepov", ppc=2,
+                                  uth=(0.6, 0.02), seed=5)
+        for sp, rs in zip(sim.species, ref.species):
+            for k in ("ux", "uy", "uz", "w"):
+                rs[k][...] = rs[k].astype(np.float32).astype(np.float64)
+            rs["gaminv"][...] = 1.0 / np.sqrt(1 + rs["ux"] ** 2 + rs["uy"] ** 2 + rs["uz"] ** 2)
+        for _ in range(3):
+            sim.step(1)
+            ref.step()
+    finally:
+        set_tuning("mixed.jmargin", old_m)
+    err = errors(sim, ref)
+    for k in ("jx", "jy", "jz", "ex", "ey", "ez", "bx", "by", "bz"):
+        assert err[k] <= 1e-4, (k, err[k])
+
+
 def test_ten_steps_against_fp64_oracle(torch, orc):
     sim, ref, _ = mixed_build(torch, orc, 3, True, "vay", "esirkepov", ppc=2)
     for sp, rs in zip(sim.species, ref.species):
