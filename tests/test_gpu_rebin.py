@@ -150,3 +150,52 @@ def test_arrival_order_rebin_keeps_the_cells(torch, orc):
         kb = np.lexsort((b["w"], cell))
         for k in a:
             np.testing.assert_array_equal(a[k][ka], b[k][kb], err_msg=k)
+
+
+def test_tile_index_on_and_next_to_tile_faces(torch, orc):
+    """Positions on the tile faces (x0 + (i n0 - 1/2) dx) and 1 to 4 ulps to
+    either side, where the tile index of particle_boundaries.F90:397 turns
+    on the rounding of its division, with random positions on the other
+    axes: the re-binning puts each particle in the tile and cell the
+    reference's formula gives (the division done on the host by numpy)."""
+    sim, ref, tng = build(torch, orc, 3, True, "vay", "esirkepov", ppc=1,
+                          d=(1.0, 0.9, 1.1))
+    rng = np.random.default_rng(3)
+    g = sim.grid
+    x0 = (sim.xmin, sim.ymin, sim.zmin)
+    d = (sim.dx, sim.dy, sim.dz)
+    n = (sim.nx, sim.ny, sim.nz)
+    nt = (g.ntx, g.nty, g.ntz)
+    cols = []
+    for a in range(3):
+        n0 = (n[a] + 1) // nt[a]
+        faces = [x0[a] + (i * n0 - 0.5) * d[a] for i in range(1, nt[a])]
+        vals = []
+        for f in faces:
+            v = f
+            vals.append(v)
+            for direction in (np.inf, -np.inf):
+                v = f
+                for _ in range(4):
+                    v = np.nextafter(v, direction)
+                    vals.append(v)
+        vals = np.array(vals)
+        k = len(vals)
+        P = [x0[b] + rng.uniform(0, n[b], k) * d[b] for b in range(3)]
+        P[a] = vals
+        cols.append(P)
+    x, y, z = (np.concatenate([c[b] for c in cols]) for b in range(3))
+    m = len(x)
+    u = np.zeros(m)
+    w = 1.0 + np.arange(m) * 1e-9
+    sp = sim.add_species("faces", -1.0, 1.0, x, y, z, u, u, u, w)
+    bins, nbins = host_bins(sim, x, y, z)
+    order = np.argsort(bins, kind="stable")
+    dst = sp.arrays()
+    np.testing.assert_array_equal(dst["x"], x[order])
+    np.testing.assert_array_equal(dst["y"], y[order])
+    np.testing.assert_array_equal(dst["z"], z[order])
+    bs = sp.active.bin_start.numpy()
+    np.testing.assert_array_equal(bs, np.concatenate(
+        [[0], np.cumsum(np.bincount(bins, minlength=nbins))]))
+    sim.check()
