@@ -458,10 +458,11 @@ int pxrhip_tiles_depose_rho(const pxrhip_grid *grid, const pxrhip_species *sp,
  * of the current block estimated when work items are cut), "mixed.jmargin"
  * (nodes of motion room per side that the work items of a mixed-precision
  * species with the fixed-point current leave around the current block of
- * their particles' old positions: 0, the default, or 1; a work item whose
- * pushed particles outgrow its LDS block is then deposited by a second
- * kernel through a block cut down to fit, the particles outside it adding
- * into global memory; other species keep 1), "mixed.jacc"
+ * their particles' old positions: -1, the default, takes 0 with the
+ * Esirkepov deposition and 1 with the direct one; a work item whose pushed
+ * particles outgrow its LDS block is deposited by a second kernel through a
+ * block cut down to fit, the particles outside it adding into global
+ * memory; other species keep 1), "mixed.jacc"
  * (LDS current of a mixed-precision species: 0 FP32 atomics, 1 FP32
  * atomics compensated by TwoSum into a second array, 2 FP64 atomics, 3 the
  * default: 64-bit fixed point), "fused.jacc" (LDS current of an FP64

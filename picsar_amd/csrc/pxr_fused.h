@@ -127,16 +127,13 @@ struct FusedArgs {
   int phase;             // PHASE_PUSH | PHASE_DEPOSIT
   double fxb;            // bound of |current contribution| per unit |q w| (fixed point)
   const int *order;      // work items in the order the queue hands them out (null: as built)
-  // Fixed-point current of a mixed-precision species (defer = 1): a work
-  // item whose current box does not fit is pushed by k_fused and deposited
-  // by k_fused_cut through a block cut down to fit; cutbox keeps, per work
-  // item, the box (R0[3], e[3]) and the largest fixed-point weight (2 ints)
-  int defer;
-  int *cutbox;
 };
 
-// phases of k_fused: gather + push + position push, current deposition
-constexpr int PHASE_PUSH = 1, PHASE_DEPOSIT = 2;
+// phases of k_fused: gather + push + position push, current deposition.
+// PHASE_CUT (fixed-point current of a mixed-precision species): a work item
+// whose current box does not fit is pushed by k_fused and deposited by
+// k_fused_cut through a block cut down to fit.
+constexpr int PHASE_PUSH = 1, PHASE_DEPOSIT = 2, PHASE_CUT = 4;
 
 struct Box6 {
   int lo[3], hi[3];
@@ -730,18 +727,9 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(
       // pushed; the deposition is left to k_fused_cut (stage 3: a block cut
       // down to fit) or to the overflow kernel (stage 1: global memory)
       if (threadIdx.x == 0 && ne > 0) {
-        const bool cut = JACC == 3 && MX && a.defer && ch.z >= 0;
+        const bool cut = JACC == 3 && MX && (a.phase & PHASE_CUT) && ch.z >= 0;
         const int slot = atomicAdd(a.ovf_count, 1);
         if (slot < a.ovf_cap) a.ovf[slot] = make_int2(c, cut ? 3 : 1);
-        if (cut) {
-          int *cb = a.cutbox + 8 * (i64)c;
-          for (int d = 0; d < 3; d++) {
-            cb[d] = R0[d];
-            cb[3 + d] = e[d];
-          }
-          cb[6] = (int)(s_wabs & 0xffffffffull);
-          cb[7] = (int)(s_wabs >> 32);
-        }
       }
       __syncthreads();
       continue;
@@ -853,15 +841,17 @@ __global__ __launch_bounds__(BS) void k_fused_overflow(FusedArgs a) {
 }
 
 // Work items of a mixed-precision species whose current box did not fit in
-// LDS (stage 3: pushed by k_fused).  The current block is cut down to the
-// block of the particles' old positions, as the work-item builder sized it,
-// grown towards the box of the pushed particles while it fits; the
-// particles whose box leaves it add into global memory.  The fixed-point
-// scale is the one k_fused found for the work item.  One workgroup per work
-// item.
+// LDS (stage 3: pushed by k_fused).  The box of the pushed particles and
+// the fixed-point scale are found again as k_fused found them, in a pass
+// over the particles.  The current block is cut down to the block of the
+// particles' old positions, as the work-item builder sized it, grown
+// towards that box while it fits; the particles whose box leaves it add
+// into global memory.  One workgroup per work item.
 template <int BS, int N, bool LO, int PUSHER, int DEPO>
 __global__ __launch_bounds__(BS) void k_fused_cut(FusedArgs a) {
   extern __shared__ double smem[];
+  __shared__ int s_box[6];
+  __shared__ unsigned long long s_wabs;
   typedef unsigned long long JT;
   const int n = min(*a.ovf_count, a.ovf_cap);
   const GridGeom &g = a.g;
@@ -877,10 +867,39 @@ __global__ __launch_bounds__(BS) void k_fused_cut(FusedArgs a) {
     const int2 ov = a.ovf[o];
     if (ov.y != 3) continue;
     const Chunk ch = a.chunks[ov.x];
-    const int *cb = a.cutbox + 8 * (i64)ov.x;
-    const int B0[3] = {cb[0], cb[1], cb[2]}, Be[3] = {cb[3], cb[4], cb[5]};
-    const unsigned long long wbits =
-        (unsigned long long)(unsigned)cb[6] | ((unsigned long long)(unsigned)cb[7] << 32);
+    // the box of the currents and the largest fixed-point weight
+    if (threadIdx.x == 0) {
+      for (int d = 0; d < 3; d++) {
+        s_box[d] = INT_MAX;
+        s_box[3 + d] = INT_MIN;
+      }
+      s_wabs = 0ull;
+    }
+    __syncthreads();
+    {
+      Box6 dep;
+      box_reset(dep);
+      double wabs = 0.0;
+      for (int k = threadIdx.x; k < ch.n; k += BS) gpd_box_mx<N, DEPO>(a, ch.begin + k, dep, wabs);
+      wave_minmax(dep);
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) wabs = fmax(wabs, __shfl_xor(wabs, o));
+      if ((threadIdx.x & 63) == 0) {
+        for (int d = 0; d < 3; d++) {
+          atomicMin(&s_box[d], dep.lo[d]);
+          atomicMax(&s_box[3 + d], dep.hi[d]);
+        }
+        atomicMax(&s_wabs, (unsigned long long)__double_as_longlong(wabs));
+      }
+    }
+    __syncthreads();
+    int B0[3], Be[3];
+    for (int d = 0; d < 3; d++) {
+      B0[d] = max(s_box[d], glo[d]);
+      Be[d] = min(s_box[3 + d], ghi[d]) - B0[d] + 1;
+    }
+    const unsigned long long wbits = s_wabs;
+    __syncthreads();  // s_box and s_wabs are read before the next work item resets them
     // the old positions' block, intersected with the box of the particles
     double ulo[3], uhi[3];
     rows_range(a.bg, ch.tile, ch.z, ch.z1, ch.y0, ch.y1, ulo, uhi);

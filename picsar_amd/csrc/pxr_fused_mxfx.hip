@@ -18,7 +18,7 @@ static int launch(int key, const FusedArgs &a, int nwg, size_t lds, hipStream_t 
       PXR_HIP(hipFuncSetAttribute((const void *)&k_fused<FUSED_BS, N_, LO_, P_, D_, true, JACC>, \
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));       \
     hipLaunchKernelGGL((k_fused<FUSED_BS, N_, LO_, P_, D_, true, JACC>), grd, blk, lds, st, a);  \
-    if (a.defer) {                                                                               \
+    if (a.phase & PHASE_CUT) {                                                                   \
       if (lds > 65536)                                                                           \
         PXR_HIP(hipFuncSetAttribute((const void *)&k_fused_cut<FUSED_BS, N_, LO_, P_, D_>,       \
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));     \

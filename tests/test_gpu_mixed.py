@@ -125,7 +125,7 @@ def test_relativistic_step_against_fp64_oracle(torch, orc, order, depo):
 @pytest.mark.parametrize("order,depo", [(1, "direct"), (2, "esirkepov"), (3, "esirkepov")])
 def test_cut_current_blocks_against_fp64_oracle(torch, orc, order, depo):
     """Work items built with no motion room around the current block of
-    their particles' old positions (mixed.jmargin 0, the default) under a
+    their particles' old positions (mixed.jmargin 0) under a
     16 KB cap on the LDS current block, and two gather+push+deposit passes
     without re-binning, so that in the second the particles (u_th = 0.6 c)
     have left the sorting cells their work items were built from: the
@@ -166,11 +166,12 @@ def test_cut_current_blocks_against_fp64_oracle(torch, orc, order, depo):
             assert v <= MIXED_TOL, (k, v)
 
 
-@pytest.mark.parametrize("jmargin", [0, 1])
+@pytest.mark.parametrize("jmargin", [0, 1, -1])
 def test_motion_room_settings_give_the_same_steps(torch, orc, jmargin):
-    """Three steps of a fast and a slow species with either motion room
-    around the work items' current blocks (mixed.jmargin 0, the default, or
-    1 node per side) stay within the ten-step bound of the oracle."""
+    """Three steps of a fast and a slow species with each motion room
+    around the work items' current blocks (mixed.jmargin 0, 1 node per side,
+    or -1: automatic, the default) stay within the ten-step bound of the
+    oracle."""
     old_m = set_tuning("mixed.jmargin", jmargin)
     try:
         sim, ref, _ = mixed_build(torch, orc, 3, True, "vay", "esirkepov", ppc=2,
