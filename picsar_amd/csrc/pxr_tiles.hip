@@ -646,16 +646,42 @@ PXR_DEV void order_bin(const i64 *bin_start, int b, int *perm) {
   }
 }
 
-// 2. k_rb_order: one thread per flagged bin sorts the source indices of its
-//    slots (insertion sort: the slots hold a few increasing runs), so that
-//    a bin lists its particles in source order, as the reference's stable
-//    counting sort does (sorting.F90:333-343), whatever order the runs
-//    arrived in.  Bins filled by one run are in order already.
+// 2. k_rb_order: the source indices of each flagged bin's slots are put in
+//    increasing order (insertion sort: the slots hold a few increasing
+//    runs), so that a bin lists its particles in source order, as the
+//    reference's stable counting sort does (sorting.F90:333-343), whatever
+//    order the runs arrived in.  Bins filled by one run are in order
+//    already.  A wave reads the flags of 64 * RB_ORDER_G consecutive bins,
+//    lists the flagged ones in LDS, and then sorts them one per lane, so
+//    that its lanes are busy with flagged bins only (at 256^3, 8 ppc: 0.32
+//    against 0.36 ms per species with one thread per bin).
+constexpr int RB_ORDER_G = 8;
+
 __global__ __launch_bounds__(256) void k_rb_order(const i64 *bin_start, i64 nbins,
                                                   const int *flag, int *perm) {
-  const i64 b = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nbins || !flag[b]) return;
-  order_bin(bin_start, (int)b, perm);
+  __shared__ int s_list[4][64 * RB_ORDER_G];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const i64 b0 = ((i64)blockIdx.x * 4 + wv) * (64 * RB_ORDER_G);
+  int *list = s_list[wv];
+  int fl[RB_ORDER_G];  // all flags loaded before any is used
+#pragma unroll
+  for (int g = 0; g < RB_ORDER_G; g++) {
+    const i64 b = b0 + 64 * g + lane;
+    fl[g] = b < nbins ? flag[b] : 0;
+  }
+  int cnt = 0;  // the same in every lane of the wave
+#pragma unroll
+  for (int g = 0; g < RB_ORDER_G; g++) {
+    const i64 b = b0 + 64 * g + lane;
+    const bool f = fl[g] != 0;
+    const unsigned long long m = __ballot(f);
+    if (f) list[cnt + __popcll(m & ((1ull << lane) - 1ull))] = (int)b;
+    cnt += __popcll(m);
+  }
+  __builtin_amdgcn_wave_barrier();  // the wave's list is written
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  for (int k = lane; k < cnt; k += 64) order_bin(bin_start, list[k], perm);
 }
 
 struct GatherArgs {
@@ -1448,7 +1474,8 @@ int pxrhip_tiles_rebin_sort(const pxrhip_grid *grid, pxrhip_species *src, pxrhip
                        (const int *)key,
                        (const int64_t *)src->tile_off, tt->ntiles, cursor,
                        (const i64 *)dst->bin_start, flag, perm, cap);
-    hipLaunchKernelGGL(k_rb_order, dim3(grid1d(nbins, 256)), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_rb_order, dim3(grid1d((nbins + RB_ORDER_G - 1) / RB_ORDER_G, 256)),
+                       dim3(256), 0, st,
                        (const i64 *)dst->bin_start, nbins, flag, perm);
     GatherArgs ga;
     for (int k = 0; k < 3; k++) {
