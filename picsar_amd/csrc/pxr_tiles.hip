@@ -646,42 +646,19 @@ PXR_DEV void order_bin(const i64 *bin_start, int b, int *perm) {
   }
 }
 
-// 2. k_rb_order: the source indices of each flagged bin's slots are put in
-//    increasing order (insertion sort: the slots hold a few increasing
-//    runs), so that a bin lists its particles in source order, as the
-//    reference's stable counting sort does (sorting.F90:333-343), whatever
-//    order the runs arrived in.  Bins filled by one run are in order
-//    already.  A wave reads the flags of 64 * RB_ORDER_G consecutive bins,
-//    lists the flagged ones in LDS, and then sorts them one per lane, so
-//    that its lanes are busy with flagged bins only (at 256^3, 8 ppc: 0.32
-//    against 0.36 ms per species with one thread per bin).
-constexpr int RB_ORDER_G = 8;
-
+// 2. k_rb_order: one thread per flagged bin sorts the source indices of its
+//    slots (insertion sort: the slots hold a few increasing runs), so that
+//    a bin lists its particles in source order, as the reference's stable
+//    counting sort does (sorting.F90:333-343), whatever order the runs
+//    arrived in.  Bins filled by one run are in order already.  (Listing a
+//    wave's flagged bins among 512 and sorting them one per lane took the
+//    256^3 plasma's pass from 0.36 to 0.32 ms, but the bunch's from 0.7 to
+//    3.2 ms: its dense flagged bins then fell to a few waves.)
 __global__ __launch_bounds__(256) void k_rb_order(const i64 *bin_start, i64 nbins,
                                                   const int *flag, int *perm) {
-  __shared__ int s_list[4][64 * RB_ORDER_G];
-  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const i64 b0 = ((i64)blockIdx.x * 4 + wv) * (64 * RB_ORDER_G);
-  int *list = s_list[wv];
-  int fl[RB_ORDER_G];  // all flags loaded before any is used
-#pragma unroll
-  for (int g = 0; g < RB_ORDER_G; g++) {
-    const i64 b = b0 + 64 * g + lane;
-    fl[g] = b < nbins ? flag[b] : 0;
-  }
-  int cnt = 0;  // the same in every lane of the wave
-#pragma unroll
-  for (int g = 0; g < RB_ORDER_G; g++) {
-    const i64 b = b0 + 64 * g + lane;
-    const bool f = fl[g] != 0;
-    const unsigned long long m = __ballot(f);
-    if (f) list[cnt + __popcll(m & ((1ull << lane) - 1ull))] = (int)b;
-    cnt += __popcll(m);
-  }
-  __builtin_amdgcn_wave_barrier();  // the wave's list is written
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  for (int k = lane; k < cnt; k += 64) order_bin(bin_start, list[k], perm);
+  const i64 b = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbins || !flag[b]) return;
+  order_bin(bin_start, (int)b, perm);
 }
 
 struct GatherArgs {
@@ -983,7 +960,11 @@ struct Tuning {
   int j_lds_bytes = -1;   // cap on the LDS current block (negative: all of lds_bytes)
   int margin_pct = 0;     // motion margin of the current block, in 1/100 cell
   int jpad = 0;           // odd row/plane pitch of the LDS current block
-  int sstride = 0;        // largest lane stride of the deposition (0: whole work item)
+  // largest lane stride S of the deposition (0: a sixty-fourth of the work
+  // item), FP64 and mixed-precision species; -1 (the default) chooses by
+  // shape order (sstride_auto)
+  int sstride = -1;
+  int mx_sstride = -1;
   // LDS current block: 64-bit fixed point by default (3): its integer LDS
   // atomics outrun FP64 ones (at 256^3, 8 ppc the fused launch takes 14.0
   // against 19.0 ms in mixed precision, 18.6 against 22.7 ms in FP64) and
@@ -1005,6 +986,23 @@ struct Tuning {
   int jmargin = -1;
 };
 static Tuning g_tune;
+
+// Lane stride of the deposition by shape order, measured at 256^3, 8 ppc
+// with 8^3-node tiles (launch ms; 0 is a sixty-fourth of the work item):
+//   order 1, direct: S 0 / 16 / 8 / 4 -> 7.05 / 5.61 / 4.94 / 4.65 mixed,
+//                    8.52 / 7.55 / 6.00 / 5.52 FP64;
+//   order 2:         S 0 / 16 / 8 -> 8.62 / 7.57 / 7.14 mixed, 10.17 / 10.20 / 9.35 FP64;
+//   order 3:         S 0 / 8 / 12 / 16 / 20 / 24 -> 12.63 / 13.38 / 12.68 / 12.29 /
+//                    12.62 / 12.58 mixed (whole-tile work items), S 0 / 16 / 32 ->
+//                    17.26 / 18.54 / 17.20 FP64 (work items of about 1,500 particles);
+//                    the bunch and the laser plasma were 1-1.6% slower with 16.
+// Lanes of a wave work on particles S apart inside runs of 64 S, so a
+// smaller S keeps a wave's particles, and the nodes they deposit to, closer.
+static int sstride_auto(int order, bool mx) {
+  if (order == 1) return 4;
+  if (order == 2) return 8;
+  return mx ? 16 : 0;
+}
 // the launch chosen by the last pxrhip_tiles_gather_push_deposit
 static int g_last_wg_per_cu = 0, g_last_lds_bytes = 0;
 }  // namespace pxr
@@ -1209,8 +1207,11 @@ int pxrhip_set_tuning(const char *key, int64_t value) {
     if (value != 0 && value != 3) return set_error(PXRHIP_ERR_ARG, "fused.jacc must be 0 or 3");
     g_tune.fp64_jacc = (int)value;
   } else if (!strcmp(key, "fused.sstride")) {
-    if (value < 0 || value > (1 << 20)) return set_error(PXRHIP_ERR_ARG, "fused.sstride must be in [0, 2^20]");
+    if (value < -1 || value > (1 << 20)) return set_error(PXRHIP_ERR_ARG, "fused.sstride must be in [-1, 2^20]");
     g_tune.sstride = (int)value;
+  } else if (!strcmp(key, "mixed.sstride")) {
+    if (value < -1 || value > (1 << 20)) return set_error(PXRHIP_ERR_ARG, "mixed.sstride must be in [-1, 2^20]");
+    g_tune.mx_sstride = (int)value;
   } else if (!strcmp(key, "fused.jpad")) {
     g_tune.jpad = value != 0;
   } else if (!strcmp(key, "rebin.stable")) {
@@ -1253,6 +1254,7 @@ int pxrhip_get_tuning(const char *key, int64_t *value) {
   else if (!strcmp(key, "leaf.bin_min")) *value = rt().leaf_bin_min;
   else if (!strcmp(key, "leaf.bin_chunk")) *value = rt().leaf_bin_chunk;
   else if (!strcmp(key, "fused.sstride")) *value = g_tune.sstride;
+  else if (!strcmp(key, "mixed.sstride")) *value = g_tune.mx_sstride;
   else if (!strcmp(key, "mixed.jacc")) *value = g_tune.jacc;
   else if (!strcmp(key, "fused.jacc")) *value = g_tune.fp64_jacc;
   else if (!strcmp(key, "mixed.lds_bytes")) *value = g_tune.mx_lds_bytes;
@@ -1474,8 +1476,7 @@ int pxrhip_tiles_rebin_sort(const pxrhip_grid *grid, pxrhip_species *src, pxrhip
                        (const int *)key,
                        (const int64_t *)src->tile_off, tt->ntiles, cursor,
                        (const i64 *)dst->bin_start, flag, perm, cap);
-    hipLaunchKernelGGL(k_rb_order, dim3(grid1d((nbins + RB_ORDER_G - 1) / RB_ORDER_G, 256)),
-                       dim3(256), 0, st,
+    hipLaunchKernelGGL(k_rb_order, dim3(grid1d(nbins, 256)), dim3(256), 0, st,
                        (const i64 *)dst->bin_start, nbins, flag, perm);
     GatherArgs ga;
     for (int k = 0; k < 3; k++) {
@@ -1629,7 +1630,10 @@ static int tiles_fused(const pxrhip_grid *grid, const pxrhip_species *sp, const 
   a.j_bytes = (int)j_bytes;
   a.spread = g_tune.spread;
   a.jpad = g_tune.jpad;
-  a.sstride = g_tune.sstride;
+  {
+    const int ss = mx ? g_tune.mx_sstride : g_tune.sstride;
+    a.sstride = ss >= 0 ? ss : sstride_auto((int)order, mx);
+  }
   a.jacc = jacc;
   a.ovf_count = ctl + 2;
   a.ovf = ovf;
