@@ -893,10 +893,10 @@ __global__ __launch_bounds__(BS) void k_fused_cut(FusedArgs a) {
       }
     }
     __syncthreads();
-    int B0[3], Be[3];
+    int B0[3], B1[3];  // the box, clamped to the arrays (empty when B1 < B0)
     for (int d = 0; d < 3; d++) {
       B0[d] = max(s_box[d], glo[d]);
-      Be[d] = min(s_box[3 + d], ghi[d]) - B0[d] + 1;
+      B1[d] = min(s_box[3 + d], ghi[d]);
     }
     const unsigned long long wbits = s_wabs;
     __syncthreads();  // s_box and s_wabs are read before the next work item resets them
@@ -909,7 +909,7 @@ __global__ __launch_bounds__(BS) void k_fused_cut(FusedArgs a) {
       int l, h;
       deposit_range(a.bg, ulo[d], uhi[d], a.bg.mv[d], l, h);
       lo[d] = max(max(l, glo[d]), B0[d]);
-      hi[d] = min(min(h, ghi[d]), B0[d] + Be[d] - 1);
+      hi[d] = min(min(h, ghi[d]), B1[d]);
       any = any && hi[d] >= lo[d];
     }
     auto fits = [&]() {
@@ -932,7 +932,7 @@ __global__ __launch_bounds__(BS) void k_fused_cut(FusedArgs a) {
           if (fits()) grown = true;
           else lo[d]++;
         }
-        if (hi[d] < B0[d] + Be[d] - 1) {
+        if (hi[d] < B1[d]) {
           hi[d]++;
           if (fits()) grown = true;
           else hi[d]--;
