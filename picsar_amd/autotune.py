@@ -12,7 +12,12 @@ and the launch of the persistent kernel are set by the keys of
   LDS each may use (2 x 79616, 1 x 155648, 3 x 52224 bytes; 0 workgroups:
   the library's automatic choice between the first and the last);
 * ``fused.spread``    -- lanes on particles S apart in the deposition;
+* ``fused.sstride``   -- the largest such S (``mixed.sstride`` for a
+  mixed-precision species; -1: the library's choice by shape order);
 * ``fused.order``     -- the queue hands out the largest work items first.
+
+Every candidate is applied on top of the tuning found on entry, so a key one
+candidate sets does not carry over to the next.
 
 ``autotune`` times the gather + push + deposit stage of a Simulation for
 each candidate with the library's stream timers and keeps the fastest.  The
@@ -37,7 +42,7 @@ from typing import Dict, Iterable, List, Optional
 from . import _lib
 
 KEYS = ("fused.chunk", "fused.wg_per_cu", "fused.lds_bytes", "fused.spread",
-        "mixed.wg_per_cu", "mixed.lds_bytes", "fused.order")
+        "mixed.wg_per_cu", "mixed.lds_bytes", "fused.order", "fused.sstride", "mixed.sstride")
 TIMER_BASE = 65000  # timer slots used here (the library has 65536)
 
 DEFAULT_CANDIDATES: List[Dict[str, int]] = [
@@ -48,7 +53,9 @@ DEFAULT_CANDIDATES: List[Dict[str, int]] = [
 ] + [{"fused.chunk": 4096, "fused.wg_per_cu": 2, "fused.lds_bytes": 79616, "fused.spread": 0},
       {"fused.chunk": 4096, "fused.wg_per_cu": 0, "fused.lds_bytes": 52224, "fused.spread": 1},
       {"fused.chunk": 4096, "fused.wg_per_cu": 0, "fused.lds_bytes": 52224, "fused.spread": 1,
-       "fused.order": 1}]
+       "fused.order": 1}] + [
+    {"fused.chunk": 4096, "fused.wg_per_cu": 0, "fused.lds_bytes": 52224, "fused.spread": 1,
+     "fused.sstride": ss} for ss in (0, 4, 8, 16, 32)]
 
 
 def get_tuning(keys: Iterable[str] = KEYS) -> Dict[str, int]:
@@ -89,7 +96,8 @@ def _for_precision(c: Dict[str, int], precision: str) -> Dict[str, int]:
     """Mixed-precision species are launched with the mixed.* keys."""
     if precision != "mixed":
         return c
-    ren = {"fused.wg_per_cu": "mixed.wg_per_cu", "fused.lds_bytes": "mixed.lds_bytes"}
+    ren = {"fused.wg_per_cu": "mixed.wg_per_cu", "fused.lds_bytes": "mixed.lds_bytes",
+           "fused.sstride": "mixed.sstride"}
     return {ren.get(k, k): v for k, v in c.items()}
 
 
@@ -168,6 +176,7 @@ def autotune(sim, candidates: Optional[List[Dict[str, int]]] = None, steps: int 
     results = []
     try:
         for c in cands:
+            set_tuning(before)
             set_tuning(c)
             _time_stage(sim, 1)
             ms = _time_stage(sim, steps)
@@ -179,6 +188,7 @@ def autotune(sim, candidates: Optional[List[Dict[str, int]]] = None, steps: int 
         set_tuning(before)
         raise
     best_ms, best = min(results, key=lambda r: r[0])
+    set_tuning(before)
     set_tuning(best)
     out = {"best": best, "ms_per_step": best_ms, "cached": False, "signature": sig,
            "trials": [{"tuning": c, "ms_per_step": ms} for ms, c in results]}

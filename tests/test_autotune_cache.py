@@ -65,3 +65,42 @@ def test_cached_result_is_applied_without_trials(tmp_path, monkeypatch):
         assert got["mixed.wg_per_cu"] == 3 and got["mixed.lds_bytes"] == 52224
     finally:
         at.set_tuning(before)
+
+
+def test_trials_start_from_the_entry_tuning(monkeypatch):
+    """Each candidate is applied on top of the tuning found on entry (a key
+    set by one candidate does not carry over to the next), the fastest is
+    applied the same way, and a mixed-precision run sets the lane stride
+    under its mixed.* name.  The trial timer is replaced: no kernel runs."""
+    from picsar_amd import autotune as at
+    try:
+        from picsar_amd import _lib
+        _lib.load()
+    except Exception as e:
+        pytest.skip(f"libpxrhip.so not loadable: {e}")
+    sim = fake_sim(precision="mixed")
+    sim.check = lambda: None
+    before = at.get_tuning()
+    seen = []
+
+    def fake_time(sim_, steps):
+        t = at.get_tuning()
+        seen.append(t)
+        return 1.0 if t["fused.chunk"] == 2048 else 2.0
+
+    monkeypatch.setattr(at, "_time_stage", fake_time)
+    cands = [{"fused.order": 1 - before["fused.order"]}, {"fused.chunk": 2048},
+             {"fused.sstride": 8}]
+    try:
+        res = at.autotune(sim, candidates=cands, cache=False)
+        # two timings per candidate (warm-up and trial)
+        assert len(seen) == 6
+        assert seen[2]["fused.order"] == before["fused.order"]  # no carry-over
+        assert seen[4]["mixed.sstride"] == 8 and seen[4]["fused.sstride"] == before["fused.sstride"]
+        assert res["best"] == {"fused.chunk": 2048}
+        got = at.get_tuning()
+        assert got["fused.chunk"] == 2048
+        assert got["fused.order"] == before["fused.order"]
+        assert got["mixed.sstride"] == before["mixed.sstride"]
+    finally:
+        at.set_tuning(before)
