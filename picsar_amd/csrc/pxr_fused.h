@@ -883,7 +883,7 @@ __global__ __launch_bounds__(BS) void k_fused_cut(FusedArgs a) {
       for (int k = threadIdx.x; k < ch.n; k += BS) gpd_box_mx<N, DEPO>(a, ch.begin + k, dep, wabs);
       wave_minmax(dep);
 #pragma unroll
-      for (int o = 32; o > 0; o >>= 1) wabs = fmax(wabs, __shfl_xor(wabs, o));
+      for (int sh = 32; sh > 0; sh >>= 1) wabs = fmax(wabs, __shfl_xor(wabs, sh));
       if ((threadIdx.x & 63) == 0) {
         for (int d = 0; d < 3; d++) {
           atomicMin(&s_box[d], dep.lo[d]);
