@@ -490,7 +490,8 @@ int pxrhip_tiles_depose_rho(const pxrhip_grid *grid, const pxrhip_species *sp,
  * launch of the last pxrhip_tiles_gather_push_deposit, and
  * "device.num_cus".  Work items whose
  * field or current block exceeds its LDS budget are completed from global
- * memory by a second kernel. */
+ * memory by a second kernel (a mixed-precision species' current: through
+ * an LDS block cut down to fit, see "mixed.jmargin"). */
 int pxrhip_set_tuning(const char *key, int64_t value);
 int pxrhip_get_tuning(const char *key, int64_t *value);
 
