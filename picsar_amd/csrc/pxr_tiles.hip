@@ -980,9 +980,9 @@ struct Tuning {
   // species with a fixed-point current leave around the current block of
   // their particles' old positions (k_fused_cut deposits the work items
   // whose particles outgrow it; other species keep 1): -1 automatic, i.e. 0
-  // with the Esirkepov deposition (at 256^3 the launch gains 1% at 8 ppc and
-  // the bunch's 19%) and 1 with the direct one (order 1 at 256^3: 2% slower
-  // with 0)
+  // with the Esirkepov deposition (at 256^3 the launch gains 1% at 8 ppc, and
+  // 19% for the 1 ppc background of the bunch workload) and 1 with the
+  // direct one (order 1 at 256^3: 2% slower with 0)
   int jmargin = -1;
 };
 static Tuning g_tune;
@@ -995,7 +995,8 @@ static Tuning g_tune;
 //   order 3:         S 0 / 8 / 12 / 16 / 20 / 24 -> 12.63 / 13.38 / 12.68 / 12.29 /
 //                    12.62 / 12.58 mixed (whole-tile work items), S 0 / 16 / 32 ->
 //                    17.26 / 18.54 / 17.20 FP64 (work items of about 1,500 particles);
-//                    the bunch and the laser plasma were 1-1.6% slower with 16.
+//                    the bunch and the laser-plasma workloads were 1-1.6% slower
+//                    with 16.
 // Lanes of a wave work on particles S apart inside runs of 64 S, so a
 // smaller S keeps a wave's particles, and the nodes they deposit to, closer.
 static int sstride_auto(int order, bool mx) {
