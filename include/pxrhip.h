@@ -450,9 +450,10 @@ int pxrhip_tiles_depose_rho(const pxrhip_grid *grid, const pxrhip_species *sp,
  * "fused.lds_bytes" (dynamic LDS per workgroup), "fused.wg_per_cu",
  * "fused.spread" (0/1, lane-spread particle order in the deposition phase),
  * "fused.sstride" (largest lane stride of that order, 0: a sixty-fourth of
- * the work item; -1, the default, chooses by shape order: 4 at order 1, 8 at
- * order 2, 0 at order 3) and "mixed.sstride" (the same for a mixed-precision
- * species: 4, 8 and 16 by default),
+ * the work item; -1, the default, sets one per work item from its particles
+ * per sorting cell and the shape order, none at order 3) and
+ * "mixed.sstride" (the same for a mixed-precision species, whose order-3
+ * work items are capped too),
  * "fused.chunk" (default particles per work item; a work item also holds no
  * more sorting-grid rows than its LDS blocks allow),
  * "fused.j_lds_bytes" (largest current block accumulated in LDS; negative:

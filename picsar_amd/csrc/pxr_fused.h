@@ -94,6 +94,8 @@ struct Chunk {
   int tile;
   int z, y0, y1;  // sorting-grid planes z..z1 and rows y0..y1 of the particles (several
   int z1;         // planes: whole planes); z < 0: unknown
+  int s;          // largest lane stride of the deposition (0: none), from its density
+  int pad;
 };
 
 struct FusedArgs {
@@ -742,7 +744,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(
     {
       // lanes take particles S apart inside super-blocks of 64*S particles
       int S = (ch.n + 63) >> 6;
-      if (a.sstride > 0 && a.sstride < S) S = a.sstride;
+      const int cap = a.sstride > 0 ? a.sstride : ch.s;
+      if (cap > 0 && cap < S) S = cap;
       const int SB = 64 * S;
       const int nkk = ((ch.n + SB - 1) / SB) * SB;
       auto pick = [&](int kk) {

@@ -732,6 +732,7 @@ struct ChunkArgs {
   i64 maxchunks;    // bound on the chunks of a launch; beyond it ...
   unsigned long long *bad;  // ... chunks would be dropped, and are counted here
   int jroom;                // motion room of the current block, nodes per side
+  int sk4;                  // 4 x lane stride per particle per sorting cell (0: no cap)
 };
 
 // plane roles in ChunkArgs::group (a value >= 0 is the last plane of the
@@ -819,6 +820,17 @@ PXR_DEV int walk_unit(const ChunkArgs &a, int t, int z, i64 pos) {
       c.z1 = zz1;
       c.y0 = y0;
       c.y1 = y1;
+      c.s = 0;
+      c.pad = 0;
+      if (a.sk4 > 0 && zz >= 0) {
+        // lanes a few sorting cells apart: sk4 / 4 times the particles per
+        // cell, to the nearest power of two (on a log scale)
+        const i64 cells = (i64)a.n3[3 * t] * (y1 - y0 + 1) * (zz1 - zz + 1);
+        const double want = (double)a.sk4 * (double)n / (4.0 * (double)cells);
+        int p2 = 1;
+        while (p2 < (1 << 20) && want > 1.4142135623730951 * p2) p2 <<= 1;
+        c.s = p2;
+      }
       a.out[pos + nch] = c;
     }
     nch++;
@@ -961,8 +973,8 @@ struct Tuning {
   int margin_pct = 0;     // motion margin of the current block, in 1/100 cell
   int jpad = 0;           // odd row/plane pitch of the LDS current block
   // largest lane stride S of the deposition (0: a sixty-fourth of the work
-  // item), FP64 and mixed-precision species; -1 (the default) chooses by
-  // shape order (sstride_auto)
+  // item), FP64 and mixed-precision species; -1 (the default) sets one per
+  // work item (sstride_k4)
   int sstride = -1;
   int mx_sstride = -1;
   // LDS current block: 64-bit fixed point by default (3): its integer LDS
@@ -987,22 +999,27 @@ struct Tuning {
 };
 static Tuning g_tune;
 
-// Lane stride of the deposition by shape order, measured at 256^3, 8 ppc
-// with 8^3-node tiles (launch ms; 0 is a sixty-fourth of the work item):
+// Lane stride of the deposition: lanes of a wave work on particles S apart
+// inside runs of 64 S.  Measured at 256^3, 8 ppc with 8^3-node tiles
+// (launch ms; 0 is a sixty-fourth of the work item):
 //   order 1, direct: S 0 / 16 / 8 / 4 -> 7.05 / 5.61 / 4.94 / 4.65 mixed,
 //                    8.52 / 7.55 / 6.00 / 5.52 FP64;
-//   order 2:         S 0 / 16 / 8 -> 8.62 / 7.57 / 7.14 mixed, 10.17 / 10.20 / 9.35 FP64;
+//   order 2:         S 0 / 16 / 8 / 4 -> 8.62 / 7.57 / 7.14 / 7.51 mixed,
+//                    10.17 / 10.20 / 9.35 FP64;
 //   order 3:         S 0 / 8 / 12 / 16 / 20 / 24 -> 12.63 / 13.38 / 12.68 / 12.29 /
 //                    12.62 / 12.58 mixed (whole-tile work items), S 0 / 16 / 32 ->
 //                    17.26 / 18.54 / 17.20 FP64 (work items of about 1,500 particles);
-//                    the bunch and the laser-plasma workloads were 1-1.6% slower
-//                    with 16.
-// Lanes of a wave work on particles S apart inside runs of 64 S, so a
-// smaller S keeps a wave's particles, and the nodes they deposit to, closer.
-static int sstride_auto(int order, bool mx) {
-  if (order == 1) return 4;
-  if (order == 2) return 8;
-  return mx ? 16 : 0;
+// the 4 ppc laser plasma chose 8 and the dense bunch 0.  The best S follows
+// the particles per sorting cell of the work item: about 0.75, 1.5 and 3
+// times it at orders 1, 2 and 3 (a wave's particles, and the nodes they
+// deposit to, stay close while lanes sit in different cells), taken to the
+// nearest power of two (17 in place of 16 at 256^3 cost 1.7%).  The
+// work-item builder sets each item's S from its count and cells; returned
+// here is 4 x that factor (0: no cap, FP64 at order 3).
+static int sstride_k4(int order, bool mx) {
+  if (order == 1) return 3;
+  if (order == 2) return 6;
+  return mx ? 12 : 0;
 }
 // the launch chosen by the last pxrhip_tiles_gather_push_deposit
 static int g_last_wg_per_cu = 0, g_last_lds_bytes = 0;
@@ -1111,7 +1128,7 @@ __global__ __launch_bounds__(256) void k_rho_tiles(RhoTileArgs a) {
 static int build_chunks(const TileTable *tt, const pxrhip_species *sp, const BoxGeom &bg, int ch,
                         i64 lds_bytes, i64 j_bytes, int fnode, int jnode, hipStream_t st,
                         Chunk **chunks_out, int **ctl_out, int2 **ovf_out, i64 *maxchunks_out,
-                        int jroom = 1) {
+                        int jroom = 1, int sk4 = 0) {
   const i64 cap = sp->capacity;
   const int nzmax = tt->nzmax;
   const i64 nunits = (i64)tt->ntiles * nzmax;
@@ -1152,6 +1169,7 @@ static int build_chunks(const TileTable *tt, const pxrhip_species *sp, const Box
   ca.maxchunks = maxchunks;
   ca.bad = rt().d_bad;
   ca.jroom = jroom;
+  ca.sk4 = sk4;
   hipLaunchKernelGGL(k_chunk_groups, dim3(grid1d(tt->ntiles, 64)), dim3(64), 0, st, ca);
   hipLaunchKernelGGL(k_chunk_count, dim3(grid1d(nunits, 256)), dim3(256), 0, st, ca);
   hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nblk), dim3(256), 0, st, cnt, nunits, bsum);
@@ -1588,9 +1606,13 @@ static int tiles_fused(const pxrhip_grid *grid, const pxrhip_species *sp, const 
   // a mixed-precision species with a fixed-point current: work items whose
   // current outgrows its LDS block are deposited by k_fused_cut
   const bool defer = mx && jacc == 3 && (phase & PHASE_DEPOSIT);
+  // lane stride of the deposition: a tuning value, else one per work item
+  const int sstride_set = mx ? g_tune.mx_sstride : g_tune.sstride;
+  const int sk4 = sstride_set >= 0 ? 0 : sstride_k4((int)order, mx);
   if ((r = build_chunks(tt, sp, bg, ch, lds_bytes, j_bytes, fnode, jnode_b, st, &chunks, &ctl,
                         &ovf, &maxchunks,
-                        !defer ? 1 : g_tune.jmargin >= 0 ? g_tune.jmargin : (depo_algo ? 1 : 0))))
+                        !defer ? 1 : g_tune.jmargin >= 0 ? g_tune.jmargin : (depo_algo ? 1 : 0),
+                        sk4)))
     return r;
   // fused kernel
   FusedArgs a;
@@ -1631,10 +1653,7 @@ static int tiles_fused(const pxrhip_grid *grid, const pxrhip_species *sp, const 
   a.j_bytes = (int)j_bytes;
   a.spread = g_tune.spread;
   a.jpad = g_tune.jpad;
-  {
-    const int ss = mx ? g_tune.mx_sstride : g_tune.sstride;
-    a.sstride = ss >= 0 ? ss : sstride_auto((int)order, mx);
-  }
+  a.sstride = sstride_set > 0 ? sstride_set : 0;  // else per work item (Chunk::s)
   a.jacc = jacc;
   a.ovf_count = ctl + 2;
   a.ovf = ovf;
